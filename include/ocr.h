@@ -1,0 +1,221 @@
+/*
+ * ocr.h — C ABI of the MI355X-native rasterisation + deferred-shading path.
+ *
+ * This is the drop-in boundary for OpenCLRenderer's draw path.  Every entry
+ * point takes plain pointers and sizes; device work is ordered on a caller
+ * supplied HIP stream passed as `void*` (a hipStream_t, NULL = the legacy
+ * default stream).  All entry points return OCR_OK (0) or a negative status;
+ * ocr_last_error() returns a thread-local message for the last failure.
+ *
+ * Record layouts below are byte-compatible with the reference's host structs,
+ * so buffers the reference's object_context / light code build can be handed
+ * over unchanged.
+ */
+#ifndef OCR_H_INCLUDED
+#define OCR_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define OCR_OK               0
+#define OCR_ERR_INVALID     -1  /* bad handle, size or argument            */
+#define OCR_ERR_HIP         -2  /* a HIP runtime call failed               */
+#define OCR_ERR_OOM         -3  /* device allocation failed                */
+#define OCR_ERR_OVERFLOW    -4  /* a work list exceeded its capacity; the
+                                   frame was grown and must be re-rendered */
+#define OCR_ERR_UNSUPPORTED -5  /* input outside what the path supports     */
+#define OCR_ERR_BUSY        -6  /* admission refused (serving)              */
+
+/* ---- fixed constants of the path (cl2.cl:17, :21-23, :4247-4249) ------- */
+#define OCR_DEPTH_FAR        350000.0f
+#define OCR_DEPTH_ICUTOFF    20
+#define OCR_OP_SIZE          500
+#define OCR_OP_SIZE_LIGHT    300
+#define OCR_BUF_ERROR        20
+#define OCR_SHADOWBIAS       50
+#define OCR_MIP_LEVELS       4
+#define OCR_ATLAS_DIM        2048
+#define OCR_NO_TRIANGLE      0xFFFFFFFFu
+
+/* object feature flags (cl2.cl:90-97) */
+#define OCR_FEATURE_SS_REFLECTIVE                    1
+#define OCR_FEATURE_TWO_SIDED                        2
+#define OCR_FEATURE_OUTLINE                          4
+#define OCR_FEATURE_IS_STATIC                        8
+#define OCR_FEATURE_DOES_NOT_RECEIVE_DYNAMIC_SHADOWS 16
+
+/* ---- records ----------------------------------------------------------- */
+
+/* struct vertex, cl2.cl:131-138 / vertex.hpp:8-38 (48 bytes).  object_id is
+ * only meaningful in vertices[0] (object_context.cpp:421-446, cl2.cl:4240);
+ * vertex_col == 0 in vertices[0] means "use the texture" (cl2.cl:5933). */
+typedef struct ocr_vertex {
+    float    pos[4];
+    float    normal[4];
+    float    vt[2];
+    uint32_t object_id;
+    uint32_t vertex_col;        /* 0xRRGGBBAA */
+} ocr_vertex;
+
+/* struct triangle, cl2.cl:148-151 / triangle.hpp:8-15 (144 bytes).  The
+ * triangle id of the id buffer is the index into the array handed to
+ * ocr_scene_create (object_context.cpp:398-453 concatenation order). */
+typedef struct ocr_triangle {
+    ocr_vertex vertices[3];
+} ocr_triangle;
+
+/* struct obj_g_descriptor, cl2.cl:99-124 / obj_g_descriptor.hpp:9-36
+ * (136 bytes of fields, 144 with cl_float4 alignment). */
+typedef struct ocr_obj_desc {
+    float    world_pos[4];
+    float    world_rot_quat[4];
+    float    old_world_pos_1[4];
+    float    old_world_pos_2[4];
+    float    old_world_rot_quat_1[4];
+    float    old_world_rot_quat_2[4];
+    float    scale;
+    uint32_t tid;               /* texture position id (texture_context::get_gpu_position_id) */
+    uint32_t rid;
+    uint32_t ssid;
+    uint32_t has_bump;
+    float    specular;
+    float    spec_mult;
+    float    diffuse;
+    int32_t  buffer_offset;
+    int32_t  feature_flag;
+    uint32_t _pad[2];
+} ocr_obj_desc;
+
+/* struct light, cl2.cl:77-87 / light.hpp:25-69 (56 bytes, 64 aligned). */
+typedef struct ocr_light {
+    float    pos[4];
+    float    col[4];
+    uint32_t shadow;
+    float    brightness;
+    float    radius;
+    float    diffuse;
+    float    godray_intensity;
+    int32_t  is_static;
+    uint32_t _pad[2];
+} ocr_light;
+
+/* One view.  Replaces the compile-time -D SCREENWIDTH/SCREENHEIGHT/FOV_CONST
+ * options (ocl.h:227-240) and the c_pos/c_rot kernel arguments
+ * (engine.cpp:1868-1877) with runtime values, so views of different sizes can
+ * share one launch.  c_rot is extrinsic-XYZ Euler radians (cl2.cl:220-236). */
+typedef struct ocr_view {
+    uint32_t width;
+    uint32_t height;
+    float    fov_const;         /* (width/2)/tan(hfov/2), engine.cpp:119-133 */
+    uint32_t use_linear_rendering;  /* object_context.hpp:87, TEST_LINEAR path */
+    float    c_pos[4];
+    float    c_rot[4];
+    float    clear_col[4];      /* object_context::set_clear_colour */
+} ocr_view;
+
+/* A source texture: tightly packed RGBA8 rows, row 0 first (sf::Image order). */
+typedef struct ocr_tex_src {
+    uint32_t       width;
+    uint32_t       height;
+    const uint8_t* rgba;
+} ocr_tex_src;
+
+/* Counters of the last render of a frame, and per-pass device times when
+ * profiling is on (hipEvent pairs on the render stream). */
+typedef struct ocr_frame_stats {
+    uint32_t n_cut;             /* cut (post clip/cull) triangles            */
+    uint32_t n_frag;            /* fragment records                          */
+    uint32_t overflow;          /* 1 if a work list hit its capacity         */
+    uint32_t _pad;
+    float    ms_clear;
+    float    ms_prearrange;
+    float    ms_depth;
+    float    ms_id;
+    float    ms_shade;
+    float    ms_total;
+} ocr_frame_stats;
+
+typedef struct ocr_ctx   ocr_ctx;
+typedef struct ocr_scene ocr_scene;
+typedef struct ocr_frame ocr_frame;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Replaces oclstuff()/build() (ocl.h:328-537): selects HIP device `device`. */
+int  ocr_create(int device, ocr_ctx** out);
+void ocr_destroy(ocr_ctx* ctx);
+/* Thread-local description of the last non-OK status. */
+const char* ocr_last_error(void);
+/* Free / total device memory, the serving admission input. */
+int  ocr_query_free_hbm(ocr_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+
+/* ---- scene: device-resident geometry, descriptors, atlas, lights -------- */
+
+/* object_context::build (object_context.cpp:646) + alloc_gpu (:346) +
+ * alloc_object_descriptors (:460): uploads `ntri` triangles and `nobj`
+ * descriptors.  Host buffers may be freed on return. */
+int  ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri,
+                      const ocr_obj_desc* objs, uint32_t nobj, ocr_scene** out);
+/* object::g_flush (object.cpp:652-857): rewrite descriptors [first, first+n). */
+int  ocr_scene_update_objs(ocr_scene* s, uint32_t first, uint32_t n,
+                           const ocr_obj_desc* objs);
+/* texture_context::alloc_gpu (texture_context.cpp:350-517) +
+ * texture::update_me_to_gpu / update_gpu_mipmaps (texture.cpp:323-493):
+ * packs the textures into 2048x2048 RGBA8 pages per power-of-two size class,
+ * uploads them (update_gpu_tex, cl2.cl:923-953) and builds 4 mip levels
+ * (generate_mips / generate_mip_mips, cl2.cl:1070-1189).  Texture i of the
+ * array is descriptor tid == i. */
+int  ocr_atlas_upload(ocr_scene* s, const ocr_tex_src* texs, uint32_t ntex);
+/* light::build (light.cpp:145-276): uploads the light list and sizes the
+ * cube shadow maps (n_shadow x 6 x dim^2 u32, dim = LIGHTBUFFERDIM). */
+int  ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n,
+                       uint32_t light_buffer_dim);
+/* engine::generate_realtime_shadowing (engine.cpp:1601-1790): clears and
+ * renders every shadow-casting light's cube map from dynamic geometry, and
+ * the static-light maps from static geometry when lights were re-uploaded. */
+int  ocr_shadow_pass(ocr_scene* s, void* stream);
+void ocr_scene_destroy(ocr_scene* s);
+
+/* Inspection of device state (tests, tools). */
+int  ocr_scene_atlas_info(ocr_scene* s, uint32_t* npages, uint32_t* nnums,
+                          uint32_t* nsizes, uint32_t* mip_start);
+int  ocr_scene_read_atlas(ocr_scene* s, uint8_t* pages, uint32_t* nums,
+                          uint32_t* sizes);
+int  ocr_scene_read_shadow_maps(ocr_scene* s, uint32_t* dynamic_maps,
+                                uint32_t* static_maps);
+
+/* ---- frames: offscreen framebuffer (replaces cl_gl_interop_texture) ----- */
+
+int  ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** out);
+void ocr_frame_destroy(ocr_frame* f);
+/* Record per-pass hipEvent timings on every render of this frame. */
+int  ocr_frame_set_profiling(ocr_frame* f, int enable);
+
+/* engine::draw_bulk_objs_n -> render_tris (engine.cpp:1794-2161):
+ * prearrange -> depth (kernel1) -> id (kernel2) -> shade (kernel3), all on
+ * `stream`, with no host round trip.  The view size must equal the frame's. */
+int  ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream);
+
+/* Replaces the CL/GL blit and its host read-back path
+ * (cl_gl_interop_texture.hpp:189-252, :312-331).  Synchronises `stream`,
+ * then copies any non-NULL output: depth u32 (UINT_MAX = background),
+ * triangle id u32 (OCR_NO_TRIANGLE where nothing is visible), RGBA8 and the
+ * encoded screen normals (ushort2 per pixel, cl2.cl:5617-5630).  Returns
+ * OCR_ERR_OVERFLOW if the last render overflowed a work list. */
+int  ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id,
+                    uint8_t* rgba8, uint16_t* normals, void* stream);
+/* Device pointers of the frame's buffers (zero-copy consumers). */
+int  ocr_frame_device_buffers(ocr_frame* f, void** depth, void** idkey,
+                              void** rgba8, void** normals);
+int  ocr_frame_get_stats(ocr_frame* f, void* stream, ocr_frame_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCR_H_INCLUDED */

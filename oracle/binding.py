@@ -1,0 +1,202 @@
+"""ctypes binding of the oracle (oracle/_build/libocr_oracle.so).
+
+Test infrastructure only: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libocr_oracle.so")
+
+
+class OracleScene(C.Structure):
+    _fields_ = [
+        ("tris", C.c_void_p), ("ntri", C.c_uint32),
+        ("objs", C.c_void_p), ("nobj", C.c_uint32),
+        ("lights", C.c_void_p), ("nlights", C.c_uint32),
+        ("atlas", C.c_void_p), ("npages", C.c_uint32),
+        ("nums", C.c_void_p), ("nnums", C.c_uint32),
+        ("sizes", C.c_void_p), ("nsizes", C.c_uint32),
+        ("mip_start", C.c_uint32),
+        ("shadow_maps", C.c_void_p),
+        ("static_maps", C.c_void_p),
+        ("lbuf_dim", C.c_uint32),
+    ]
+
+
+class OracleOut(C.Structure):
+    _fields_ = [
+        ("depth", C.c_void_p), ("tri_id", C.c_void_p), ("rgba8", C.c_void_p),
+        ("normals", C.c_void_p),
+        ("n_cut", C.c_uint32), ("n_frag", C.c_uint32), ("n_samples", C.c_uint64),
+        ("n_visible_tris", C.c_uint32), ("n_visible_objs", C.c_uint32),
+        ("ms_prearrange", C.c_double), ("ms_depth", C.c_double), ("ms_id", C.c_double),
+        ("ms_shade", C.c_double),
+    ]
+
+
+class SinCos(C.Structure):
+    _fields_ = [("s", C.c_float * 3), ("c", C.c_float * 3)]
+
+
+class TexSrc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.c_void_p)]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def camera_sincos(c_rot):
+    """sin/cos of the camera Euler angles, binary64 rounded once to binary32."""
+    sc = SinCos()
+    for i in range(3):
+        a = float(np.float32(c_rot[i]))
+        sc.s[i] = float(np.float32(math.sin(a)))
+        sc.c[i] = float(np.float32(math.cos(a)))
+    return sc
+
+
+class Oracle:
+    def __init__(self, path=LIB_PATH):
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run oracle/Makefile or build())")
+        self.lib = L = C.CDLL(path)
+        L.oracle_render.argtypes = [C.POINTER(OracleScene), C.c_void_p, C.POINTER(SinCos), C.c_int,
+                                    C.POINTER(OracleOut)]
+        L.oracle_shadow_maps.argtypes = [C.POINTER(OracleScene), C.c_int, C.c_void_p, C.c_void_p]
+        L.oracle_atlas_plan.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
+                                        C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]
+        L.oracle_atlas_fill.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                        C.c_void_p]
+        L.oracle_face_sincos.argtypes = [C.c_void_p]
+        f, u, i = C.c_float, C.c_uint32, C.c_int
+        fp = C.POINTER(C.c_float)
+        L.oracle_rot.argtypes = [fp, fp, C.POINTER(SinCos), fp]
+        L.oracle_back_rot.argtypes = [fp, C.POINTER(SinCos), fp]
+        L.oracle_rot_quat.argtypes = [fp, fp, fp]
+        L.oracle_back_rot_quat.argtypes = [fp, fp, fp]
+        L.oracle_clip.argtypes = [fp, fp]
+        L.oracle_clip.restype = i
+        L.oracle_calc_min_max.argtypes = [fp, f, f, fp]
+        L.oracle_point_in_tri.argtypes = [f, f, fp, fp]
+        L.oracle_point_in_tri.restype = i
+        L.oracle_walk.argtypes = [fp, f, f, u, i, C.c_void_p, C.c_void_p, C.c_void_p, i]
+        L.oracle_walk.restype = i
+        for name in ("oracle_wang_hash", "oracle_xorshift"):
+            getattr(L, name).argtypes = [u]
+            getattr(L, name).restype = u
+        for name in ("oracle_rational_acos", "oracle_log2_approx", "oracle_exp", "oracle_tan",
+                     "oracle_powr5", "oracle_pow4"):
+            getattr(L, name).argtypes = [f]
+            getattr(L, name).restype = f
+        L.oracle_gamma_transform.argtypes = [fp, fp]
+        L.oracle_gamma_inverse.argtypes = [fp, fp]
+        L.oracle_ret_cubeface.argtypes = [fp, fp]
+        L.oracle_ret_cubeface.restype = i
+        L.oracle_get_barycentric.argtypes = [fp, fp, fp, fp, fp, fp, fp]
+        L.oracle_encode_normal.argtypes = [fp]
+        L.oracle_encode_normal.restype = u
+        L.oracle_texture_mod.argtypes = [fp, fp]
+
+    # --------------------------------------------------------------- helpers
+    @staticmethod
+    def _f(a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def call_vec(self, name, *arrays, out_len=3):
+        keep, ptrs = [], []
+        for a in arrays:
+            arr, p = self._f(a)
+            keep.append(arr)
+            ptrs.append(p)
+        out, po = self._f(np.zeros(out_len, dtype=np.float32))
+        getattr(self.lib, name)(*ptrs, po)
+        return out
+
+    def face_sincos(self):
+        arr = (SinCos * 6)()
+        self.lib.oracle_face_sincos(C.cast(arr, C.c_void_p))
+        return arr
+
+    # ----------------------------------------------------------------- atlas
+    def atlas(self, textures):
+        n = len(textures)
+        if n == 0:
+            return (np.zeros((1, 2048, 2048, 4), np.uint8), np.zeros(2, np.uint32),
+                    np.full(2, 16, np.uint32), 0)
+        w = np.array([t.width for t in textures], dtype=np.uint32)
+        h = np.array([t.height for t in textures], dtype=np.uint32)
+        nums = np.zeros(n * 5, dtype=np.uint32)
+        sizes = np.zeros(n * 5, dtype=np.uint32)
+        npages = C.c_uint32(0)
+        mip_start = C.c_uint32(0)
+        rc = self.lib.oracle_atlas_plan(_ptr(w), _ptr(h), n, C.byref(npages), _ptr(nums),
+                                        _ptr(sizes), C.byref(mip_start))
+        if rc != 0:
+            raise ValueError(f"oracle_atlas_plan failed ({rc})")
+        pages = np.zeros((npages.value, 2048, 2048, 4), dtype=np.uint8)
+        srcs = (TexSrc * n)()
+        keep = []
+        for i, t in enumerate(textures):
+            rgba = np.ascontiguousarray(t.rgba, dtype=np.uint8)
+            keep.append(rgba)
+            srcs[i].width, srcs[i].height, srcs[i].rgba = t.width, t.height, rgba.ctypes.data
+        self.lib.oracle_atlas_fill(C.cast(srcs, C.c_void_p), n, _ptr(nums), _ptr(sizes),
+                                   mip_start.value, _ptr(pages))
+        return pages, nums, sizes[:npages.value].copy(), mip_start.value
+
+    # ----------------------------------------------------------------- scene
+    def _scene(self, scene, atlas, maps):
+        pages, nums, sizes, mip_start = atlas
+        dyn, stat = maps if maps is not None else (None, None)
+        s = OracleScene()
+        s.tris, s.ntri = _ptr(scene.tris), scene.tris.shape[0]
+        s.objs, s.nobj = _ptr(scene.objs), scene.objs.shape[0]
+        s.lights, s.nlights = _ptr(scene.lights), scene.lights.shape[0]
+        s.atlas, s.npages = _ptr(pages), pages.shape[0]
+        s.nums, s.nnums = _ptr(nums), nums.shape[0]
+        s.sizes, s.nsizes = _ptr(sizes), sizes.shape[0]
+        s.mip_start = mip_start
+        s.shadow_maps = _ptr(dyn)
+        s.static_maps = _ptr(stat)
+        s.lbuf_dim = scene.light_buffer_dim
+        return s
+
+    def shadow_maps(self, scene, nthreads=1):
+        L = scene.light_buffer_dim
+        nsh = int(np.sum(scene.lights["shadow"] == 1))
+        nst = int(np.sum((scene.lights["shadow"] != 0) & (scene.lights["is_static"] != 0)))
+        dyn = np.zeros((max(nsh, 1), 6, L, L), dtype=np.uint32)
+        stat = np.zeros((max(nst, 1), 6, L, L), dtype=np.uint32)
+        dummy_atlas = (np.zeros((1, 1, 1, 4), np.uint8), np.zeros(1, np.uint32),
+                       np.zeros(1, np.uint32), 0)
+        s = self._scene(scene, dummy_atlas, None)
+        self.lib.oracle_shadow_maps(C.byref(s), nthreads, _ptr(dyn), _ptr(stat))
+        return dyn, stat
+
+    def render(self, scene, view, atlas=None, maps=None, nthreads=1, shade=True):
+        if atlas is None:
+            atlas = self.atlas(scene.textures)
+        if maps is None and np.any(scene.lights["shadow"] != 0):
+            maps = self.shadow_maps(scene, nthreads)
+        v = np.ascontiguousarray(view)
+        W, H = int(v["width"][0]), int(v["height"][0])
+        depth = np.zeros((H, W), np.uint32)
+        tri = np.zeros((H, W), np.uint32)
+        rgba = np.zeros((H, W, 4), np.uint8) if shade else None
+        nrm = np.zeros((H, W, 2), np.uint16) if shade else None
+        out = OracleOut()
+        out.depth, out.tri_id, out.rgba8, out.normals = _ptr(depth), _ptr(tri), _ptr(rgba), _ptr(nrm)
+        s = self._scene(scene, atlas, maps)
+        cam = camera_sincos(v["c_rot"][0])
+        rc = self.lib.oracle_render(C.byref(s), _ptr(v), C.byref(cam), nthreads, C.byref(out))
+        if rc != 0:
+            raise RuntimeError(f"oracle_render failed ({rc})")
+        stats = {k: getattr(out, k) for k, _ in OracleOut._fields_[4:]}
+        return {"depth": depth, "tri_id": tri, "rgba": rgba, "normals": nrm, "stats": stats}
