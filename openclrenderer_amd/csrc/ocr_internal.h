@@ -1,0 +1,160 @@
+// ocr_internal.h — device records, handles and launch helpers shared by the
+// runtime and the kernels.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ocr.h"
+#include "ocr_math.h"
+
+namespace ocr {
+
+// Triangle positions + owning object: the 40 bytes prearrange and the raster
+// passes need (vertex_pos + vertices[0].object_id, cl2.cl:4300-4304).
+struct alignas(8) TriPos {
+  float p[9];
+  uint32_t oid;
+};
+static_assert(sizeof(TriPos) == 40, "TriPos layout");
+
+// Shading attributes of a triangle (normals, vt, vertex colours), read only
+// for visible triangles by the shading pass (cl2.cl:5872-5938).
+struct TriAttr {
+  float n[9];
+  float vt[6];
+  uint32_t col[3];
+};
+static_assert(sizeof(TriAttr) == 72, "TriAttr layout");
+
+// The obj_g_descriptor fields the path reads (52 bytes), padded to 64.
+struct alignas(16) ObjDesc {
+  float wp[3];
+  float scale;
+  float q[4];
+  uint32_t tid;
+  float specular;
+  float spec_mult;
+  float diffuse;
+  int32_t flags;
+  uint32_t _pad[3];
+};
+static_assert(sizeof(ObjDesc) == 64, "ObjDesc layout");
+
+// A clipped, culled, projected triangle: screen x, y and camera z of its
+// three vertices, its 1/det constant and its sort key (2*tri + sub for a
+// camera view, cube face for a shadow map).
+struct alignas(16) CutTri {
+  float x[3];
+  float y[3];
+  float z[3];
+  float rconst;
+  uint32_t key;
+  uint32_t oid;
+};
+static_assert(sizeof(CutTri) == 48, "CutTri layout");
+
+// Everything a kernel needs to know about one view.
+struct ViewParams {
+  uint32_t width, height;
+  float fov;
+  uint32_t linear;
+  float cpos[3];
+  float _pad0;
+  SinCos cam;
+  float clear[4];
+};
+
+// Device counters of one frame's work lists.
+enum Counter : int {
+  kCutCount = 0,
+  kFragCount = 1,
+  kOverflow = 2,
+  kCounterWords = 8,
+};
+
+struct Ctx {
+  int device = 0;
+  hipDeviceProp_t prop{};
+  SinCos face_sc[6];
+};
+
+struct Atlas {
+  uint8_t* pages = nullptr;   // npages x 2048 x 2048 x RGBA8
+  uint32_t* nums = nullptr;
+  uint32_t* sizes = nullptr;
+  uint32_t npages = 0, nnums = 0, nsizes = 0, mip_start = 0;
+};
+
+struct Scene {
+  Ctx* ctx = nullptr;
+  uint32_t ntri = 0, nobj = 0;
+  uint32_t max_tid = 0;              // largest descriptor texture id (validated at render)
+  TriPos* tri_pos = nullptr;
+  TriAttr* tri_attr = nullptr;
+  ObjDesc* objs = nullptr;
+  Atlas atlas;
+  ocr_light* lights = nullptr;
+  uint32_t nlights = 0, n_shadow = 0, n_static = 0, lbuf_dim = 1024;
+  std::vector<ocr_light> host_lights;
+  uint32_t* shadow_maps = nullptr;   // n_shadow x 6 x dim^2
+  uint32_t* static_maps = nullptr;   // n_static x 6 x dim^2
+  bool static_dirty = false;
+  // work lists of the shadow passes
+  CutTri* sh_cuts = nullptr;
+  uint2* sh_frags = nullptr;
+  uint32_t* sh_counters = nullptr;
+  uint32_t sh_cap_cut = 0, sh_cap_frag = 0;
+  hipEvent_t ev_sh0 = nullptr, ev_sh1 = nullptr;
+  float ms_shadow = 0.f;
+};
+
+struct Frame {
+  Ctx* ctx = nullptr;
+  uint32_t width = 0, height = 0;
+  uint32_t* depth = nullptr;
+  unsigned long long* idkey = nullptr;
+  uint32_t* rgba = nullptr;
+  uint32_t* normals = nullptr;
+  CutTri* cuts = nullptr;
+  uint2* frags = nullptr;
+  uint32_t* counters = nullptr;
+  uint32_t cap_cut = 0, cap_frag = 0;
+  uint32_t want_frag = 0;            // capacity asked for after an overflow
+  bool profiling = false;
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool rendered = false;
+};
+
+// ---- error plumbing -------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+#define OCR_HIP_CHECK(expr)                                                     \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      return ::ocr::fail(OCR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- kernel launchers (defined next to their kernels) --------------------
+hipError_t launch_clear_frame(Frame* f, hipStream_t s);
+hipError_t launch_prearrange(const Scene* sc, const ViewParams& v, Frame* f, hipStream_t s);
+hipError_t launch_depth(const ViewParams& v, Frame* f, hipStream_t s);
+hipError_t launch_id(const ViewParams& v, Frame* f, hipStream_t s);
+hipError_t launch_shade(const Scene* sc, const ViewParams& v, Frame* f, hipStream_t s);
+hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, uint32_t* map,
+                               hipStream_t s);
+hipError_t launch_atlas_upload(const Atlas& a, uint32_t tid, const uint8_t* src, uint32_t w,
+                               uint32_t h, hipStream_t s);
+hipError_t launch_mip(const Atlas& a, uint32_t src_id, uint32_t dst_id, uint32_t gw, uint32_t gh,
+                      hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
+
+constexpr int kWave = 64;
+
+}  // namespace ocr

@@ -1,0 +1,339 @@
+// ocr_math.h — device arithmetic of the raster + shading path (gfx950).
+//
+// The reference kernels are built with -cl-fast-relaxed-math and FP_CONTRACT
+// ON (ocl.h:227, cl2.cl:3), which leaves rounding implementation defined.
+// This library fixes it so that the depth and triangle-id buffers are
+// reproducible bit for bit on any IEEE host:
+//   * binary32 round-to-nearest-even, no contraction (every file is built
+//     with -ffp-contract=off and says so with the pragma below); the
+//     reference's mad(a,b,c) is an explicit fmaf;
+//   * division and sqrt correctly rounded (-fhip-fp32-correctly-rounded-
+//     divide-sqrt);
+//   * fast_normalize(v) = v * (1/sqrt(dot(v,v))), dot summed left to right;
+//   * pow(x,2) = x*x; pow(x,4) and powr(x,5) are built in binary64 from the
+//     exact square and rounded once; exp and tan are evaluated in binary64
+//     and rounded once;
+//   * float -> u32 conversions saturate, NaN -> 0.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ocr {
+
+constexpr float kDepthFar = 350000.0f;       // cl2.cl:17
+constexpr float kMulint = 4294967296.0f;     // (float)UINT_MAX, cl2.cl:19
+constexpr int kIcutoff = 20;                 // depth_icutoff, cl2.cl:21-23
+constexpr int kOpSize = 500;                 // cl2.cl:4247
+constexpr int kOpSizeLight = 300;            // cl2.cl:4249
+constexpr uint32_t kBufError = 20u;          // cl2.cl:5387
+constexpr float kPi = 3.1415927f;            // cl2.cl:14
+constexpr float kAmbient = 0.2f;             // main.cpp:81
+constexpr float kSsaoRad = 2.f;              // main.cpp:82
+constexpr float kSsaoDiv = 2.5f;             // cl2.cl:2253-2255
+constexpr int kShadowBias = 50;              // ocl.h:234
+constexpr int kMipLevels = 4;                // cl2.cl:5
+constexpr float kMipBias = 1.1f;             // cl2.cl:5750-5752
+
+struct f3 { float x, y, z; };
+struct f4 { float x, y, z, w; };
+
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ f3 operator-(f3 a) { return mk3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ f3 fast_normalize(f3 v) { return v * (1.0f / sqrtf(dot(v, v))); }
+__device__ __forceinline__ float fast_length(f3 v) { return sqrtf(dot(v, v)); }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(fminf(a, b), c); }
+__device__ __forceinline__ float max3f(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
+
+__device__ __forceinline__ uint32_t sat_u32(float f) {
+  if (!(f > 0.0f)) return 0u;
+  if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)f;
+}
+
+// exp in binary64: Cody-Waite reduction by ln2 + degree-13 Taylor polynomial.
+__device__ __forceinline__ double exp_d(double x) {
+  if (x != x) return x;
+  if (x > 100.0) return __longlong_as_double(0x7FF0000000000000ll);
+  if (x < -120.0) return 0.0;
+  const double kInvLn2 = 1.44269504088896338700e+00;
+  const double kLn2Hi = 6.93147180369123816490e-01;
+  const double kLn2Lo = 1.90821492927058770002e-10;
+  double k = floor(fma(x, kInvLn2, 0.5));
+  double r = fma(-k, kLn2Lo, fma(-k, kLn2Hi, x));
+  double p = 1.0 / 6227020800.0;
+  p = fma(p, r, 1.0 / 479001600.0);
+  p = fma(p, r, 1.0 / 39916800.0);
+  p = fma(p, r, 1.0 / 3628800.0);
+  p = fma(p, r, 1.0 / 362880.0);
+  p = fma(p, r, 1.0 / 40320.0);
+  p = fma(p, r, 1.0 / 5040.0);
+  p = fma(p, r, 1.0 / 720.0);
+  p = fma(p, r, 1.0 / 120.0);
+  p = fma(p, r, 1.0 / 24.0);
+  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  long long ki = (long long)k;
+  double scale = __longlong_as_double((ki + 1023) << 52);
+  return p * scale;
+}
+
+// tan in binary64: reduction by pi/2 + Taylor sin (deg 19) / cos (deg 18).
+__device__ __forceinline__ double tan_d(double x) {
+  const double kTwoOverPi = 6.36619772367581382433e-01;
+  const double kPio2Hi = 1.57079632673412561417e+00;
+  const double kPio2Lo = 6.07710050650619224932e-11;
+  double k = floor(fma(x, kTwoOverPi, 0.5));
+  double r = fma(-k, kPio2Lo, fma(-k, kPio2Hi, x));
+  double r2 = r * r;
+  double s = -1.0 / 121645100408832000.0;
+  s = fma(s, r2, 1.0 / 355687428096000.0);
+  s = fma(s, r2, -1.0 / 1307674368000.0);
+  s = fma(s, r2, 1.0 / 6227020800.0);
+  s = fma(s, r2, -1.0 / 39916800.0);
+  s = fma(s, r2, 1.0 / 362880.0);
+  s = fma(s, r2, -1.0 / 5040.0);
+  s = fma(s, r2, 1.0 / 120.0);
+  s = fma(s, r2, -1.0 / 6.0);
+  s = fma(s * r2, r, r);
+  double c = 1.0 / 6402373705728000.0;
+  c = fma(c, r2, -1.0 / 20922789888000.0);
+  c = fma(c, r2, 1.0 / 87178291200.0);
+  c = fma(c, r2, -1.0 / 479001600.0);
+  c = fma(c, r2, 1.0 / 3628800.0);
+  c = fma(c, r2, -1.0 / 40320.0);
+  c = fma(c, r2, 1.0 / 720.0);
+  c = fma(c, r2, -1.0 / 24.0);
+  c = fma(c, r2, 0.5);
+  c = fma(-c, r2, 1.0);
+  long long ki = (long long)k;
+  return (ki & 1) ? -c / s : s / c;
+}
+
+__device__ __forceinline__ float exp_f(float x) { return (float)exp_d((double)x); }
+__device__ __forceinline__ float tan_f(float x) { return (float)tan_d((double)x); }
+__device__ __forceinline__ float powr5(float x) { double d = x; double d2 = d * d; return (float)((d2 * d2) * d); }
+__device__ __forceinline__ float pow4(float x) { double d = x; double d2 = d * d; return (float)(d2 * d2); }
+
+// sin/cos of a rotation (camera or cube face), computed on the host.
+struct SinCos { float s[3]; float c[3]; };
+
+// rot, cl2.cl:220-236 (extrinsic XYZ about c_pos).
+__device__ __forceinline__ f3 rot(f3 point, f3 c_pos, const SinCos& sc) {
+  const float cx = sc.c[0], cy = sc.c[1], cz = sc.c[2];
+  const float sx = sc.s[0], sy = sc.s[1], sz = sc.s[2];
+  f3 rel = point - c_pos;
+  float inner = fmaf(sz, rel.y, cz * rel.x);
+  f3 r;
+  r.x = fmaf(cy, inner, -sy * rel.z);
+  float t = fmaf(cy, rel.z, sy * inner);
+  float u = fmaf(cz, rel.y, -sz * rel.x);
+  r.y = fmaf(sx, t, cx * u);
+  r.z = fmaf(cx, t, -sx * u);
+  return r;
+}
+
+// back_rot with c_pos = 0, cl2.cl:272-349.
+__device__ __forceinline__ f3 back_rot(f3 rel, const SinCos& sc) {
+  const float cx = sc.c[0], cy = sc.c[1], cz = sc.c[2];
+  const float sx = sc.s[0], sy = sc.s[1], sz = sc.s[2];
+  f3 r;
+  r.x = cz * fmaf(cy, rel.x, fmaf(sx, sy * rel.y, cx * sy * rel.z)) + sz * (sx * rel.z - cx * rel.y);
+  r.y = fmaf(sz, cy * rel.x, fmaf(fmaf(cx, cz, sx * sy * sz), rel.y, fmaf(-sx, cz, cx * sy * sz) * rel.z));
+  r.z = fmaf(-sy, rel.x, cy * fmaf(sx, rel.y, cx * rel.z));
+  return r;
+}
+
+// rot_quat, cl2.cl:350-357.
+__device__ __forceinline__ f3 rot_quat(f3 p, f4 q) {
+  float d = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+  float inv = 1.0f / sqrtf(d);
+  f3 qv = mk3(q.x * inv, q.y * inv, q.z * inv);
+  float qw = q.w * inv;
+  f3 t = cross(qv, p) * 2.f;
+  return (p + t * qw) + cross(qv, t);
+}
+
+// back_rot_quat, cl2.cl:359-370.
+__device__ __forceinline__ f3 back_rot_quat(f3 p, f4 q) {
+  f4 c{-q.x, -q.y, -q.z, q.w};
+  float len_sq = c.x * c.x + c.y * c.y + c.z * c.z + c.w * c.w;
+  return rot_quat(p, f4{c.x / len_sq, c.y / len_sq, c.z / len_sq, c.w / len_sq});
+}
+
+// depth_project, cl2.cl:535-545.
+__device__ __forceinline__ f3 depth_project(f3 r, float width, float height, float fovc) {
+  float f = fovc / r.z;
+  return mk3(fmaf(r.x, f, width / 2.f), fmaf(r.y, f, height / 2.f), r.z);
+}
+
+// generate_new_triangles, cl2.cl:577-663: near/far clip into 0..2 triangles.
+__device__ __forceinline__ int clip_near(const f3 pts[3], f3 out[2][3]) {
+  int idv = 0, behind[3] = {0, 0, 0}, nb = 0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    if (pts[i].z <= (float)kIcutoff || pts[i].z > kDepthFar) behind[nb++] = i;
+    else idv = i;
+  }
+  if (nb > 2) return 0;
+  if (nb == 0) { out[0][0] = pts[0]; out[0][1] = pts[1]; out[0][2] = pts[2]; return 1; }
+  int g1, g2, g3;
+  if (nb == 1) {
+    int id = behind[0];
+    g1 = id;
+    g2 = (id + 1) >= 3 ? id - 2 : id + 1;
+    g3 = (id + 2) >= 3 ? id - 1 : id + 2;
+  } else {
+    g2 = behind[0]; g3 = behind[1]; g1 = idv;
+  }
+  f3 p1 = pts[g2] + ((pts[g1] - pts[g2]) * ((float)kIcutoff - pts[g2].z)) / (pts[g1].z - pts[g2].z);
+  f3 p2 = pts[g3] + ((pts[g1] - pts[g3]) * ((float)kIcutoff - pts[g3].z)) / (pts[g1].z - pts[g3].z);
+  if (nb == 1) {
+    out[0][0] = p1; out[0][1] = pts[g2]; out[0][2] = pts[g3];
+    out[1][0] = p1; out[1][1] = pts[g3]; out[1][2] = p2;
+    return 2;
+  }
+  out[0][behind[0]] = p1;
+  out[0][behind[1]] = p2;
+  out[0][idv] = pts[g1];
+  return 1;
+}
+
+// backface_cull_expanded, cl2.cl:488-494 (1 = keep).
+__device__ __forceinline__ bool front_facing(f3 p0, f3 p1, f3 p2) {
+  return cross(p1 - p0, p2 - p0).z < 0.f;
+}
+
+// calc_min_max, cl2.cl:420-441.
+__device__ __forceinline__ void calc_min_max(const float xs[3], const float ys[3], float width,
+                                             float height, float mm[4]) {
+  float x0 = roundf(xs[0]), x1 = roundf(xs[1]), x2 = roundf(xs[2]);
+  float y0 = roundf(ys[0]), y1 = roundf(ys[1]), y2 = roundf(ys[2]);
+  mm[0] = clampf(min3f(x0, x1, x2) - 1, 0.0f, width - 1);
+  mm[1] = clampf(max3f(x0, x1, x2), 0.0f, width - 1);
+  mm[2] = clampf(min3f(y0, y1, y2) - 1, 0.0f, height - 1);
+  mm[3] = clampf(max3f(y0, y1, y2), 0.0f, height - 1);
+}
+
+// calc_rconstant_v, cl2.cl:403-406, on rounded coordinates.
+__device__ __forceinline__ float rconstant(f3 x, f3 y) {
+  return 1.0f / (x.y * y.z + x.x * (y.y - y.z) - x.z * y.y + (x.z - x.y) * y.x);
+}
+
+// interpolate_get_const, cl2.cl:408-418.
+__device__ __forceinline__ void plane_const(f3 f, f3 x, f3 y, float rc, float& A, float& B, float& C) {
+  A = ((f.y * y.z + f.x * (y.y - y.z) - f.z * y.y + (f.z - f.y) * y.x) * rc);
+  B = (-(f.y * x.z + f.x * (x.y - x.z) - f.z * x.y + (f.z - f.y) * x.x) * rc);
+  C = f.x - A * x.x - B * y.x;
+}
+
+// point_in_tri, cl2.cl:4798-4807 (on rounded vertex coordinates).
+struct EdgeSetup { float A2s; float s0, sx, sy, t0, tx, ty; };
+__device__ __forceinline__ bool point_in_tri(float px, float py, float p0x, float p0y, float p1x,
+                                             float p1y, float p2x, float p2y) {
+  float A = 0.5f * (-p1y * p2x + p0y * (-p1x + p2x) + p0x * (p1y - p2y) + p1x * p2y);
+  float sign = A < 0 ? -1.f : 1.f;
+  float s = (p0y * p2x - p0x * p2y + (p2y - p0y) * px + (p0x - p2x) * py) * sign;
+  float t = (p0x * p1y - p0y * p1x + (p0y - p1y) * px + (p1x - p0x) * py) * sign;
+  return s > -0.0001f && t > -0.0001f && (s + t) < 2.0001f * A * sign;
+}
+
+// wang_hash / rand_xorshift, cl2.cl:1919-1936.
+__device__ __forceinline__ uint32_t wang_hash(uint32_t seed) {
+  seed = (seed ^ 61u) ^ (seed >> 16);
+  seed *= 9u;
+  seed = seed ^ (seed >> 4);
+  seed *= 0x27d4eb2du;
+  seed = seed ^ (seed >> 15);
+  return seed;
+}
+__device__ __forceinline__ uint32_t xorshift(uint32_t s) {
+  s ^= (s << 13);
+  s ^= (s >> 17);
+  s ^= (s << 5);
+  return s;
+}
+
+// rational_acos, cl2.cl:2469-2477.
+__device__ __forceinline__ float rational_acos(float x) {
+  const float a = -0.939115566365855f;
+  const float b = 0.9217841528914573f;
+  const float c = -1.2845906244690837f;
+  const float d = 0.295624144969963174f;
+  return kPi / 2.f + (a * x + b * x * x * x) / (1.f + c * x * x + d * pow4(x));
+}
+
+// log2_approx, cl2.cl:1498-1505.
+__device__ __forceinline__ float log2_approx(float val) {
+  int x = __float_as_int(val);
+  float log_2 = (float)(((x >> 23) & 255) - 128);
+  x &= ~(255 << 23);
+  x += 127 << 23;
+  float u = __int_as_float(x);
+  log_2 += ((-0.3358287811f) * u + 2.0f) * u - 0.65871759316667f;
+  return log_2;
+}
+
+// gamma_transform_approx / gamma_inverse_approx, cl2.cl:5763-5782.
+__device__ __forceinline__ float gamma_fwd(float c) {
+  return 0.012522878f * c + 0.682171111f * c * c + 0.305306011f * c * c * c;
+}
+__device__ __forceinline__ float gamma_inv(float c) {
+  float s1 = sqrtf(c), s2 = sqrtf(s1), s3 = sqrtf(s2);
+  return 0.585122381f * s1 + 0.783140355f * s2 - 0.368262736f * s3;
+}
+
+// ret_cubeface, cl2.cl:1745-1790.
+__device__ __forceinline__ int cubeface(f3 point, f3 light) {
+  f3 rel = point - light;
+  float ax = fabsf(rel.x), ay = fabsf(rel.y), az = fabsf(rel.z);
+  if (ax >= ay && ax >= az) return rel.x < 0 ? 4 : 5;
+  if (ay > ax && ay >= az) return rel.y < 0 ? 1 : 3;
+  if (az > ax && az > ay) { if (rel.z < 0) return 2; }
+  return 0;
+}
+
+// get_barycentric, cl2.cl:5372-5384.
+__device__ __forceinline__ void barycentric(f3 p, f3 a, f3 b, f3 c, float& u, float& v, float& w) {
+  f3 v0 = b - a, v1 = c - a, v2 = p - a;
+  float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
+  float d20 = dot(v2, v0), d21 = dot(v2, v1);
+  float denom = d00 * d11 - d01 * d01;
+  v = (d11 * d20 - d01 * d21) / denom;
+  w = (d00 * d21 - d01 * d20) / denom;
+  u = 1.0f - v - w;
+}
+
+// float_to_short / encode_normal, cl2.cl:5592-5630 ("POW2e16-1" expands
+// without parentheses; the ushort conversion saturates).
+__device__ __forceinline__ uint32_t to_short(float r) {
+  float f = ((r + 1) / 2) * 65536 - 1;
+  if (!(f > 0.f)) return 0u;
+  if (f >= 65535.f) return 65535u;
+  return (uint32_t)f;
+}
+__device__ __forceinline__ uint32_t encode_normal(f3 v) {
+  if (v.x * v.x + v.y * v.y < 0.0001f) v.x = 0.01f;
+  float inv = 1.0f / sqrtf(v.x * v.x + v.y * v.y);
+  float sq = sqrtf(fmaxf(v.z * 0.5f + 0.5f, 0.f));
+  return to_short((v.x * inv) * sq) | (to_short((v.y * inv) * sq) << 16);
+}
+
+__device__ __forceinline__ uint32_t quant8(float v) {
+  float c = clampf(v, 0.f, 1.f);
+  return (uint32_t)floorf(c * 255.f + 0.5f);
+}
+
+}  // namespace ocr

@@ -1,0 +1,613 @@
+// runtime.hip — the C ABI of include/ocr.h: context, device-resident scene
+// (geometry, descriptors, atlas, lights, cube maps), offscreen frames and the
+// frame orchestration (clear -> prearrange -> depth -> id -> shade).
+//
+// Replaces the reference's OpenCL state (ocl.h:328-537, clstate.h:36-120),
+// object_context_data's device buffers (object_context.hpp:82-165),
+// texture_context::alloc_gpu (texture_context.cpp:350-517), light::build
+// (light.cpp:145-276), engine::render_tris (engine.cpp:1794-2161) and
+// engine::generate_realtime_shadowing (engine.cpp:1601-1790).
+#pragma clang fp contract(off)
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <new>
+
+#include "ocr_internal.h"
+
+namespace ocr {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+template <typename T>
+static int dev_alloc(T** p, size_t count, const char* what) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(OCR_ERR_OOM, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+  }
+  return OCR_OK;
+}
+
+template <typename T>
+static void dev_free(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+// sin/cos of an angle: binary64 evaluation rounded once to binary32.
+static void sincos_of(const float ang[3], SinCos& sc) {
+  for (int i = 0; i < 3; i++) {
+    sc.s[i] = (float)sin((double)ang[i]);
+    sc.c[i] = (float)cos((double)ang[i]);
+  }
+}
+
+// The six cube-face rotations r_struct (cl2.cl:2549-2574, :4487-4511); the
+// reference builds with -cl-single-precision-constant, so these are binary32.
+static void face_table(SinCos out[6]) {
+  const float pi = kPi;
+  const float ang[6][3] = {{0, 0, 0},          {pi / 2.0f, 0, 0},         {0, pi, 0},
+                           {3.0f * pi / 2.0f, 0, 0}, {0, 3.0f * pi / 2.0f, 0}, {0, pi / 2.0f, 0}};
+  for (int i = 0; i < 6; i++) sincos_of(ang[i], out[i]);
+}
+
+static ViewParams make_view_params(const ocr_view* v) {
+  ViewParams p;
+  memset(&p, 0, sizeof(p));
+  p.width = v->width;
+  p.height = v->height;
+  p.fov = v->fov_const;
+  p.linear = v->use_linear_rendering;
+  for (int i = 0; i < 3; i++) p.cpos[i] = v->c_pos[i];
+  sincos_of(v->c_rot, p.cam);
+  for (int i = 0; i < 4; i++) p.clear[i] = v->clear_col[i];
+  return p;
+}
+
+}  // namespace ocr
+
+using namespace ocr;
+
+struct ocr_ctx : public Ctx {};
+struct ocr_scene : public Scene {};
+struct ocr_frame : public Frame {};
+
+extern "C" {
+
+const char* ocr_last_error(void) { return g_last_error.c_str(); }
+
+int ocr_create(int device, ocr_ctx** out) {
+  if (!out) return fail(OCR_ERR_INVALID, "ocr_create: null out");
+  *out = nullptr;
+  int n = 0;
+  OCR_HIP_CHECK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(OCR_ERR_INVALID, "ocr_create: no such device");
+  OCR_HIP_CHECK(hipSetDevice(device));
+  ocr_ctx* c = new (std::nothrow) ocr_ctx();
+  if (!c) return fail(OCR_ERR_OOM, "ocr_create: host allocation");
+  c->device = device;
+  hipError_t e = hipGetDeviceProperties(&c->prop, device);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(OCR_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+  }
+  face_table(c->face_sc);
+  *out = c;
+  return OCR_OK;
+}
+
+void ocr_destroy(ocr_ctx* ctx) { delete ctx; }
+
+int ocr_query_free_hbm(ocr_ctx* ctx, size_t* free_bytes, size_t* total_bytes) {
+  if (!ctx) return fail(OCR_ERR_INVALID, "ocr_query_free_hbm: null ctx");
+  OCR_HIP_CHECK(hipSetDevice(ctx->device));
+  size_t f = 0, t = 0;
+  OCR_HIP_CHECK(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return OCR_OK;
+}
+
+// ----------------------------------------------------------------- scene
+static void pack_obj(const ocr_obj_desc& s, ObjDesc& d) {
+  memset(&d, 0, sizeof(d));
+  for (int i = 0; i < 3; i++) d.wp[i] = s.world_pos[i];
+  d.scale = s.scale;
+  for (int i = 0; i < 4; i++) d.q[i] = s.world_rot_quat[i];
+  d.tid = s.tid;
+  d.specular = s.specular;
+  d.spec_mult = s.spec_mult;
+  d.diffuse = s.diffuse;
+  d.flags = s.feature_flag;
+}
+
+int ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri, const ocr_obj_desc* objs,
+                     uint32_t nobj, ocr_scene** out) {
+  if (!ctx || !out || (ntri && !tris) || (nobj && !objs))
+    return fail(OCR_ERR_INVALID, "ocr_scene_create: bad argument");
+  if (ntri >= 0x7FFFFFFFu) return fail(OCR_ERR_UNSUPPORTED, "ocr_scene_create: too many triangles");
+  *out = nullptr;
+  OCR_HIP_CHECK(hipSetDevice(ctx->device));
+  ocr_scene* s = new (std::nothrow) ocr_scene();
+  if (!s) return fail(OCR_ERR_OOM, "ocr_scene_create: host allocation");
+  s->ctx = ctx;
+  s->ntri = ntri;
+  s->nobj = nobj;
+  // Split the 144-byte AoS triangle into the 40-byte position record read by
+  // every raster pass and the 72-byte attribute record read only by shading.
+  std::vector<TriPos> pos(ntri);
+  std::vector<TriAttr> attr(ntri);
+  for (uint32_t t = 0; t < ntri; t++) {
+    const ocr_triangle& T = tris[t];
+    for (int v = 0; v < 3; v++) {
+      for (int k = 0; k < 3; k++) {
+        pos[t].p[3 * v + k] = T.vertices[v].pos[k];
+        attr[t].n[3 * v + k] = T.vertices[v].normal[k];
+      }
+      attr[t].vt[2 * v] = T.vertices[v].vt[0];
+      attr[t].vt[2 * v + 1] = T.vertices[v].vt[1];
+      attr[t].col[v] = T.vertices[v].vertex_col;
+    }
+    pos[t].oid = T.vertices[0].object_id;
+  }
+  std::vector<ObjDesc> od(nobj);
+  for (uint32_t o = 0; o < nobj; o++) {
+    pack_obj(objs[o], od[o]);
+    s->max_tid = std::max(s->max_tid, objs[o].tid);
+  }
+  int rc;
+  if ((rc = dev_alloc(&s->tri_pos, ntri, "triangles")) || (rc = dev_alloc(&s->tri_attr, ntri, "attributes")) ||
+      (rc = dev_alloc(&s->objs, nobj, "descriptors")) || (rc = dev_alloc(&s->sh_counters, kCounterWords, "counters"))) {
+    ocr_scene_destroy(s);
+    return rc;
+  }
+  hipError_t e = hipSuccess;
+  if (ntri) e = hipMemcpy(s->tri_pos, pos.data(), sizeof(TriPos) * ntri, hipMemcpyHostToDevice);
+  if (e == hipSuccess && ntri) e = hipMemcpy(s->tri_attr, attr.data(), sizeof(TriAttr) * ntri, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nobj) e = hipMemcpy(s->objs, od.data(), sizeof(ObjDesc) * nobj, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev_sh0);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev_sh1);
+  if (e != hipSuccess) {
+    ocr_scene_destroy(s);
+    return fail(OCR_ERR_HIP, std::string("ocr_scene_create: ") + hipGetErrorString(e));
+  }
+  *out = s;
+  return OCR_OK;
+}
+
+int ocr_scene_update_objs(ocr_scene* s, uint32_t first, uint32_t n, const ocr_obj_desc* objs) {
+  if (!s || (n && !objs) || first > s->nobj || n > s->nobj - first)
+    return fail(OCR_ERR_INVALID, "ocr_scene_update_objs: range outside the scene");
+  if (!n) return OCR_OK;
+  std::vector<ObjDesc> od(n);
+  for (uint32_t o = 0; o < n; o++) {
+    pack_obj(objs[o], od[o]);
+    s->max_tid = std::max(s->max_tid, objs[o].tid);
+  }
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  OCR_HIP_CHECK(hipMemcpy(s->objs + first, od.data(), sizeof(ObjDesc) * n, hipMemcpyHostToDevice));
+  return OCR_OK;
+}
+
+static void free_atlas(Atlas& a) {
+  dev_free(a.pages);
+  dev_free(a.nums);
+  dev_free(a.sizes);
+  a.npages = a.nnums = a.nsizes = a.mip_start = 0;
+}
+
+void ocr_scene_destroy(ocr_scene* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->ctx->device);
+  dev_free(s->tri_pos);
+  dev_free(s->tri_attr);
+  dev_free(s->objs);
+  free_atlas(s->atlas);
+  dev_free(s->lights);
+  dev_free(s->shadow_maps);
+  dev_free(s->static_maps);
+  dev_free(s->sh_cuts);
+  dev_free(s->sh_frags);
+  dev_free(s->sh_counters);
+  if (s->ev_sh0) (void)hipEventDestroy(s->ev_sh0);
+  if (s->ev_sh1) (void)hipEventDestroy(s->ev_sh1);
+  delete s;
+}
+
+// --------------------------------------------------------------- atlas
+static uint32_t pow2_ceil(uint32_t v) {
+  uint32_t n = 1;
+  while (n < v) n <<= 1;
+  return n;
+}
+
+int ocr_atlas_upload(ocr_scene* s, const ocr_tex_src* texs, uint32_t ntex) {
+  if (!s || (ntex && !texs)) return fail(OCR_ERR_INVALID, "ocr_atlas_upload: bad argument");
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  // texture_context.cpp:94-165: count slots per power-of-two size class
+  // (base + 4 mips per texture) and cut them into 2048^2 pages, smallest
+  // class first.
+  std::vector<uint32_t> base(ntex);
+  std::map<uint32_t, uint32_t> per_size;
+  for (uint32_t t = 0; t < ntex; t++) {
+    if (!texs[t].rgba || texs[t].width == 0 || texs[t].height == 0)
+      return fail(OCR_ERR_INVALID, "ocr_atlas_upload: empty texture");
+    base[t] = pow2_ceil(std::max(texs[t].width, texs[t].height));
+    if (base[t] < 16 || base[t] > OCR_ATLAS_DIM)
+      return fail(OCR_ERR_UNSUPPORTED, "ocr_atlas_upload: texture size must round to 16..2048");
+    for (int j = 0; j <= OCR_MIP_LEVELS; j++) per_size[base[t] >> j]++;
+  }
+  struct Page { uint32_t size; uint32_t free; };
+  std::vector<Page> pages;
+  for (auto& kv : per_size) {
+    uint32_t per_page = (OCR_ATLAS_DIM / kv.first) * (OCR_ATLAS_DIM / kv.first);
+    uint32_t rem = kv.second;
+    while (rem >= per_page) { pages.push_back({kv.first, per_page}); rem -= per_page; }
+    if (rem) pages.push_back({kv.first, rem});
+  }
+  // texture_context.cpp:167-250: each texture, then each texture's mips,
+  // takes the last free slot of the first page of its size.
+  std::vector<uint32_t> nums(std::max<uint32_t>(ntex * (1 + OCR_MIP_LEVELS), 1 + OCR_MIP_LEVELS), 0);
+  auto take = [&](uint32_t size, uint32_t& desc) -> bool {
+    for (size_t p = 0; p < pages.size(); p++)
+      if (pages[p].size == size && pages[p].free > 0) {
+        pages[p].free--;
+        desc = ((uint32_t)p << 16) | pages[p].free;
+        return true;
+      }
+    return false;
+  };
+  for (uint32_t t = 0; t < ntex; t++)
+    if (!take(base[t], nums[t])) return fail(OCR_ERR_INVALID, "ocr_atlas_upload: page plan");
+  for (uint32_t t = 0; t < ntex; t++)
+    for (int j = 0; j < OCR_MIP_LEVELS; j++)
+      if (!take(base[t] >> (j + 1), nums[ntex + t * OCR_MIP_LEVELS + j]))
+        return fail(OCR_ERR_INVALID, "ocr_atlas_upload: mip page plan");
+  std::vector<uint32_t> sizes(std::max<size_t>(pages.size(), 2), 16);
+  for (size_t p = 0; p < pages.size(); p++) sizes[p] = pages[p].size;
+
+  free_atlas(s->atlas);
+  Atlas& a = s->atlas;
+  a.npages = (uint32_t)std::max<size_t>(pages.size(), 1);
+  a.nnums = (uint32_t)nums.size();
+  a.nsizes = (uint32_t)sizes.size();
+  a.mip_start = ntex;
+  int rc;
+  size_t page_bytes = (size_t)OCR_ATLAS_DIM * OCR_ATLAS_DIM * 4;
+  if ((rc = dev_alloc(&a.pages, page_bytes * a.npages, "atlas")) || (rc = dev_alloc(&a.nums, a.nnums, "nums")) ||
+      (rc = dev_alloc(&a.sizes, a.nsizes, "sizes"))) {
+    free_atlas(a);
+    return rc;
+  }
+  OCR_HIP_CHECK(hipMemset(a.pages, 0, page_bytes * a.npages));
+  OCR_HIP_CHECK(hipMemcpy(a.nums, nums.data(), sizeof(uint32_t) * a.nnums, hipMemcpyHostToDevice));
+  OCR_HIP_CHECK(hipMemcpy(a.sizes, sizes.data(), sizeof(uint32_t) * a.nsizes, hipMemcpyHostToDevice));
+  // texture::update_me_to_gpu + update_gpu_mipmaps (texture.cpp:323-493).
+  size_t max_bytes = 0;
+  for (uint32_t t = 0; t < ntex; t++) max_bytes = std::max(max_bytes, (size_t)texs[t].width * texs[t].height * 4);
+  uint8_t* staging = nullptr;
+  if (ntex && (rc = dev_alloc(&staging, max_bytes, "texture staging"))) return rc;
+  hipStream_t st = nullptr;
+  int status = OCR_OK;
+  for (uint32_t t = 0; t < ntex && status == OCR_OK; t++) {
+    const ocr_tex_src& src = texs[t];
+    hipError_t e = hipMemcpy(staging, src.rgba, (size_t)src.width * src.height * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_atlas_upload(a, t, staging, src.width, src.height, st);
+    if (e == hipSuccess) e = launch_mip(a, t, t * OCR_MIP_LEVELS + ntex, src.width, src.height, st);
+    for (uint32_t lvl = 0; lvl + 1 < OCR_MIP_LEVELS && e == hipSuccess; lvl++) {
+      uint32_t proper = t * OCR_MIP_LEVELS + ntex + lvl;
+      e = launch_mip(a, proper, proper + 1, src.width, src.height, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) status = fail(OCR_ERR_HIP, std::string("ocr_atlas_upload: ") + hipGetErrorString(e));
+  }
+  dev_free(staging);
+  return status;
+}
+
+int ocr_scene_atlas_info(ocr_scene* s, uint32_t* npages, uint32_t* nnums, uint32_t* nsizes, uint32_t* mip_start) {
+  if (!s) return fail(OCR_ERR_INVALID, "ocr_scene_atlas_info: null scene");
+  if (npages) *npages = s->atlas.npages;
+  if (nnums) *nnums = s->atlas.nnums;
+  if (nsizes) *nsizes = s->atlas.nsizes;
+  if (mip_start) *mip_start = s->atlas.mip_start;
+  return OCR_OK;
+}
+
+int ocr_scene_read_atlas(ocr_scene* s, uint8_t* pages, uint32_t* nums, uint32_t* sizes) {
+  if (!s || !s->atlas.pages) return fail(OCR_ERR_INVALID, "ocr_scene_read_atlas: no atlas");
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  OCR_HIP_CHECK(hipDeviceSynchronize());
+  size_t page_bytes = (size_t)OCR_ATLAS_DIM * OCR_ATLAS_DIM * 4;
+  if (pages) OCR_HIP_CHECK(hipMemcpy(pages, s->atlas.pages, page_bytes * s->atlas.npages, hipMemcpyDeviceToHost));
+  if (nums) OCR_HIP_CHECK(hipMemcpy(nums, s->atlas.nums, 4 * s->atlas.nnums, hipMemcpyDeviceToHost));
+  if (sizes) OCR_HIP_CHECK(hipMemcpy(sizes, s->atlas.sizes, 4 * s->atlas.nsizes, hipMemcpyDeviceToHost));
+  return OCR_OK;
+}
+
+// ---------------------------------------------------------------- lights
+int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_t light_buffer_dim) {
+  if (!s || (n && !lights)) return fail(OCR_ERR_INVALID, "ocr_lights_upload: bad argument");
+  if (light_buffer_dim < 8 || light_buffer_dim > 8192)
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light buffer dimension must be 8..8192");
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  dev_free(s->lights);
+  dev_free(s->shadow_maps);
+  dev_free(s->static_maps);
+  for (uint32_t i = 0; i < n; i++)
+    if (lights[i].shadow > 1)
+      return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light.shadow must be 0 or 1");
+  s->host_lights.assign(lights, lights + n);
+  s->nlights = n;
+  s->lbuf_dim = light_buffer_dim;
+  s->n_shadow = s->n_static = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    if (lights[i].shadow == 1) s->n_shadow++;
+    if (lights[i].shadow && lights[i].is_static) s->n_static++;
+  }
+  int rc;
+  size_t per = (size_t)6 * light_buffer_dim * light_buffer_dim;
+  if ((rc = dev_alloc(&s->lights, n, "lights")) ||
+      (rc = dev_alloc(&s->shadow_maps, per * std::max<uint32_t>(s->n_shadow, 1), "shadow maps")) ||
+      (rc = dev_alloc(&s->static_maps, per * std::max<uint32_t>(s->n_static, 1), "static shadow maps")))
+    return rc;
+  if (n) OCR_HIP_CHECK(hipMemcpy(s->lights, lights, sizeof(ocr_light) * n, hipMemcpyHostToDevice));
+  OCR_HIP_CHECK(launch_fill_u32(s->shadow_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_shadow, 1), nullptr));
+  OCR_HIP_CHECK(launch_fill_u32(s->static_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_static, 1), nullptr));
+  OCR_HIP_CHECK(hipDeviceSynchronize());
+  s->static_dirty = true;
+  return OCR_OK;
+}
+
+static int ensure_shadow_lists(ocr_scene* s, uint32_t cap_cut, uint32_t cap_frag) {
+  if (s->sh_cuts && s->sh_cap_cut >= cap_cut && s->sh_frags && s->sh_cap_frag >= cap_frag) return OCR_OK;
+  dev_free(s->sh_cuts);
+  dev_free(s->sh_frags);
+  s->sh_cap_cut = s->sh_cap_frag = 0;
+  int rc;
+  if ((rc = dev_alloc(&s->sh_cuts, cap_cut, "shadow cut list")) || (rc = dev_alloc(&s->sh_frags, cap_frag, "shadow fragment list")))
+    return rc;
+  s->sh_cap_cut = cap_cut;
+  s->sh_cap_frag = cap_frag;
+  return OCR_OK;
+}
+
+int ocr_shadow_pass(ocr_scene* s, void* stream) {
+  if (!s) return fail(OCR_ERR_INVALID, "ocr_shadow_pass: null scene");
+  if (s->n_shadow == 0 && !(s->static_dirty && s->n_static)) return OCR_OK;
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  if (!s->sh_cuts) {
+    // Each triangle reaches at most 6 faces x 2 clipped pieces.
+    uint64_t cut = std::min<uint64_t>((uint64_t)s->ntri * 12 + 1024, 0xFFFFFFF0ull);
+    uint64_t frag = std::min<uint64_t>((uint64_t)s->ntri * 4 + (1u << 20), 0xFFFFFFF0ull);
+    int rc = ensure_shadow_lists(s, (uint32_t)cut, (uint32_t)frag);
+    if (rc) return rc;
+  }
+  const size_t per = (size_t)6 * s->lbuf_dim * s->lbuf_dim;
+  // engine.cpp:1613-1625: clear every dynamic map, then render each
+  // shadow-casting light from the dynamic geometry; static lights' maps are
+  // rendered from static geometry when the lights were (re)uploaded.
+  OCR_HIP_CHECK(launch_fill_u32(s->shadow_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_shadow, 1), st));
+  uint32_t nn = 0, kk = 0;
+  for (uint32_t i = 0; i < s->nlights; i++) {
+    const ocr_light& l = s->host_lights[i];
+    if (l.shadow == 1) {
+      OCR_HIP_CHECK(launch_shadow_light(s, l.pos, 0, s->shadow_maps + per * nn, st));
+      nn++;
+    }
+    if (l.shadow && l.is_static && s->static_dirty) {
+      OCR_HIP_CHECK(launch_fill_u32(s->static_maps + per * kk, 0xFFFFFFFFu, per, st));
+      OCR_HIP_CHECK(launch_shadow_light(s, l.pos, 1, s->static_maps + per * kk, st));
+    }
+    if (l.shadow && l.is_static) kk++;
+  }
+  s->static_dirty = false;
+  return OCR_OK;
+}
+
+int ocr_scene_read_shadow_maps(ocr_scene* s, uint32_t* dynamic_maps, uint32_t* static_maps) {
+  if (!s) return fail(OCR_ERR_INVALID, "ocr_scene_read_shadow_maps: null scene");
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  OCR_HIP_CHECK(hipDeviceSynchronize());
+  uint32_t ov = 0;
+  OCR_HIP_CHECK(hipMemcpy(&ov, s->sh_counters + kOverflow, 4, hipMemcpyDeviceToHost));
+  const size_t per = (size_t)6 * s->lbuf_dim * s->lbuf_dim * 4;
+  if (dynamic_maps && s->n_shadow)
+    OCR_HIP_CHECK(hipMemcpy(dynamic_maps, s->shadow_maps, per * s->n_shadow, hipMemcpyDeviceToHost));
+  if (static_maps && s->n_static)
+    OCR_HIP_CHECK(hipMemcpy(static_maps, s->static_maps, per * s->n_static, hipMemcpyDeviceToHost));
+  if (ov) return fail(OCR_ERR_OVERFLOW, "shadow work list overflowed");
+  return OCR_OK;
+}
+
+// ---------------------------------------------------------------- frames
+int ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** out) {
+  if (!ctx || !out || width < 4 || height < 4) return fail(OCR_ERR_INVALID, "ocr_frame_create: bad argument (frames are at least 4x4)");
+  if ((uint64_t)width * height >= (1ull << 31) || width > 32768 || height > 32768)
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_create: frame too large");
+  *out = nullptr;
+  OCR_HIP_CHECK(hipSetDevice(ctx->device));
+  ocr_frame* f = new (std::nothrow) ocr_frame();
+  if (!f) return fail(OCR_ERR_OOM, "ocr_frame_create: host allocation");
+  f->ctx = ctx;
+  f->width = width;
+  f->height = height;
+  size_t npix = (size_t)width * height;
+  int rc;
+  if ((rc = dev_alloc(&f->depth, npix, "depth")) || (rc = dev_alloc(&f->idkey, npix, "id buffer")) ||
+      (rc = dev_alloc(&f->rgba, npix, "rgba")) || (rc = dev_alloc(&f->normals, npix, "normals")) ||
+      (rc = dev_alloc(&f->counters, kCounterWords, "counters"))) {
+    ocr_frame_destroy(f);
+    return rc;
+  }
+  for (int i = 0; i < 6; i++) {
+    if (hipEventCreate(&f->ev[i]) != hipSuccess) {
+      ocr_frame_destroy(f);
+      return fail(OCR_ERR_HIP, "ocr_frame_create: hipEventCreate");
+    }
+  }
+  hipError_t e = hipMemset(f->counters, 0, 4 * kCounterWords);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    ocr_frame_destroy(f);
+    return fail(OCR_ERR_HIP, std::string("ocr_frame_create: ") + hipGetErrorString(e));
+  }
+  *out = f;
+  return OCR_OK;
+}
+
+void ocr_frame_destroy(ocr_frame* f) {
+  if (!f) return;
+  (void)hipSetDevice(f->ctx->device);
+  dev_free(f->depth);
+  dev_free(f->idkey);
+  dev_free(f->rgba);
+  dev_free(f->normals);
+  dev_free(f->cuts);
+  dev_free(f->frags);
+  dev_free(f->counters);
+  for (int i = 0; i < 6; i++)
+    if (f->ev[i]) (void)hipEventDestroy(f->ev[i]);
+  delete f;
+}
+
+int ocr_frame_set_profiling(ocr_frame* f, int enable) {
+  if (!f) return fail(OCR_ERR_INVALID, "ocr_frame_set_profiling: null frame");
+  f->profiling = enable != 0;
+  return OCR_OK;
+}
+
+static int ensure_frame_lists(ocr_frame* f, uint32_t ntri) {
+  uint64_t want_cut = (uint64_t)ntri * 2 + 64;
+  uint64_t want_frag = std::max<uint64_t>((uint64_t)ntri * 2 + (uint64_t)f->width * f->height / 64 + 65536,
+                                          f->want_frag);
+  want_cut = std::min<uint64_t>(want_cut, 0xFFFFFFF0ull);
+  want_frag = std::min<uint64_t>(want_frag, 0xFFFFFFF0ull);
+  if (f->cuts && f->cap_cut >= want_cut && f->frags && f->cap_frag >= want_frag) return OCR_OK;
+  OCR_HIP_CHECK(hipDeviceSynchronize());
+  if (!f->cuts || f->cap_cut < want_cut) {
+    dev_free(f->cuts);
+    f->cap_cut = 0;
+    int rc = dev_alloc(&f->cuts, want_cut, "cut list");
+    if (rc) return rc;
+    f->cap_cut = (uint32_t)want_cut;
+  }
+  if (!f->frags || f->cap_frag < want_frag) {
+    dev_free(f->frags);
+    f->cap_frag = 0;
+    int rc = dev_alloc(&f->frags, want_frag, "fragment list");
+    if (rc) return rc;
+    f->cap_frag = (uint32_t)want_frag;
+  }
+  return OCR_OK;
+}
+
+int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
+  if (!s || !view || !f) return fail(OCR_ERR_INVALID, "ocr_render: null argument");
+  if (view->width != f->width || view->height != f->height)
+    return fail(OCR_ERR_INVALID, "ocr_render: view size differs from the frame");
+  if (!(view->fov_const > 0.f)) return fail(OCR_ERR_INVALID, "ocr_render: fov_const must be > 0");
+  if (!s->atlas.pages) return fail(OCR_ERR_INVALID, "ocr_render: scene has no atlas (ocr_atlas_upload)");
+  if (s->nobj && s->max_tid >= std::max<uint32_t>(s->atlas.mip_start, 1))
+    return fail(OCR_ERR_INVALID, "ocr_render: an object's texture id is outside the atlas");
+  if (!s->lights && s->nlights) return fail(OCR_ERR_INVALID, "ocr_render: lights missing");
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  int rc = ensure_frame_lists(f, s->ntri);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  ViewParams vp = make_view_params(view);
+  if (!s->lights) {
+    // a scene with no lights still needs valid (empty) light and map buffers
+    rc = ocr_lights_upload(s, nullptr, 0, s->lbuf_dim);
+    if (rc) return rc;
+  }
+  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[0], st));
+  OCR_HIP_CHECK(launch_clear_frame(f, st));
+  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[1], st));
+  OCR_HIP_CHECK(launch_prearrange(s, vp, f, st));
+  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[2], st));
+  OCR_HIP_CHECK(launch_depth(vp, f, st));
+  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[3], st));
+  OCR_HIP_CHECK(launch_id(vp, f, st));
+  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[4], st));
+  OCR_HIP_CHECK(launch_shade(s, vp, f, st));
+  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[5], st));
+  f->rendered = true;
+  return OCR_OK;
+}
+
+int ocr_frame_get_stats(ocr_frame* f, void* stream, ocr_frame_stats* out) {
+  if (!f || !out) return fail(OCR_ERR_INVALID, "ocr_frame_get_stats: null argument");
+  OCR_HIP_CHECK(hipSetDevice(f->ctx->device));
+  OCR_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  uint32_t c[kCounterWords];
+  OCR_HIP_CHECK(hipMemcpy(c, f->counters, sizeof(c), hipMemcpyDeviceToHost));
+  memset(out, 0, sizeof(*out));
+  out->n_cut = c[kCutCount];
+  out->n_frag = c[kFragCount];
+  out->overflow = c[kOverflow] ? 1u : 0u;
+  if (f->profiling && f->rendered) {
+    float ms[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < 5; i++) OCR_HIP_CHECK(hipEventElapsedTime(&ms[i], f->ev[i], f->ev[i + 1]));
+    out->ms_clear = ms[0];
+    out->ms_prearrange = ms[1];
+    out->ms_depth = ms[2];
+    out->ms_id = ms[3];
+    out->ms_shade = ms[4];
+    OCR_HIP_CHECK(hipEventElapsedTime(&out->ms_total, f->ev[0], f->ev[5]));
+  }
+  return OCR_OK;
+}
+
+int ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id, uint8_t* rgba8, uint16_t* normals,
+                   void* stream) {
+  if (!f) return fail(OCR_ERR_INVALID, "ocr_frame_read: null frame");
+  OCR_HIP_CHECK(hipSetDevice(f->ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  OCR_HIP_CHECK(hipStreamSynchronize(st));
+  uint32_t c[kCounterWords];
+  OCR_HIP_CHECK(hipMemcpy(c, f->counters, sizeof(c), hipMemcpyDeviceToHost));
+  if (c[kOverflow]) {
+    // Grow the lists so the next render of this frame fits.
+    f->want_frag = (uint32_t)std::min<uint64_t>((uint64_t)c[kFragCount] + c[kFragCount] / 4 + 1024, 0xFFFFFFF0ull);
+    return fail(OCR_ERR_OVERFLOW, "ocr_frame_read: fragment list overflowed; re-render the frame");
+  }
+  size_t npix = (size_t)f->width * f->height;
+  if (depth) OCR_HIP_CHECK(hipMemcpy(depth, f->depth, 4 * npix, hipMemcpyDeviceToHost));
+  if (rgba8) OCR_HIP_CHECK(hipMemcpy(rgba8, f->rgba, 4 * npix, hipMemcpyDeviceToHost));
+  if (normals) OCR_HIP_CHECK(hipMemcpy(normals, f->normals, 4 * npix, hipMemcpyDeviceToHost));
+  if (tri_id) {
+    std::vector<unsigned long long> k(npix);
+    std::vector<uint32_t> d(npix);
+    OCR_HIP_CHECK(hipMemcpy(k.data(), f->idkey, 8 * npix, hipMemcpyDeviceToHost));
+    OCR_HIP_CHECK(hipMemcpy(d.data(), f->depth, 4 * npix, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < npix; i++)
+      tri_id[i] = (k[i] && d[i] != 0xFFFFFFFFu) ? (uint32_t)(((k[i] >> 32) - 1ull) >> 1) : OCR_NO_TRIANGLE;
+  }
+  return OCR_OK;
+}
+
+int ocr_frame_device_buffers(ocr_frame* f, void** depth, void** idkey, void** rgba8, void** normals) {
+  if (!f) return fail(OCR_ERR_INVALID, "ocr_frame_device_buffers: null frame");
+  if (depth) *depth = f->depth;
+  if (idkey) *idkey = f->idkey;
+  if (rgba8) *rgba8 = f->rgba;
+  if (normals) *normals = f->normals;
+  return OCR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,412 @@
+// shade.hip — deferred shading, one work item per pixel (kernel3,
+// cl2.cl:5792-6408): un-project the quantised depth, object-space
+// barycentrics, mip-mapped atlas fetch (texture_filter_diff), SSAO, point
+// lights with cube-map shadows (generate_hard_occlusion) and Cook-Torrance
+// specular; writes RGBA8 and the encoded screen normal.
+//
+// Pixels are processed in 16x16 tiles (the reference's 16x16 work groups,
+// engine.cpp:2018-2022) so the SSAO neighbourhood and the tile's triangles
+// stay in the XCD's L2.
+#pragma clang fp contract(off)
+
+#include "ocr_internal.h"
+
+namespace ocr {
+
+struct ShadeArgs {
+  ViewParams v;
+  const uint32_t* depth;
+  const unsigned long long* idkey;
+  const CutTri* cuts;
+  const TriPos* tri_pos;
+  const TriAttr* tri_attr;
+  const ObjDesc* objs;
+  const uint8_t* pages;
+  const uint32_t* nums;
+  const uint32_t* sizes;
+  uint32_t mip_start;
+  const ocr_light* lights;
+  uint32_t nlights;
+  const uint32_t* shadow_maps;
+  const uint32_t* static_maps;
+  uint32_t L;
+  const uint32_t* counters;
+  uint32_t* rgba;
+  uint32_t* normals;
+  SinCos face[6];
+  uint32_t* tex_touch;      // count mode: 1 bit per atlas texel
+  uint32_t* shadow_touch;   // count mode: 1 bit per dynamic cube-map texel
+  uint32_t* static_touch;   // count mode: 1 bit per static cube-map texel
+};
+
+template <bool kCount>
+__device__ __forceinline__ void touch(uint32_t* bits, size_t idx) {
+  if constexpr (kCount) atomicOr(&bits[idx >> 5], 1u << (idx & 31));
+}
+
+// read_tex_array_all_precalculated, cl2.cl:820-851.
+template <bool kCount>
+__device__ __forceinline__ f4 read_tex_pre(const ShadeArgs& a, float cx, float cy, int which, int slice,
+                                           float width) {
+  const float imax_tex_size = 1.f / 2048;
+  float ihnum = width * imax_tex_size;
+  float tnumy = floorf((float)which * ihnum);
+  float tnumx = (float)which - tnumy / ihnum;
+  cx = clampf(cx, 0.001f, width - 0.001f);
+  cy = clampf(cy, 0.001f, width - 0.001f);
+  int ix = (int)fmaf(tnumx, width, cx);
+  int iy = (int)fmaf(tnumy, width, cy);
+  size_t t = (size_t)slice * 2048 * 2048 + (size_t)iy * 2048 + (size_t)ix;
+  touch<kCount>(a.tex_touch, t);
+  uint32_t c = *reinterpret_cast<const uint32_t*>(a.pages + t * 4);
+  return f4{(float)(c & 0xFF), (float)((c >> 8) & 0xFF), (float)((c >> 16) & 0xFF), (float)(c >> 24)};
+}
+
+// return_bilinear_col_all_precalculated, cl2.cl:1426-1455.
+template <bool kCount>
+__device__ __forceinline__ f4 bilinear_pre(const ShadeArgs& a, float mx, float my, int which, int slice,
+                                           float width) {
+  float px = floorf(mx), py = floorf(my);
+  f4 c0 = read_tex_pre<kCount>(a, px, py, which, slice, width);
+  f4 c1 = read_tex_pre<kCount>(a, px + 1, py, which, slice, width);
+  f4 c2 = read_tex_pre<kCount>(a, px, py + 1, which, slice, width);
+  f4 c3 = read_tex_pre<kCount>(a, px + 1, py + 1, which, slice, width);
+  float ux = mx - px, uy = my - py;
+  float bx = 1.f - ux, by = 1.f - uy;
+  f4 r;
+  r.x = fmaf(c0.x, bx, c1.x * ux) * by + fmaf(c2.x, bx, c3.x * ux) * uy;
+  r.y = fmaf(c0.y, bx, c1.y * ux) * by + fmaf(c2.y, bx, c3.y * ux) * uy;
+  r.z = fmaf(c0.z, bx, c1.z * ux) * by + fmaf(c2.z, bx, c3.z * ux) * uy;
+  r.w = fmaf(c0.w, bx, c1.w * ux) * by + fmaf(c2.w, bx, c3.w * ux) * uy;
+  return r;
+}
+
+// texture_mod, cl2.cl:1457-1468.
+__device__ __forceinline__ float tmod(float v) {
+  v = v >= 1 ? 1.0f - (v - floorf(v)) : v;
+  v = v < 0 ? 1.0f + fabsf(v) - fabsf(floorf(v)) : v;
+  return v;
+}
+
+// texture_filter_diff, cl2.cl:1511-1573.
+template <bool kCount>
+__device__ __forceinline__ f4 texture_filter_diff(const ShadeArgs& a, float vtx, float vty, float dx,
+                                                  float dy, int tid2) {
+  int nv = (int)a.nums[tid2];
+  int tsize = (int)a.sizes[nv >> 16];
+  float vmx = tmod(vtx), vmy = tmod(vty);
+  float tx = dx * (float)tsize, ty = dy * (float)tsize;
+  float worst = sqrtf(tx * tx + ty * ty);
+  float wf = fmaxf(log2_approx(worst), 0.f);
+  float mip_lower = clampf(floorf(wf), 0.f, (float)kMipLevels);
+  float fmd = wf - mip_lower;
+  int tid_lower = mip_lower == 0 ? tid2 : (int)(mip_lower - 1 + (float)a.mip_start + (float)(tid2 * kMipLevels));
+  int tid_higher = (int)(clampf(mip_lower, 0.f, kMipLevels - 1.f) + (float)a.mip_start + (float)(tid2 * kMipLevels));
+  int lnv = (int)a.nums[tid_lower], hnv = (int)a.nums[tid_higher];
+  float sl = (float)a.sizes[lnv >> 16], sh = (float)a.sizes[hnv >> 16];
+  f4 c1 = bilinear_pre<kCount>(a, vmx * sl, vmy * sl, lnv & 0xFFFF, lnv >> 16, sl);
+  f4 c2 = bilinear_pre<kCount>(a, vmx * sh, vmy * sh, hnv & 0xFFFF, hnv >> 16, sh);
+  const float i255 = 1.f / 255.f;
+  return f4{(c1.x + (c2.x - c1.x) * fmd) * i255, (c1.y + (c2.y - c1.y) * fmd) * i255,
+            (c1.z + (c2.z - c1.z) * fmd) * i255, (c1.w + (c2.w - c1.w) * fmd) * i255};
+}
+
+// generate_ssao, cl2.cl:2194-2260.
+__device__ __forceinline__ float ssao(const ShadeArgs& a, int sx, int sy) {
+  const uint32_t W = a.v.width, H = a.v.height;
+  uint32_t seed2 = xorshift(wang_hash((uint32_t)sx + W * H * (uint32_t)sy));
+  float foffset = (float)seed2 / kMulint;
+  float depth = ((float)a.depth[(size_t)sy * W + sx] / kMulint) * kDepthFar;
+  float rad = kSsaoRad;
+  rad += foffset / 2.f;
+  float world_rad = rad * a.v.fov / depth;
+  float fsx = (float)sx, fsy = (float)sy;
+  float acc = 0.f;
+  for (int y = -2; y <= 2; y++) {
+    float oy = roundf((float)y * world_rad);
+    float wy = clampf(fsy + oy, 1.f, (float)H - 2.f);
+    for (int x = -2; x <= 2; x++) {
+      float ox = roundf((float)x * world_rad);
+      float wx = clampf(fsx + ox, 1.f, (float)W - 2.f);
+      float d2 = ((float)a.depth[(size_t)(int)wy * W + (int)wx] / kMulint) * kDepthFar;
+      for (float z = -2.f; z <= 2.f; z += 1.f)
+        if (d2 > depth + z) acc += 1.f;
+    }
+  }
+  acc /= 125.f;
+  return 1.f - (1.f - acc) / kSsaoDiv;
+}
+
+// bilinear_interpolate, cl2.cl:2286-2302.
+__device__ __forceinline__ float bilerp(float cx, float cy, float v0, float v1, float v2, float v3) {
+  float mx = cx - 0.5f, my = cy - 0.5f;
+  float ux = mx - floorf(mx), uy = my - floorf(my);
+  float bx = 1.0f - ux, by = 1.0f - uy;
+  return (v0 * bx + v1 * ux) * by + (v2 * bx + v3 * ux) * uy;
+}
+
+// generate_hard_occlusion, cl2.cl:2536-2701 (SMOOTH_SHADOWS).
+template <bool kCount>
+__device__ __forceinline__ float hard_occlusion(const ShadeArgs& a, f3 lpos, f3 normal, f3 ptl,
+                                                const uint32_t* buf, uint32_t* touch_bits, int face,
+                                                f3 gpos, int shnum) {
+  const uint32_t L = a.L;
+  ptl = fast_normalize(ptl);
+  f3 local_pos = rot(gpos, lpos, a.face[face]);
+  float lf = (float)L;
+  f3 post = depth_project(local_pos, lf, lf, lf / 2.0f);
+  float dpth = post.z;
+  size_t map_off = ((size_t)face + (size_t)shnum * 6) * L * L;
+  const uint32_t* map = buf + map_off;
+  float px = clampf(post.x, 3.f, lf - 4.f);
+  float py = clampf(post.y, 3.f, lf - 4.f);
+  int ix = (int)px, iy = (int)py;
+  float acos_res = rational_acos(clampf(dot(normal, ptl), 0.05f, 0.95f));
+  float bias = kShadowBias * tan_f(acos_res);
+  bias = clampf(bias, 0.1f * kShadowBias, (float)kShadowBias);
+  uint32_t cond = 0;   // 4x4 taps, bit (y+1)*4 + (x+1)
+#pragma unroll
+  for (int y = -1; y <= 2; y++)
+#pragma unroll
+    for (int x = -1; x <= 2; x++) {
+      size_t t = (size_t)(iy + y) * L + (size_t)(ix + x);
+      touch<kCount>(touch_bits, map_off + t);
+      float ldp = ((float)map[t] / kMulint) * kDepthFar;
+      cond |= (dpth > ldp + bias ? 1u : 0u) << ((y + 1) * 4 + (x + 1));
+    }
+  float shadow = 0.f;
+#pragma unroll
+  for (int y = -1; y <= 1; y++)
+#pragma unroll
+    for (int x = -1; x <= 1; x++) {
+      float v0 = (float)((cond >> ((y + 1) * 4 + x + 1)) & 1u);
+      float v1 = (float)((cond >> ((y + 1) * 4 + x + 2)) & 1u);
+      float v2 = (float)((cond >> ((y + 2) * 4 + x + 1)) & 1u);
+      float v3 = (float)((cond >> ((y + 2) * 4 + x + 2)) & 1u);
+      shadow += bilerp(px + 0.5f + (float)x, py + 0.5f + (float)y, v0, v1, v2, v3);
+    }
+  shadow /= 9.f;
+  return shadow;
+}
+
+__device__ __forceinline__ f4 vcol(uint32_t c) {
+  return f4{(float)(c >> 24) / 255.f, (float)((c >> 16) & 0xFF) / 255.f, (float)((c >> 8) & 0xFF) / 255.f,
+            (float)(c & 0xFF) / 255.f};
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(256) k_shade(ShadeArgs a) {
+  const uint32_t W = a.v.width, H = a.v.height;
+  const uint32_t tiles_x = (W + 15) / 16;
+  const uint32_t tile = blockIdx.x;
+  const int x = (int)((tile % tiles_x) * 16 + (threadIdx.x & 15));
+  const int y = (int)((tile / tiles_x) * 16 + (threadIdx.x >> 4));
+  if (x >= (int)W || y >= (int)H) return;
+  if (a.counters[kOverflow]) return;
+  const size_t pix = (size_t)y * W + x;
+  const uint32_t ft = a.depth[pix];
+  const unsigned long long key = a.idkey[pix];
+  if (ft == 0xFFFFFFFFu || key == 0ull) {
+    a.rgba[pix] = quant8(a.v.clear[0]) | (quant8(a.v.clear[1]) << 8) | (quant8(a.v.clear[2]) << 16) |
+                  (quant8(a.v.clear[3]) << 24);
+    a.normals[pix] = 0u;
+    return;
+  }
+  const float fov = a.v.fov;
+  const CutTri ct = a.cuts[(uint32_t)key];
+  const uint32_t tri = (uint32_t)(key >> 32) - 1u;
+  const uint32_t tri_global = tri >> 1;
+  const TriPos T = a.tri_pos[tri_global];
+  const TriAttr TA = a.tri_attr[tri_global];
+  const ObjDesc G = a.objs[ct.oid];
+  const f3 camera_pos = mk3(a.v.cpos[0], a.v.cpos[1], a.v.cpos[2]);
+  const f3 p1 = mk3(T.p[0], T.p[1], T.p[2]) * G.scale;
+  const f3 p2 = mk3(T.p[3], T.p[4], T.p[5]) * G.scale;
+  const f3 p3 = mk3(T.p[6], T.p[7], T.p[8]) * G.scale;
+  const f3 wpos = mk3(G.wp[0], G.wp[1], G.wp[2]);
+  const f4 q = f4{G.q[0], G.q[1], G.q[2], G.q[3]};
+
+  const float actual_depth = ((float)ft / kMulint) * kDepthFar;
+  f3 local_position = mk3(((float)x - (float)W / 2.0f) * actual_depth / fov,
+                          ((float)y - (float)H / 2.0f) * actual_depth / fov, actual_depth);
+  f3 global_position = back_rot(local_position, a.v.cam) + camera_pos;
+  f3 object_local = back_rot_quat(global_position - wpos, q);
+  float l1, l2, l3;
+  barycentric(object_local, p1, p2, p3, l1, l2, l3);
+  const float vtx = fmaf(TA.vt[0], l1, fmaf(TA.vt[2], l2, TA.vt[4] * l3));
+  const float vty = fmaf(TA.vt[1], l1, fmaf(TA.vt[3], l2, TA.vt[5] * l3));
+  f3 normal = mk3(fmaf(TA.n[0], l1, fmaf(TA.n[3], l2, TA.n[6] * l3)),
+                  fmaf(TA.n[1], l1, fmaf(TA.n[4], l2, TA.n[7] * l3)),
+                  fmaf(TA.n[2], l1, fmaf(TA.n[5], l2, TA.n[8] * l3)));
+  normal = rot_quat(normal, q);
+
+  // get_vtdiff, cl2.cl:5691-5760.
+  float vdx, vdy;
+  {
+    const float fx = (float)x, fy = (float)y;
+    f3 xpv = mk3(roundf(ct.x[0]), roundf(ct.x[1]), roundf(ct.x[2]));
+    f3 ypv = mk3(roundf(ct.y[0]), roundf(ct.y[1]), roundf(ct.y[2]));
+    f3 depths = mk3(1.0f / ct.z[0], 1.0f / ct.z[1], 1.0f / ct.z[2]);
+    float DA, DB, DC;
+    plane_const(depths, xpv, ypv, ct.rconst, DA, DB, DC);
+    float dmx = fmaf(DA, fx + 1, fmaf(DB, fy, DC));
+    float dmy = fmaf(DA, fx, fmaf(DB, fy + 1, DC));
+    f3 lmx = mk3((fx + 1 - (float)W / 2.f) / fov, (fy - (float)H / 2.f) / fov, 1) / dmx;
+    f3 lmy = mk3((fx - (float)W / 2.f) / fov, (fy + 1 - (float)H / 2.f) / fov, 1) / dmy;
+    f3 gmx = back_rot_quat((back_rot(lmx, a.v.cam) + camera_pos) - wpos, q);
+    f3 gmy = back_rot_quat((back_rot(lmy, a.v.cam) + camera_pos) - wpos, q);
+    float lx1, lx2, lx3, ly1, ly2, ly3;
+    barycentric(gmx, p1, p2, p3, lx1, lx2, lx3);
+    barycentric(gmy, p1, p2, p3, ly1, ly2, ly3);
+    float vtxx = fmaf(TA.vt[0], lx1, fmaf(TA.vt[2], lx2, TA.vt[4] * lx3));
+    float vtxy = fmaf(TA.vt[1], lx1, fmaf(TA.vt[3], lx2, TA.vt[5] * lx3));
+    float vtyx = fmaf(TA.vt[0], ly1, fmaf(TA.vt[2], ly2, TA.vt[4] * ly3));
+    float vtyy = fmaf(TA.vt[1], ly1, fmaf(TA.vt[3], ly2, TA.vt[5] * ly3));
+    const float mip_bias = 1.f / kMipBias;
+    vdx = (fabsf(vtxx - vtx) + fabsf(vtyx - vtx)) * mip_bias;
+    vdy = (fabsf(vtxy - vty) + fabsf(vtyy - vty)) * mip_bias;
+  }
+
+  f4 col;
+  if (TA.col[0] == 0u) {
+    col = texture_filter_diff<kCount>(a, vtx, vty, vdx, vdy, (int)G.tid);
+  } else {
+    f4 c0 = vcol(TA.col[0]), c1 = vcol(TA.col[1]), c2 = vcol(TA.col[2]);
+    col = f4{fmaf(c0.x, l1, fmaf(c1.x, l2, c2.x * l3)), fmaf(c0.y, l1, fmaf(c1.y, l2, c2.y * l3)),
+             fmaf(c0.z, l1, fmaf(c1.z, l2, c2.z * l3)), fmaf(c0.w, l1, fmaf(c1.w, l2, c2.w * l3))};
+  }
+  const bool linear = a.v.linear != 0;
+  if (linear) { col.x = gamma_fwd(col.x); col.y = gamma_fwd(col.y); col.z = gamma_fwd(col.z); }
+
+  uint32_t seed2 = xorshift(wang_hash((uint32_t)x + (uint32_t)y * W * H));
+  uint32_t seed3 = xorshift(seed2);
+  uint32_t seed4 = xorshift(seed3);
+  f3 rseed = mk3(((float)seed2 / kMulint - 0.5f) * 2, ((float)seed3 / kMulint - 0.5f) * 2,
+                 ((float)seed4 / kMulint - 0.5f) * 2);
+
+  const f3 l2p = fast_normalize(camera_pos - global_position);
+  const int ff = G.flags;
+  const bool two_sided = (ff & OCR_FEATURE_TWO_SIDED) > 0;
+  const bool recv_dyn = !((ff & OCR_FEATURE_DOES_NOT_RECEIVE_DYNAMIC_SHADOWS) > 0);
+  const bool is_front = front_facing(mk3(ct.x[0], ct.y[0], ct.z[0]), mk3(ct.x[1], ct.y[1], ct.z[1]),
+                                     mk3(ct.x[2], ct.y[2], ct.z[2]));
+  if (!is_front && two_sided) normal = -normal;
+  const float occ_ssao = ssao(a, x, y);
+  normal = fast_normalize(normal);
+  const f3 lighting_normal = fast_normalize(normal + rseed / 100.f);
+  const float ambient = linear ? gamma_fwd(kAmbient) : kAmbient;
+
+  int shnum = 0, static_num = 0;
+  f3 diffuse_sum = mk3(0, 0, 0), specular_sum = mk3(0, 0, 0);
+  for (uint32_t i = 0; i < a.nlights; i++) {
+    const ocr_light l = a.lights[i];
+    const f3 lpos = mk3(l.pos[0], l.pos[1], l.pos[2]);
+    f3 ptl = lpos - global_position;
+    float occlusion = 1;
+    if (l.shadow && l.is_static) {
+      int face = cubeface(global_position, lpos);
+      occlusion = 1.f - hard_occlusion<kCount>(a, lpos, normal, ptl, a.static_maps, a.static_touch, face,
+                                              global_position, static_num);
+      static_num++;
+    }
+    float distance = fast_length(ptl);
+    float t = (distance / l.radius) + 1;
+    float illumination = l.brightness / (t * t);
+    const float cutoff = 0.1f;
+    illumination -= cutoff;
+    illumination *= 1.f / (1.f - cutoff);
+    if (illumination <= 0) continue;
+    f3 light_col = mk3(l.col[0], l.col[1], l.col[2]);
+    if (linear) light_col = mk3(gamma_fwd(light_col.x), gamma_fwd(light_col.y), gamma_fwd(light_col.z));
+    if (l.shadow && recv_dyn) {
+      int face = cubeface(global_position, lpos);
+      float dyn = 1.f - hard_occlusion<kCount>(a, lpos, normal, ptl, a.shadow_maps, a.shadow_touch, face,
+                                              global_position, shnum);
+      occlusion = fminf(occlusion, dyn);
+      shnum++;
+    }
+    ptl = fast_normalize(ptl);
+    float light = dot(ptl, lighting_normal);
+    light *= occlusion;
+    light = fmaxf(light, 0.f);
+    float diffuse = (1.0f - ambient) * light;
+    diffuse_sum = diffuse_sum + light_col * ((diffuse + ambient) * l.diffuse * G.diffuse * illumination);
+    const f3 Hv = fast_normalize(l2p + ptl);
+    const f3 N = normal;
+    const float kS = 0.4f;
+    float ndh = fmaxf(0.f, dot(N, Hv));
+    float ndv = fmaxf(0.f, dot(N, l2p));
+    float vdh = fmaxf(0.f, dot(l2p, Hv));
+    float ndl = fmaxf(0.f, dot(N, ptl));
+    const float F0 = 0.4f;
+    float fresnel = F0 + (1 - F0) * powr5(1.f - vdh);
+    float rough = clampf(1.f - G.specular, 0.001f, 10.f);
+    const float gauss_constant = 0.8346f;
+    float alpha = rational_acos(ndh);
+    float microfacet = gauss_constant * exp_f(-alpha * alpha / (rough * rough));
+    float sv = 2 * ndh / vdh;
+    float c1 = sv * ndv;
+    float c2 = sv * ndl;
+    float geometric = min3f(1.f, c1, c2);
+    float spec = (fresnel * microfacet * geometric) / (kPi * ndv);
+    specular_sum = specular_sum + (light_col * (spec * kS * illumination)) * G.spec_mult;
+    specular_sum = mk3(fmaxf(specular_sum.x, 0.f), fmaxf(specular_sum.y, 0.f), fmaxf(specular_sum.z, 0.f));
+    specular_sum = specular_sum * occlusion;
+  }
+  specular_sum = specular_sum * occ_ssao;
+  diffuse_sum = diffuse_sum * occ_ssao;
+  const float rsc = 0.7f;
+  f3 colclamp = (mk3(col.x, col.y, col.z) + mk3(0, 0, 0)) + specular_sum * rsc;
+  f3 fc = mk3(fmaf(colclamp.x, diffuse_sum.x, specular_sum.x * (1.f - rsc)),
+              fmaf(colclamp.y, diffuse_sum.y, specular_sum.y * (1.f - rsc)),
+              fmaf(colclamp.z, diffuse_sum.z, specular_sum.z * (1.f - rsc)));
+  if (linear) fc = mk3(gamma_inv(fc.x), gamma_inv(fc.y), gamma_inv(fc.z));
+  a.rgba[pix] = quant8(fc.x) | (quant8(fc.y) << 8) | (quant8(fc.z) << 16) | (quant8(col.w) << 24);
+  a.normals[pix] = encode_normal(normal);
+}
+
+static ShadeArgs make_args(const Scene* sc, const ViewParams& v, Frame* f) {
+  ShadeArgs a;
+  a.v = v;
+  a.depth = f->depth;
+  a.idkey = f->idkey;
+  a.cuts = f->cuts;
+  a.tri_pos = sc->tri_pos;
+  a.tri_attr = sc->tri_attr;
+  a.objs = sc->objs;
+  a.pages = sc->atlas.pages;
+  a.nums = sc->atlas.nums;
+  a.sizes = sc->atlas.sizes;
+  a.mip_start = sc->atlas.mip_start;
+  a.lights = sc->lights;
+  a.nlights = sc->nlights;
+  a.shadow_maps = sc->shadow_maps;
+  a.static_maps = sc->static_maps;
+  a.L = sc->lbuf_dim;
+  a.counters = f->counters;
+  a.rgba = f->rgba;
+  a.normals = f->normals;
+  for (int i = 0; i < 6; i++) a.face[i] = sc->ctx->face_sc[i];
+  a.tex_touch = a.shadow_touch = a.static_touch = nullptr;
+  return a;
+}
+
+hipError_t launch_shade(const Scene* sc, const ViewParams& v, Frame* f, hipStream_t s) {
+  ShadeArgs a = make_args(sc, v, f);
+  unsigned tiles = ((v.width + 15) / 16) * ((v.height + 15) / 16);
+  hipLaunchKernelGGL(k_shade<false>, dim3(tiles), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_shade_count(const Scene* sc, const ViewParams& v, Frame* f, uint32_t* tex_bits,
+                              uint32_t* shadow_bits, uint32_t* static_bits, hipStream_t s) {
+  ShadeArgs a = make_args(sc, v, f);
+  a.tex_touch = tex_bits;
+  a.shadow_touch = shadow_bits;
+  a.static_touch = static_bits;
+  unsigned tiles = ((v.width + 15) / 16) * ((v.height + 15) / 16);
+  hipLaunchKernelGGL(k_shade<true>, dim3(tiles), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace ocr

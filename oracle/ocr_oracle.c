@@ -392,7 +392,7 @@ static void prearrange_all(const oracle_scene* sc, const ocr_view* v, const orac
         cbase[i] = ncut; fbase[i] = nfrag;
         ncut += (uint32_t)ts[i].num;
         for (int k = 0; k < ts[i].num; k++)
-            if (ts[i].keep[k]) { nfrag += ts[i].nfrag[k]; nsurv++; }
+            if (ts[i].keep[k]) { nfrag += ts[i].nfrag[k]; if (ts[i].nfrag[k]) nsurv++; }
     }
     pl->cuts = (cut_tri*)malloc(sizeof(cut_tri) * (ncut ? ncut : 1));
     pl->frags = (frag_rec*)malloc(sizeof(frag_rec) * (nfrag ? nfrag : 1));

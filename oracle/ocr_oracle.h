@@ -57,7 +57,7 @@ typedef struct oracle_out {
     uint32_t* tri_id;     /* W*H, may be NULL */
     uint8_t*  rgba8;      /* W*H*4, may be NULL (skips shading) */
     uint16_t* normals;    /* W*H*2, may be NULL */
-    uint32_t  n_cut;      /* cut triangles that survived cull */
+    uint32_t  n_cut;      /* cut triangles that survived cull with >= 1 fragment */
     uint32_t  n_frag;     /* fragment records */
     uint64_t  n_samples;  /* covered samples (point_in_tri passes) in the depth pass */
     uint32_t  n_visible_tris;  /* distinct triangles in the id buffer */
