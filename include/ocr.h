@@ -126,12 +126,13 @@ typedef struct ocr_tex_src {
 } ocr_tex_src;
 
 /* Counters of the last render of a frame, and per-pass device times when
- * profiling is on (hipEvent pairs on the render stream). */
+ * profiling is on: hipEvent pairs on the render stream, averaged over every
+ * render since profiling was (re)enabled. */
 typedef struct ocr_frame_stats {
     uint32_t n_cut;             /* cut (post clip/cull) triangles            */
     uint32_t n_frag;            /* fragment records                          */
     uint32_t overflow;          /* 1 if a work list hit its capacity         */
-    uint32_t _pad;
+    uint32_t n_profiled;        /* renders the pass times are averaged over  */
     float    ms_clear;
     float    ms_prearrange;
     float    ms_depth;
@@ -193,7 +194,13 @@ int  ocr_scene_read_shadow_maps(ocr_scene* s, uint32_t* dynamic_maps,
 
 int  ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** out);
 void ocr_frame_destroy(ocr_frame* f);
-/* Record per-pass hipEvent timings on every render of this frame. */
+/* Sets the fragment-list capacity of the frame's next render (the
+ * reference's fixed g_tid_buf_max_len, engine.cpp:600-618).  A render that
+ * needs more sets the overflow word; ocr_frame_read then reports
+ * OCR_ERR_OVERFLOW and the next render grows the list to fit. */
+int  ocr_frame_reserve(ocr_frame* f, uint32_t max_fragments);
+/* Record per-pass hipEvent timings on every render of this frame (up to
+ * 1024 renders are averaged; enabling again restarts the average). */
 int  ocr_frame_set_profiling(ocr_frame* f, int enable);
 
 /* engine::draw_bulk_objs_n -> render_tris (engine.cpp:1794-2161):
@@ -213,6 +220,12 @@ int  ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id,
 int  ocr_frame_device_buffers(ocr_frame* f, void** depth, void** idkey,
                               void** rgba8, void** normals);
 int  ocr_frame_get_stats(ocr_frame* f, void* stream, ocr_frame_stats* out);
+/* Instrumented re-run of the shading pass of the last render: counts the
+ * distinct atlas texels and cube-map texels it reads (the X_tex / X_shadow
+ * terms of the shading pass's algorithmic byte count).  Allocates scratch. */
+int  ocr_frame_count_texels(ocr_scene* s, const ocr_view* view, ocr_frame* f,
+                            uint64_t* atlas_texels, uint64_t* shadow_texels,
+                            void* stream);
 
 #ifdef __cplusplus
 }

@@ -69,7 +69,7 @@ FRAME_STATS = np.dtype([
     ("n_cut", "<u4"),
     ("n_frag", "<u4"),
     ("overflow", "<u4"),
-    ("_pad", "<u4"),
+    ("n_profiled", "<u4"),
     ("ms_clear", "<f4"),
     ("ms_prearrange", "<f4"),
     ("ms_depth", "<f4"),

@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libocr.so")
+CALIB_LIB = os.path.join(HERE, "libocr_calib.so")
 SOURCES = ["runtime.hip", "raster.hip", "shade.hip", "atlas.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -54,6 +55,10 @@ def build(verbose=False, jobs=4):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
+    # the calibration copy kernel used by profiling runs (separate library)
+    tmp = CALIB_LIB + ".tmp"
+    _run([HIPCC, *FLAGS, "-shared", os.path.join(CSRC, "calib.hip"), "-o", tmp], verbose)
+    os.replace(tmp, CALIB_LIB)
     return LIB
 
 

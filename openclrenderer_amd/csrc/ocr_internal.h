@@ -126,8 +126,12 @@ struct Frame {
   uint32_t* counters = nullptr;
   uint32_t cap_cut = 0, cap_frag = 0;
   uint32_t want_frag = 0;            // capacity asked for after an overflow
+  bool exact_frag = false;           // want_frag set by ocr_frame_reserve
   bool profiling = false;
-  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  static constexpr int kEvents = 6;       // clear, prearrange, depth, id, shade, end
+  static constexpr int kRing = 1024;      // renders averaged per profiling window
+  std::vector<hipEvent_t> ev;             // kRing x kEvents, created on first use
+  int n_profiled = 0;
   bool rendered = false;
 };
 
@@ -147,6 +151,8 @@ hipError_t launch_prearrange(const Scene* sc, const ViewParams& v, Frame* f, hip
 hipError_t launch_depth(const ViewParams& v, Frame* f, hipStream_t s);
 hipError_t launch_id(const ViewParams& v, Frame* f, hipStream_t s);
 hipError_t launch_shade(const Scene* sc, const ViewParams& v, Frame* f, hipStream_t s);
+hipError_t launch_shade_count(const Scene* sc, const ViewParams& v, Frame* f, uint32_t* tex_bits,
+                              uint32_t* shadow_bits, uint32_t* static_bits, hipStream_t s);
 hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, uint32_t* map,
                                hipStream_t s);
 hipError_t launch_atlas_upload(const Atlas& a, uint32_t tid, const uint8_t* src, uint32_t w,

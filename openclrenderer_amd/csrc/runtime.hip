@@ -454,12 +454,6 @@ int ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** 
     ocr_frame_destroy(f);
     return rc;
   }
-  for (int i = 0; i < 6; i++) {
-    if (hipEventCreate(&f->ev[i]) != hipSuccess) {
-      ocr_frame_destroy(f);
-      return fail(OCR_ERR_HIP, "ocr_frame_create: hipEventCreate");
-    }
-  }
   hipError_t e = hipMemset(f->counters, 0, 4 * kCounterWords);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
@@ -480,21 +474,39 @@ void ocr_frame_destroy(ocr_frame* f) {
   dev_free(f->cuts);
   dev_free(f->frags);
   dev_free(f->counters);
-  for (int i = 0; i < 6; i++)
-    if (f->ev[i]) (void)hipEventDestroy(f->ev[i]);
+  for (hipEvent_t e : f->ev)
+    if (e) (void)hipEventDestroy(e);
   delete f;
 }
 
 int ocr_frame_set_profiling(ocr_frame* f, int enable) {
   if (!f) return fail(OCR_ERR_INVALID, "ocr_frame_set_profiling: null frame");
   f->profiling = enable != 0;
+  f->n_profiled = 0;
+  if (f->profiling && f->ev.empty()) {
+    OCR_HIP_CHECK(hipSetDevice(f->ctx->device));
+    f->ev.assign((size_t)Frame::kRing * Frame::kEvents, nullptr);
+    for (auto& e : f->ev) OCR_HIP_CHECK(hipEventCreate(&e));
+  }
+  return OCR_OK;
+}
+
+int ocr_frame_reserve(ocr_frame* f, uint32_t max_fragments) {
+  if (!f || max_fragments == 0) return fail(OCR_ERR_INVALID, "ocr_frame_reserve: bad argument");
+  OCR_HIP_CHECK(hipSetDevice(f->ctx->device));
+  OCR_HIP_CHECK(hipDeviceSynchronize());
+  dev_free(f->frags);
+  f->cap_frag = 0;
+  f->want_frag = max_fragments;
+  f->exact_frag = true;
   return OCR_OK;
 }
 
 static int ensure_frame_lists(ocr_frame* f, uint32_t ntri) {
   uint64_t want_cut = (uint64_t)ntri * 2 + 64;
-  uint64_t want_frag = std::max<uint64_t>((uint64_t)ntri * 2 + (uint64_t)f->width * f->height / 64 + 65536,
-                                          f->want_frag);
+  uint64_t want_frag = f->exact_frag ? (uint64_t)f->want_frag
+                                     : std::max<uint64_t>((uint64_t)ntri * 2 + (uint64_t)f->width * f->height / 64 + 65536,
+                                                          f->want_frag);
   want_cut = std::min<uint64_t>(want_cut, 0xFFFFFFF0ull);
   want_frag = std::min<uint64_t>(want_frag, 0xFFFFFFF0ull);
   if (f->cuts && f->cap_cut >= want_cut && f->frags && f->cap_frag >= want_frag) return OCR_OK;
@@ -535,17 +547,22 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
     rc = ocr_lights_upload(s, nullptr, 0, s->lbuf_dim);
     if (rc) return rc;
   }
-  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[0], st));
+  const bool prof = f->profiling && f->n_profiled < Frame::kRing;
+  hipEvent_t* ev = prof ? &f->ev[(size_t)f->n_profiled * Frame::kEvents] : nullptr;
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[0], st));
   OCR_HIP_CHECK(launch_clear_frame(f, st));
-  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[1], st));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[1], st));
   OCR_HIP_CHECK(launch_prearrange(s, vp, f, st));
-  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[2], st));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[2], st));
   OCR_HIP_CHECK(launch_depth(vp, f, st));
-  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[3], st));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], st));
   OCR_HIP_CHECK(launch_id(vp, f, st));
-  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[4], st));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], st));
   OCR_HIP_CHECK(launch_shade(s, vp, f, st));
-  if (f->profiling) OCR_HIP_CHECK(hipEventRecord(f->ev[5], st));
+  if (prof) {
+    OCR_HIP_CHECK(hipEventRecord(ev[5], st));
+    f->n_profiled++;
+  }
   f->rendered = true;
   return OCR_OK;
 }
@@ -560,16 +577,71 @@ int ocr_frame_get_stats(ocr_frame* f, void* stream, ocr_frame_stats* out) {
   out->n_cut = c[kCutCount];
   out->n_frag = c[kFragCount];
   out->overflow = c[kOverflow] ? 1u : 0u;
-  if (f->profiling && f->rendered) {
-    float ms[5] = {0, 0, 0, 0, 0};
-    for (int i = 0; i < 5; i++) OCR_HIP_CHECK(hipEventElapsedTime(&ms[i], f->ev[i], f->ev[i + 1]));
-    out->ms_clear = ms[0];
-    out->ms_prearrange = ms[1];
-    out->ms_depth = ms[2];
-    out->ms_id = ms[3];
-    out->ms_shade = ms[4];
-    OCR_HIP_CHECK(hipEventElapsedTime(&out->ms_total, f->ev[0], f->ev[5]));
+  out->n_profiled = (uint32_t)f->n_profiled;
+  if (f->n_profiled > 0) {
+    double sum[6] = {0, 0, 0, 0, 0, 0};
+    for (int r = 0; r < f->n_profiled; r++) {
+      hipEvent_t* ev = &f->ev[(size_t)r * Frame::kEvents];
+      float ms = 0.f;
+      for (int i = 0; i < 5; i++) {
+        OCR_HIP_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        sum[i] += ms;
+      }
+      OCR_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[5]));
+      sum[5] += ms;
+    }
+    const double n = (double)f->n_profiled;
+    out->ms_clear = (float)(sum[0] / n);
+    out->ms_prearrange = (float)(sum[1] / n);
+    out->ms_depth = (float)(sum[2] / n);
+    out->ms_id = (float)(sum[3] / n);
+    out->ms_shade = (float)(sum[4] / n);
+    out->ms_total = (float)(sum[5] / n);
   }
+  return OCR_OK;
+}
+
+static uint64_t popcount_bits(const std::vector<uint32_t>& v) {
+  uint64_t n = 0;
+  for (uint32_t w : v) n += (uint64_t)__builtin_popcount(w);
+  return n;
+}
+
+int ocr_frame_count_texels(ocr_scene* s, const ocr_view* view, ocr_frame* f, uint64_t* atlas_texels,
+                           uint64_t* shadow_texels, void* stream) {
+  if (!s || !view || !f || !f->rendered) return fail(OCR_ERR_INVALID, "ocr_frame_count_texels: render first");
+  if (view->width != f->width || view->height != f->height)
+    return fail(OCR_ERR_INVALID, "ocr_frame_count_texels: view size differs from the frame");
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  const size_t tex_words = ((size_t)s->atlas.npages * OCR_ATLAS_DIM * OCR_ATLAS_DIM + 31) / 32;
+  const size_t per = (size_t)6 * s->lbuf_dim * s->lbuf_dim;
+  const size_t sh_words = (per * std::max<uint32_t>(s->n_shadow, 1) + 31) / 32;
+  const size_t st_words = (per * std::max<uint32_t>(s->n_static, 1) + 31) / 32;
+  uint32_t *tb = nullptr, *sb = nullptr, *tt = nullptr;
+  int rc;
+  if ((rc = dev_alloc(&tb, tex_words, "texel bitmap")) || (rc = dev_alloc(&sb, sh_words, "shadow bitmap")) ||
+      (rc = dev_alloc(&tt, st_words, "static bitmap"))) {
+    dev_free(tb); dev_free(sb); dev_free(tt);
+    return rc;
+  }
+  hipError_t e = launch_fill_u32(tb, 0u, tex_words, st);
+  if (e == hipSuccess) e = launch_fill_u32(sb, 0u, sh_words, st);
+  if (e == hipSuccess) e = launch_fill_u32(tt, 0u, st_words, st);
+  ViewParams vp = make_view_params(view);
+  if (e == hipSuccess) e = launch_shade_count(s, vp, f, tb, sb, tt, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  std::vector<uint32_t> h1, h2, h3;
+  if (e == hipSuccess) {
+    h1.resize(tex_words); h2.resize(sh_words); h3.resize(st_words);
+    e = hipMemcpy(h1.data(), tb, 4 * tex_words, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(h2.data(), sb, 4 * sh_words, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(h3.data(), tt, 4 * st_words, hipMemcpyDeviceToHost);
+  }
+  dev_free(tb); dev_free(sb); dev_free(tt);
+  if (e != hipSuccess) return fail(OCR_ERR_HIP, std::string("ocr_frame_count_texels: ") + hipGetErrorString(e));
+  if (atlas_texels) *atlas_texels = popcount_bits(h1);
+  if (shadow_texels) *shadow_texels = popcount_bits(h2) + popcount_bits(h3);
   return OCR_OK;
 }
 
@@ -584,6 +656,7 @@ int ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id, uint8_t* rgb
   if (c[kOverflow]) {
     // Grow the lists so the next render of this frame fits.
     f->want_frag = (uint32_t)std::min<uint64_t>((uint64_t)c[kFragCount] + c[kFragCount] / 4 + 1024, 0xFFFFFFF0ull);
+    f->exact_frag = false;
     return fail(OCR_ERR_OVERFLOW, "ocr_frame_read: fragment list overflowed; re-render the frame");
   }
   size_t npix = (size_t)f->width * f->height;

@@ -21,7 +21,7 @@ EXPORTS = [
     "ocr_shadow_pass", "ocr_scene_destroy", "ocr_scene_atlas_info", "ocr_scene_read_atlas",
     "ocr_scene_read_shadow_maps", "ocr_frame_create", "ocr_frame_destroy",
     "ocr_frame_set_profiling", "ocr_render", "ocr_frame_read", "ocr_frame_device_buffers",
-    "ocr_frame_get_stats",
+    "ocr_frame_get_stats", "ocr_frame_count_texels", "ocr_frame_reserve",
 ]
 
 
@@ -37,7 +37,7 @@ class TexSrc(C.Structure):
 
 class FrameStats(C.Structure):
     _fields_ = [("n_cut", C.c_uint32), ("n_frag", C.c_uint32), ("overflow", C.c_uint32),
-                ("_pad", C.c_uint32), ("ms_clear", C.c_float), ("ms_prearrange", C.c_float),
+                ("n_profiled", C.c_uint32), ("ms_clear", C.c_float), ("ms_prearrange", C.c_float),
                 ("ms_depth", C.c_float), ("ms_id", C.c_float), ("ms_shade", C.c_float),
                 ("ms_total", C.c_float)]
 
@@ -74,10 +74,12 @@ def load_library(path=LIB_PATH):
     L.ocr_frame_destroy.argtypes = [vp]
     L.ocr_frame_destroy.restype = None
     L.ocr_frame_set_profiling.argtypes = [vp, i32]
+    L.ocr_frame_reserve.argtypes = [vp, u32]
     L.ocr_render.argtypes = [vp, vp, vp, vp]
     L.ocr_frame_read.argtypes = [vp, vp, vp, vp, vp, vp]
     L.ocr_frame_device_buffers.argtypes = [vp, pp, pp, pp, pp]
     L.ocr_frame_get_stats.argtypes = [vp, vp, C.POINTER(FrameStats)]
+    L.ocr_frame_count_texels.argtypes = [vp, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp]
     _lib = L
     return L
 
@@ -211,7 +213,19 @@ class Frame:
     def stats(self, stream=None):
         s = FrameStats()
         check(self.lib.ocr_frame_get_stats(self.h, stream, C.byref(s)))
-        return {k: getattr(s, k) for k, _ in FrameStats._fields_ if k != "_pad"}
+        return {k: getattr(s, k) for k, _ in FrameStats._fields_}
+
+    def reserve(self, max_fragments):
+        check(self.lib.ocr_frame_reserve(self.h, max_fragments))
+
+    def set_profiling(self, on=True):
+        check(self.lib.ocr_frame_set_profiling(self.h, 1 if on else 0))
+
+    def count_texels(self, gscene, view, stream=None):
+        v = np.ascontiguousarray(view, dtype=abi.VIEW)
+        a, s = C.c_uint64(), C.c_uint64()
+        check(self.lib.ocr_frame_count_texels(gscene.h, _p(v), self.h, C.byref(a), C.byref(s), stream))
+        return a.value, s.value
 
     def device_buffers(self):
         ptrs = [C.c_void_p() for _ in range(4)]

@@ -453,3 +453,47 @@ def config4_scene_sizes(n_scenes=8, seed=SEED_BASE + 4):
     rng = SplitMix64(seed ^ 0x5EED)
     return [int(round(math.exp(math.log(50_000) + (math.log(2_000_000) - math.log(50_000)) * rng.uniform())))
             for _ in range(n_scenes)]
+
+
+def random_scene(seed, n_tris=60, width=64, height=48, n_objs=4, near_cross=True,
+                 duplicate_ties=True, vertex_colours=True, two_sided=True, shadow=0):
+    """Small seeded scenes for parity tests: random triangles in front of a
+    camera at the origin looking +z, some straddling the near plane, some
+    duplicated in a later object (coplanar depth ties), some objects two-sided
+    and some triangles vertex-coloured."""
+    rng = np.random.default_rng(seed)
+    objs = []
+    for o in range(n_objs):
+        flags = abi.FEATURE_TWO_SIDED if (two_sided and o % 2 == 1) else 0
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        objs.append(_desc((rng.uniform(-50, 50), rng.uniform(-50, 50), rng.uniform(200, 600)),
+                          tuple(q), float(rng.uniform(0.5, 2.0)), flags=flags))
+    tris = np.zeros(n_tris, dtype=abi.TRIANGLE)
+    for t in range(n_tris):
+        oid = int(rng.integers(0, n_objs))
+        c = rng.uniform(-120, 120, 3)
+        if near_cross and t % 7 == 0:
+            c[2] = -rng.uniform(150, 450)   # pulls a vertex towards / behind the near plane
+        for v in range(3):
+            p = c + rng.normal(scale=rng.uniform(5, 80), size=3)
+            tris["v"]["pos"][t, v, :3] = p
+            n = rng.normal(size=3)
+            tris["v"]["normal"][t, v, :3] = n / np.linalg.norm(n)
+            tris["v"]["vt"][t, v] = rng.uniform(-0.5, 1.5, 2)
+        tris["v"]["object_id"][t, 0] = oid
+        if vertex_colours and t % 5 == 3:
+            tris["v"]["vertex_col"][t] = rng.integers(1, 2 ** 32 - 1, 3, dtype=np.uint64).astype(np.uint32)
+    if duplicate_ties and n_tris >= 4:
+        for t in range(0, n_tris // 4, 3):
+            src = n_tris - 1 - t
+            tris[t]["v"]["pos"] = tris[src]["v"]["pos"]
+            tris["v"]["object_id"][t, 0] = tris["v"]["object_id"][src, 0]
+    objs = np.concatenate(objs)
+    lights = np.concatenate([make_light((rng.uniform(-300, 300), rng.uniform(-300, 300), rng.uniform(-100, 100)),
+                                        shadow=shadow, radius=20000.0),
+                             make_light((0.0, -400.0, 300.0), col=(0.3, 0.5, 1.0), radius=900.0)])
+    tex = value_noise_texture(64, int(rng.integers(0, 2 ** 31)), lattice=4)
+    sc = Scene(tris, objs, lights, [tex], 64, f"random{seed}")
+    view = make_view(width, height, (0.0, 0.0, -10.0), c_rot=tuple(rng.uniform(-0.2, 0.2, 3)), hfov=70.0)
+    return sc, view

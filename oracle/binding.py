@@ -74,6 +74,9 @@ class Oracle:
         L.oracle_atlas_fill.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
                                         C.c_void_p]
         L.oracle_face_sincos.argtypes = [C.c_void_p]
+        L.oracle_cut_list.argtypes = [C.POINTER(OracleScene), C.c_void_p, C.POINTER(SinCos), C.c_void_p,
+                                      C.c_void_p, C.c_int]
+        L.oracle_cut_list.restype = C.c_int
         f, u, i = C.c_float, C.c_uint32, C.c_int
         fp = C.POINTER(C.c_float)
         L.oracle_rot.argtypes = [fp, fp, C.POINTER(SinCos), fp]
@@ -167,6 +170,18 @@ class Oracle:
         s.static_maps = _ptr(stat)
         s.lbuf_dim = scene.light_buffer_dim
         return s
+
+    def cut_list(self, scene, view, atlas=None):
+        if atlas is None:
+            atlas = (np.zeros((1, 1, 1, 4), np.uint8), np.zeros(1, np.uint32), np.zeros(1, np.uint32), 0)
+        s = self._scene(scene, atlas, None)
+        v = np.ascontiguousarray(view)
+        cam = camera_sincos(v["c_rot"][0])
+        cap = 2 * scene.tris.shape[0] + 16
+        cuts = np.zeros((cap, 3, 3), np.float32)
+        keys = np.zeros(cap, np.uint32)
+        n = self.lib.oracle_cut_list(C.byref(s), _ptr(v), C.byref(cam), _ptr(cuts), _ptr(keys), cap)
+        return cuts[:n], keys[:n]
 
     def shadow_maps(self, scene, nthreads=1):
         L = scene.light_buffer_dim

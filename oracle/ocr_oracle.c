@@ -319,7 +319,7 @@ static inline void walk_fragment(const f3 tp[3], float ewidth, float eheight, ui
 }
 
 /* ------------------------------------------------------- prearrange */
-typedef struct { uint32_t tri, chunk, cut; float rconst; uint32_t oid; } frag_rec;
+typedef struct { uint32_t tri, chunk, cut; float rconst; uint32_t oid, sub; } frag_rec;
 typedef struct { f3 p[3]; } cut_tri;
 
 typedef struct {
@@ -409,6 +409,7 @@ static void prearrange_all(const oracle_scene* sc, const ocr_view* v, const orac
                 frag_rec* r = &pl->frags[f++];
                 r->tri = (uint32_t)i; r->chunk = a; r->cut = c_id; r->rconst = ts[i].rconst[k];
                 r->oid = oid;
+                r->sub = (uint32_t)k;
             }
         }
     }
@@ -1225,6 +1226,31 @@ int oracle_atlas_fill(const ocr_tex_src* texs, uint32_t ntex, const uint32_t* nu
 }
 
 /* ---------------------------------------------------- test exports */
+int oracle_cut_list(const oracle_scene* sc, const ocr_view* v, const oracle_sincos* cam,
+                    float* cuts, uint32_t* keys, int max_out) {
+    prelist pl;
+    prearrange_all(sc, v, cam, 1, &pl);
+    /* survivors (>= 1 fragment) in fragment order, key = 2*tri + sub */
+    int n = 0;
+    uint32_t last_cut = 0xFFFFFFFFu;
+    for (uint32_t f = 0; f < pl.nfrag; f++) {
+        const frag_rec* r = &pl.frags[f];
+        if (r->cut == last_cut) continue;
+        last_cut = r->cut;
+        if (n < max_out) {
+            for (int j = 0; j < 3; j++) {
+                cuts[n * 9 + j * 3 + 0] = pl.cuts[r->cut].p[j].x;
+                cuts[n * 9 + j * 3 + 1] = pl.cuts[r->cut].p[j].y;
+                cuts[n * 9 + j * 3 + 2] = pl.cuts[r->cut].p[j].z;
+            }
+            keys[n] = 2u * r->tri + r->sub;
+        }
+        n++;
+    }
+    free(pl.cuts); free(pl.frags);
+    return n;
+}
+
 void oracle_rot(const float p[3], const float cpos[3], const oracle_sincos* sc, float out[3]) {
     f3 r = rot(ld3(p), ld3(cpos), sc); out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }

@@ -97,6 +97,10 @@ int oracle_atlas_fill(const ocr_tex_src* texs, uint32_t ntex, const uint32_t* nu
                       const uint32_t* sizes, uint32_t mip_start, uint8_t* pages);
 
 /* ---- single functions, exported for closed-form / brute-force tests ---- */
+/* Surviving cut triangles of a view in prearrange order (n x 9 floats:
+ * x,y,z per vertex) with key 2*tri + sub; returns the count. */
+int   oracle_cut_list(const oracle_scene* sc, const ocr_view* v, const oracle_sincos* cam,
+                      float* cuts, uint32_t* keys, int max_out);
 void  oracle_rot(const float p[3], const float cpos[3], const oracle_sincos* sc, float out[3]);
 void  oracle_back_rot(const float p[3], const oracle_sincos* sc, float out[3]);
 void  oracle_rot_quat(const float p[3], const float q[4], float out[3]);

@@ -1,0 +1,53 @@
+"""The C-ABI library loads and exports every entry point include/ocr.h
+declares, and the record layouts match the reference's host structs.  No
+compute calls: these run without a GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from openclrenderer_amd import abi, runtime
+from openclrenderer_amd import build as b
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "ocr.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(ocr_[a-z_0-9]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_lists_match_binding():
+    assert declared_functions() == sorted(runtime.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(b.LIB):
+        b.build()
+    lib = ctypes.CDLL(b.LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_record_sizes_match_reference_structs():
+    # cl2.cl:131-151 (vertex 48 B, triangle 144 B), :99-124 (descriptor 144 B
+    # with cl_float4 alignment), :77-87 (light 56 B -> 64 B aligned)
+    assert abi.VERTEX.itemsize == 48
+    assert abi.TRIANGLE.itemsize == 144
+    assert abi.OBJ_DESC.itemsize == 144
+    assert abi.LIGHT.itemsize == 64
+    assert abi.OBJ_DESC.fields["feature_flag"][1] == 132
+    assert abi.OBJ_DESC.fields["scale"][1] == 96
+    assert abi.LIGHT.fields["is_static"][1] == 52
+
+
+def test_header_constants():
+    text = open(os.path.join(ROOT, "include", "ocr.h")).read()
+    assert "#define OCR_DEPTH_FAR        350000.0f" in text
+    assert "#define OCR_OP_SIZE          500" in text
+    assert "#define OCR_OP_SIZE_LIGHT    300" in text
+    assert "#define OCR_BUF_ERROR        20" in text
+    assert np.uint32(abi.NO_TRIANGLE) == 0xFFFFFFFF

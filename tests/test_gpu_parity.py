@@ -1,0 +1,183 @@
+"""Parity of the HIP path with the oracle, through the C ABI.
+
+Bar: depth and triangle-id buffers bit-exact; RGBA8 within 1/255 per channel
+(the arithmetic contract makes it exact in practice, the tolerance is the
+north-star bar); encoded normals within 1 unit; shadow maps and atlas bytes
+bit-exact.
+"""
+import numpy as np
+import pytest
+
+from openclrenderer_amd import abi, runtime, scenes
+
+pytestmark = pytest.mark.gpu
+UINT_MAX = 0xFFFFFFFF
+RGBA_TOL = 1      # 1/255 per channel after 8-bit quantisation
+
+
+def render_gpu(ctx, sc, view, shadows=None):
+    g = runtime.GpuScene(ctx, sc)
+    if shadows is None:
+        shadows = bool(np.any(sc.lights["shadow"]))
+    if shadows:
+        g.shadow_pass()
+    fr = runtime.Frame(ctx, int(view["width"][0]), int(view["height"][0]))
+    out = fr.render_read(g, view)
+    return g, fr, out
+
+
+def assert_frame_parity(out, ref, label=""):
+    assert np.array_equal(out["depth"], ref["depth"]), f"{label} depth: {(out['depth'] != ref['depth']).sum()} px"
+    assert np.array_equal(out["tri_id"], ref["tri_id"]), f"{label} tri id: {(out['tri_id'] != ref['tri_id']).sum()} px"
+    d = np.abs(out["rgba"].astype(int) - ref["rgba"].astype(int))
+    assert d.max() <= RGBA_TOL, f"{label} rgba max diff {d.max()} at {np.argwhere(d > RGBA_TOL)[:4]}"
+    dn = np.abs(out["normals"].astype(int) - ref["normals"].astype(int))
+    assert dn.max() <= 1, f"{label} normals max diff {dn.max()}"
+
+
+def test_config1_bit_exact(gpu_ctx, oracle):
+    sc, view = scenes.config1()
+    g, fr, out = render_gpu(gpu_ctx, sc, view)
+    ref = oracle.render(sc, view)
+    assert_frame_parity(out, ref, "cfg1")
+    st = fr.stats()
+    assert st["n_frag"] == ref["stats"]["n_frag"]
+    assert st["n_cut"] == ref["stats"]["n_cut"]
+    assert (out["depth"] != UINT_MAX).sum() > 100000
+
+
+def test_config1_with_shadow_maps(gpu_ctx, oracle):
+    sc, view = scenes.config1(shadow=1)
+    g, fr, out = render_gpu(gpu_ctx, sc, view)
+    dyn, stat = g.read_shadow_maps()
+    odyn, ostat = oracle.shadow_maps(sc)
+    assert np.array_equal(dyn, odyn)
+    assert (dyn != UINT_MAX).sum() > 1000
+    ref = oracle.render(sc, view, maps=(odyn, ostat))
+    assert_frame_parity(out, ref, "cfg1s")
+
+
+@pytest.mark.parametrize("seed,size", [(0, (64, 48)), (1, (37, 23)), (2, (130, 97)), (3, (64, 48)),
+                                       (4, (201, 33)), (5, (16, 256)), (6, (64, 48)), (7, (99, 101))])
+def test_random_scenes(gpu_ctx, oracle, seed, size):
+    sc, view = scenes.random_scene(seed, width=size[0], height=size[1])
+    view["fov_const"] = scenes.fov_const_from_hfov(70.0, size[0])
+    g, fr, out = render_gpu(gpu_ctx, sc, view)
+    ref = oracle.render(sc, view)
+    assert_frame_parity(out, ref, f"seed{seed}")
+
+
+def test_random_scene_shadows_static_and_flags(gpu_ctx, oracle):
+    """Two shadow lights, one static; static objects; an object that does not
+    receive dynamic shadows (FEATURE_FLAG_* at cl2.cl:90-97)."""
+    sc, view = scenes.random_scene(21, n_tris=120, width=96, height=72, shadow=1)
+    sc.lights = np.concatenate([sc.lights, scenes.make_light((50.0, -200.0, 100.0), shadow=1, is_static=1)])
+    sc.objs["feature_flag"][0] |= abi.FEATURE_IS_STATIC
+    sc.objs["feature_flag"][2] |= abi.FEATURE_DOES_NOT_RECEIVE_DYNAMIC_SHADOWS
+    g, fr, out = render_gpu(gpu_ctx, sc, view)
+    dyn, stat = g.read_shadow_maps()
+    odyn, ostat = oracle.shadow_maps(sc)
+    assert np.array_equal(dyn, odyn)
+    assert np.array_equal(stat, ostat)
+    ref = oracle.render(sc, view, maps=(odyn, ostat))
+    assert_frame_parity(out, ref, "flags")
+
+
+def test_atlas_bytes_mixed_sizes(gpu_ctx, oracle):
+    rng = np.random.default_rng(5)
+    texs = []
+    for (w, h) in [(256, 256), (100, 60), (16, 16), (512, 512), (33, 17), (256, 256), (64, 128)]:
+        rgba = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+        texs.append(scenes.Texture(w, h, rgba))
+    sc, view = scenes.random_scene(1)
+    sc.textures = texs
+    g = runtime.GpuScene(gpu_ctx, sc)
+    pages, nums, sizes, ms = g.read_atlas()
+    op, on, osz, oms = oracle.atlas(texs)
+    assert ms == oms
+    assert np.array_equal(nums[:on.size], on)
+    assert np.array_equal(sizes[:osz.size], osz)
+    assert np.array_equal(pages, op)
+
+
+def test_empty_and_fully_clipped_scenes(gpu_ctx, oracle):
+    sc, view = scenes.random_scene(2)
+    # every triangle behind the camera (z < near plane)
+    sc.objs["world_pos"][:, 2] = -5000.0
+    g, fr, out = render_gpu(gpu_ctx, sc, view)
+    assert np.all(out["depth"] == UINT_MAX)
+    assert np.all(out["tri_id"] == UINT_MAX)
+    ref = oracle.render(sc, view)
+    assert_frame_parity(out, ref, "clipped")
+    sc0 = scenes.Scene(sc.tris[:0], sc.objs, sc.lights, sc.textures, sc.light_buffer_dim)
+    g0, fr0, out0 = render_gpu(gpu_ctx, sc0, view)
+    assert np.all(out0["depth"] == UINT_MAX)
+
+
+def test_vertex_colour_only_scene_without_textures(gpu_ctx, oracle):
+    sc, view = scenes.random_scene(9)
+    sc.tris["v"]["vertex_col"] = 0xFF8040FF
+    sc.textures = []
+    sc.objs["tid"] = 0
+    g, fr, out = render_gpu(gpu_ctx, sc, view)
+    ref = oracle.render(sc, view)
+    assert_frame_parity(out, ref, "vcol")
+
+
+def test_overflow_is_reported_then_grown(gpu_ctx, oracle):
+    sc, view = scenes.config1()
+    g = runtime.GpuScene(gpu_ctx, sc)
+    fr = runtime.Frame(gpu_ctx, 800, 600)
+    fr.reserve(100)
+    fr.render(g, view)
+    with pytest.raises(runtime.OcrError) as ei:
+        fr.read()
+    assert ei.value.code == abi.OCR_ERR_OVERFLOW
+    fr.render(g, view)
+    out = fr.read()
+    assert_frame_parity(out, oracle.render(sc, view), "regrown")
+
+
+def test_repeat_renders_are_identical_and_objects_update(gpu_ctx, oracle):
+    sc, view = scenes.config1()
+    g = runtime.GpuScene(gpu_ctx, sc)
+    fr = runtime.Frame(gpu_ctx, 800, 600)
+    a = fr.render_read(g, view)
+    b = fr.render_read(g, view)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    # object::g_flush: move the sphere and re-render
+    objs = sc.objs.copy()
+    objs["world_pos"][0, :3] = [120.0, -40.0, 60.0]
+    g.update_objs(0, objs[:1])
+    c = fr.render_read(g, view)
+    sc2 = scenes.Scene(sc.tris, objs, sc.lights, sc.textures, sc.light_buffer_dim)
+    assert_frame_parity(c, oracle.render(sc2, view), "moved")
+    assert not np.array_equal(a["depth"], c["depth"])
+
+
+def test_invalid_inputs_are_rejected(gpu_ctx):
+    sc, view = scenes.config1()
+    g = runtime.GpuScene(gpu_ctx, sc)
+    fr = runtime.Frame(gpu_ctx, 800, 600)
+    bad = view.copy()
+    bad["width"] = 640
+    with pytest.raises(runtime.OcrError):
+        fr.render(g, bad)
+    objs = sc.objs.copy()
+    objs["tid"][3] = 7          # no such texture
+    g.update_objs(0, objs)
+    with pytest.raises(runtime.OcrError):
+        fr.render(g, view)
+    with pytest.raises(runtime.OcrError):
+        runtime.Frame(gpu_ctx, 2, 2)
+
+
+def test_config2_full_frame(gpu_ctx, oracle):
+    """1920x1080, 1,000,024 triangles: the bench workload, compared in full."""
+    sc, view = scenes.config2()
+    g, fr, out = render_gpu(gpu_ctx, sc, view)
+    ref = oracle.render(sc, view, nthreads=16)
+    assert_frame_parity(out, ref, "cfg2")
+    st = fr.stats()
+    assert st["n_frag"] == ref["stats"]["n_frag"]
