@@ -11,9 +11,12 @@ CSRC = os.path.join(HERE, "csrc")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libocr.so")
 CALIB_LIB = os.path.join(HERE, "libocr_calib.so")
+FACADE_LIB = os.path.join(HERE, "libocr_engine.so")
+FACADE_DEMO = os.path.join(HERE, "ocr_facade_demo")
 SOURCES = ["runtime.hip", "raster.hip", "shade.hip", "atlas.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("OCR_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: no implicit fused multiply-add (results must match the
@@ -55,6 +58,18 @@ def build(verbose=False, jobs=4):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
+    # the C++ draw API over the C ABI, and its demo program
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    # host-only C++ (g++), linked against the C ABI and the HIP runtime API
+    hip = ["-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    tmp = FACADE_LIB + ".tmp"
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", inc, os.path.join(CSRC, "facade.cpp"),
+          "-L", HERE, "-locr", *hip, "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+    os.replace(tmp, FACADE_LIB)
+    tmp = FACADE_DEMO + ".tmp"
+    _run([CXX, "-O2", "-std=c++17", "-Wall", "-I", inc, os.path.join(CSRC, "facade_demo.cpp"),
+          "-L", HERE, "-locr_engine", "-locr", *hip, "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+    os.replace(tmp, FACADE_DEMO)
     # the calibration copy kernel used by profiling runs (separate library)
     tmp = CALIB_LIB + ".tmp"
     _run([HIPCC, *FLAGS, "-shared", os.path.join(CSRC, "calib.hip"), "-o", tmp], verbose)

@@ -1,0 +1,415 @@
+// facade.cpp — the reference's draw API (engine / object_context /
+// objects_container / texture_context / light) implemented over the C ABI.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <set>
+#include <string>
+
+#include "../../include/ocr_engine.hpp"
+
+// The HIP runtime header declares a `texture` template that collides with the
+// draw API's texture class, so the few runtime entry points used here are
+// declared by their C ABI (hipError_t is an int enum; events and streams are
+// opaque pointers).
+extern "C" {
+int hipEventCreateWithFlags(void** event, unsigned flags);
+int hipEventRecord(void* event, void* stream);
+int hipEventSynchronize(void* event);
+int hipEventDestroy(void* event);
+int hipDeviceSynchronize(void);
+}
+static const unsigned kHipEventDisableTiming = 0x2;
+
+// ------------------------------------------------------------------ statics
+cl_uint engine::width = 0, engine::height = 0, engine::depth = 0;
+cl_uint engine::l_size = 1024;
+cl_float4 engine::c_pos = {{0, 0, 0, 0}};
+cl_float4 engine::c_rot = {{0, 0, 0, 0}};
+
+std::vector<light*> light::lightlist;
+std::vector<cl_uint> light::active;
+bool light::static_lights_are_dirty = true;
+bool light::dirty_gpu = true;
+
+static ocr_ctx* g_ctx = nullptr;
+
+static void report(const char* where, int rc) {
+  if (rc != OCR_OK) fprintf(stderr, "[ocr] %s failed (%d): %s\n", where, rc, ocr_last_error());
+}
+
+ocr_ctx* engine::ctx() {
+  if (!g_ctx) report("ocr_create", ocr_create(0, &g_ctx));
+  return g_ctx;
+}
+
+void compute_event::wait() const {
+  if (ev) (void)hipEventSynchronize(ev.get());
+}
+
+static compute_event record_event(void* stream) {
+  void* e = nullptr;
+  compute_event out;
+  if (hipEventCreateWithFlags(&e, kHipEventDisableTiming) != 0) return out;
+  if (hipEventRecord(e, stream) != 0) {
+    (void)hipEventDestroy(e);
+    return out;
+  }
+  out.ev = std::shared_ptr<void>(e, [](void* p) { (void)hipEventDestroy(p); });
+  return out;
+}
+
+// ---------------------------------------------------------- objects_container
+void objects_container::set_pos(cl_float4 p) {
+  pos = p;
+  for (auto& o : objs) o.set_pos(pos);
+}
+void objects_container::set_rot_quat(quaternion q) {
+  rot_quat = q;
+  for (auto& o : objs) o.set_rot_quat(q);
+}
+void objects_container::set_dynamic_scale(float s) {
+  dynamic_scale = s;
+  for (auto& o : objs) o.set_dynamic_scale(s);
+}
+void objects_container::set_specular(float v) { for (auto& o : objs) o.specular = v; }
+void objects_container::set_spec_mult(float v) { for (auto& o : objs) o.spec_mult = v; }
+void objects_container::set_diffuse(float v) { for (auto& o : objs) o.diffuse = v; }
+void objects_container::set_two_sided(bool v) { for (auto& o : objs) o.set_two_sided(v); }
+void objects_container::set_outlined(bool v) { for (auto& o : objs) o.set_outlined(v); }
+void objects_container::set_ss_reflective(int v) { for (auto& o : objs) o.set_ss_reflective(v != 0); }
+void objects_container::set_is_static(bool v) { for (auto& o : objs) o.set_is_static(v); }
+void objects_container::set_does_not_receive_dynamic_shadows(bool v) {
+  for (auto& o : objs) o.set_does_not_receive_dynamic_shadows(v);
+}
+void objects_container::set_children_texture_id(cl_uint tid) { for (auto& o : objs) o.tid = tid; }
+
+// ------------------------------------------------------------------- texture
+void texture::set_create_colour(uint8_t r, uint8_t g, uint8_t b, uint8_t a, cl_uint w, cl_uint h) {
+  width = w;
+  height = h;
+  rgba.resize((size_t)w * h * 4);
+  for (size_t i = 0; i < (size_t)w * h; i++) {
+    rgba[i * 4] = r; rgba[i * 4 + 1] = g; rgba[i * 4 + 2] = b; rgba[i * 4 + 3] = a;
+  }
+}
+
+void texture::set_raw(cl_uint w, cl_uint h, const uint8_t* data) {
+  width = w;
+  height = h;
+  rgba.assign(data, data + (size_t)w * h * 4);
+}
+
+texture* texture_context::make_new() {
+  all_textures.emplace_back(new texture());
+  all_textures.back()->id = gid++;
+  return all_textures.back().get();
+}
+
+texture* texture_context::id_to_tex(int id) {
+  for (auto& t : all_textures)
+    if (t->id == id) return t.get();
+  return nullptr;
+}
+
+int texture_context::get_gpu_position_id(int id) const {
+  for (size_t i = 0; i < texture_id_orders.size(); i++)
+    if (texture_id_orders[i] == id) return (int)i;
+  return -1;
+}
+
+int texture_context::alloc_gpu(object_context& ctx, ocr_scene* scene) {
+  // texture_context.cpp:350-420: textures used by active objects, plus
+  // force_load ones, in ascending id order (a std::set in the reference)
+  std::set<int> in_use;
+  for (auto* c : ctx.containers) {
+    if (!c->isactive) continue;
+    for (auto& o : c->objs)
+      if (o.isactive && id_to_tex((int)o.tid)) in_use.insert((int)o.tid);
+  }
+  for (auto& t : all_textures)
+    if (t->force_load) in_use.insert(t->id);
+  texture_id_orders.assign(in_use.begin(), in_use.end());
+  mipmap_start = (int)texture_id_orders.size();
+  std::vector<ocr_tex_src> srcs;
+  for (size_t i = 0; i < texture_id_orders.size(); i++) {
+    texture* t = id_to_tex(texture_id_orders[i]);
+    t->gpu_id = (int)i;
+    srcs.push_back(ocr_tex_src{t->width, t->height, t->rgba.data()});
+  }
+  if (!scene) return OCR_OK;
+  int rc = ocr_atlas_upload(scene, srcs.data(), (uint32_t)srcs.size());
+  report("ocr_atlas_upload", rc);
+  return rc;
+}
+
+// --------------------------------------------------------------------- light
+light* light::add_light(const light* l) {
+  light* n = new light(*l);
+  lightlist.push_back(n);
+  active.push_back(1);
+  dirty_gpu = true;
+  if (n->shadow && n->is_static) static_lights_are_dirty = true;
+  return n;
+}
+
+void light::remove_light(light* l) {
+  for (size_t i = 0; i < lightlist.size(); i++)
+    if (lightlist[i] == l) {
+      delete l;
+      lightlist.erase(lightlist.begin() + (long)i);
+      active.erase(active.begin() + (long)i);
+      dirty_gpu = true;
+      return;
+    }
+}
+
+void light::set_active(bool a) {
+  for (size_t i = 0; i < lightlist.size(); i++)
+    if (lightlist[i] == this) active[i] = a ? 1 : 0;
+  dirty_gpu = true;
+}
+
+int light::get_num_shadowcasting_lights() {
+  int n = 0;
+  for (size_t i = 0; i < lightlist.size(); i++) n += active[i] && lightlist[i]->shadow == 1;
+  return n;
+}
+
+int light::get_num_static_shadowcasters() {
+  int n = 0;
+  for (size_t i = 0; i < lightlist.size(); i++) n += active[i] && lightlist[i]->shadow && lightlist[i]->is_static;
+  return n;
+}
+
+// light::build (light.cpp:145-198): the active lights packed in list order.
+light_gpu light::build(light_gpu* old_dat) {
+  static cl_uint generation = 0;
+  light_gpu dat;
+  for (size_t i = 0; i < lightlist.size(); i++) {
+    if (!active[i]) continue;
+    const light& l = *lightlist[i];
+    ocr_light g;
+    memset(&g, 0, sizeof(g));
+    for (int k = 0; k < 4; k++) { g.pos[k] = l.pos.s[k]; g.col[k] = l.col.s[k]; }
+    g.shadow = l.shadow;
+    g.brightness = l.brightness;
+    g.radius = l.radius;
+    g.diffuse = l.diffuse;
+    g.godray_intensity = l.godray_intensity;
+    g.is_static = l.is_static;
+    dat.lights.push_back(g);
+  }
+  dat.generation = ++generation;
+  dirty_gpu = false;
+  (void)old_dat;
+  return dat;
+}
+
+// ---------------------------------------------------------- object_context
+object_context::object_context() {}
+
+object_context::~object_context() {
+  if (gpu_dat.frame) ocr_frame_destroy(gpu_dat.frame);
+  if (gpu_dat.scene) ocr_scene_destroy(gpu_dat.scene);
+}
+
+objects_container* object_context::make_new() {
+  owned.emplace_back(new objects_container());
+  objects_container* c = owned.back().get();
+  c->parent = this;
+  c->id = (cl_uint)(owned.size() - 1);
+  containers.push_back(c);
+  return c;
+}
+
+void object_context::destroy(objects_container* obj) {
+  containers.erase(std::remove(containers.begin(), containers.end(), obj), containers.end());
+  for (auto it = owned.begin(); it != owned.end(); ++it)
+    if (it->get() == obj) {
+      owned.erase(it);
+      break;
+    }
+}
+
+// object_context::load_active (object_context.cpp:127): run each active,
+// unloaded container's load function.
+void object_context::load_active() {
+  for (auto* c : containers) {
+    if (!c->isactive || c->isloaded) continue;
+    c->call_load_func(c);
+    c->isloaded = true;
+    for (auto& o : c->objs) {
+      o.isloaded = 1;
+      o.set_pos(c->pos);
+      o.set_rot_quat(c->rot_quat);
+      if (c->dynamic_scale != 1.f) o.set_dynamic_scale(c->dynamic_scale);
+    }
+  }
+}
+
+// generate_gpu_object_descriptor (object_context.cpp:228-339)
+void object_context::write_descriptors(std::vector<ocr_obj_desc>& out) {
+  out.clear();
+  for (object* o : built_objects) {
+    ocr_obj_desc d;
+    memset(&d, 0, sizeof(d));
+    for (int k = 0; k < 3; k++) d.world_pos[k] = o->pos.s[k];
+    d.world_rot_quat[0] = o->rot_quat.x; d.world_rot_quat[1] = o->rot_quat.y;
+    d.world_rot_quat[2] = o->rot_quat.z; d.world_rot_quat[3] = o->rot_quat.w;
+    memcpy(d.old_world_pos_1, d.world_pos, 16);
+    memcpy(d.old_world_pos_2, d.world_pos, 16);
+    memcpy(d.old_world_rot_quat_1, d.world_rot_quat, 16);
+    memcpy(d.old_world_rot_quat_2, d.world_rot_quat, 16);
+    d.scale = o->dynamic_scale;
+    int tid = tex_ctx.get_gpu_position_id((int)o->tid);
+    d.tid = tid < 0 ? 0u : (uint32_t)tid;
+    d.rid = o->rid;
+    d.ssid = o->ssid;
+    d.has_bump = o->has_bump;
+    d.specular = o->specular;
+    d.spec_mult = o->spec_mult;
+    d.diffuse = o->diffuse;
+    d.feature_flag = o->feature_flag;
+    out.push_back(d);
+  }
+}
+
+void object_context::build(bool force, bool async, compute_event* async_render_event) {
+  (void)force; (void)async; (void)async_render_event;
+  built_objects.clear();
+  std::vector<triangle> tris;
+  cl_uint n = 0;
+  // alloc_gpu (object_context.cpp:398-453): active containers' objects in
+  // order; every triangle's vertices[0].object_id is its object's index
+  for (auto* c : containers) {
+    if (!c->isactive) continue;
+    for (auto& o : c->objs) {
+      o.object_g_id = (cl_int)n;
+      built_objects.push_back(&o);
+      for (const triangle& t : o.tri_list) {
+        tris.push_back(t);
+        tris.back().vertices[0].object_id = n;
+      }
+      n++;
+    }
+  }
+  tex_ctx.alloc_gpu(*this, nullptr);          // decides the texture order
+  std::vector<ocr_obj_desc> descs;
+  write_descriptors(descs);
+  if (gpu_dat.scene) ocr_scene_destroy(gpu_dat.scene);
+  gpu_dat.scene = nullptr;
+  ocr_scene* s = nullptr;
+  int rc = ocr_scene_create(engine::ctx(), tris.data(), (uint32_t)tris.size(), descs.data(),
+                            (uint32_t)descs.size(), &s);
+  report("ocr_scene_create", rc);
+  if (rc) return;
+  tex_ctx.alloc_gpu(*this, s);
+  gpu_dat.scene = s;
+  gpu_dat.tri_num = (cl_uint)tris.size();
+  gpu_dat.obj_num = (cl_uint)descs.size();
+}
+
+void object_context::flush_locations(bool force, compute_event* render_event) {
+  (void)force; (void)render_event;
+  if (!gpu_dat.scene || built_objects.empty()) return;
+  std::vector<ocr_obj_desc> descs;
+  write_descriptors(descs);
+  report("ocr_scene_update_objs", ocr_scene_update_objs(gpu_dat.scene, 0, (uint32_t)descs.size(), descs.data()));
+}
+
+bool object_context_data::ensure_screen_buffers(int w, int h) {
+  if (frame && s_w == w && s_h == h) return true;
+  if (frame) ocr_frame_destroy(frame);
+  frame = nullptr;
+  int rc = ocr_frame_create(engine::ctx(), (uint32_t)w, (uint32_t)h, &frame);
+  report("ocr_frame_create", rc);
+  if (rc) return false;
+  s_w = w;
+  s_h = h;
+  return true;
+}
+
+// -------------------------------------------------------------------- engine
+// calculate_fov_constant_from_hfov (engine.cpp:119-133), then the value goes
+// through std::to_string(fov) + "f" into the kernel build (engine.cpp:474-477).
+float engine::get_fov_const() const {
+  float fov_radians = (horizontal_fov_degrees / 360.f) * 2 * M_PI;
+  float triangle_angle = fov_radians / 2;
+  float fov_constant = ((float)width / 2) / tanf(triangle_angle);
+  return strtof(std::to_string(fov_constant).c_str(), nullptr);
+}
+
+void engine::load(cl_uint w, cl_uint h, cl_uint d, const std::string& name, const std::string& loc,
+                  bool only_3d, bool fullscreen) {
+  (void)name; (void)loc; (void)only_3d; (void)fullscreen;
+  width = w;
+  height = h;
+  depth = d;
+  l_size = 1024;
+  (void)ctx();
+}
+
+void engine::sync_lights(object_context_data& dat) {
+  if (!dat.scene || !light_data) return;
+  if (uploaded_light_generation == light_data->generation && lights_scene == dat.scene) return;
+  int rc = ocr_lights_upload(dat.scene, light_data->lights.data(), (uint32_t)light_data->lights.size(), l_size);
+  report("ocr_lights_upload", rc);
+  last_status = rc;
+  uploaded_light_generation = light_data->generation;
+  lights_scene = dat.scene;
+}
+
+compute_event engine::generate_realtime_shadowing(object_context_data& dat) {
+  sync_lights(dat);
+  if (!dat.scene) return compute_event();
+  last_status = ocr_shadow_pass(dat.scene, nullptr);
+  report("ocr_shadow_pass", last_status);
+  return record_event(nullptr);
+}
+
+compute_event engine::draw_bulk_objs_n(object_context_data& dat) {
+  if (!dat.scene || !dat.ensure_screen_buffers((int)width, (int)height)) return compute_event();
+  sync_lights(dat);
+  if (!light_data) {
+    last_status = ocr_lights_upload(dat.scene, nullptr, 0, l_size);
+    report("ocr_lights_upload", last_status);
+  }
+  ocr_view v;
+  memset(&v, 0, sizeof(v));
+  v.width = width;
+  v.height = height;
+  v.fov_const = get_fov_const();
+  v.use_linear_rendering = dat.use_linear_rendering;
+  for (int k = 0; k < 4; k++) {
+    v.c_pos[k] = c_pos.s[k];
+    v.c_rot[k] = c_rot.s[k];
+    v.clear_col[k] = dat.clear_col.s[k];
+  }
+  last_status = ocr_render(dat.scene, &v, dat.frame, nullptr);
+  report("ocr_render", last_status);
+  dat.frame_id++;
+  last_view = v;
+  return record_event(nullptr);
+}
+
+void engine::blit_to_screen(object_context_data& dat) {
+  if (!dat.frame) return;
+  size_t npix = (size_t)dat.s_w * dat.s_h;
+  dat.depth.resize(npix);
+  dat.tri_id.resize(npix);
+  dat.rgba8.resize(npix * 4);
+  int rc = ocr_frame_read(dat.frame, dat.depth.data(), dat.tri_id.data(), dat.rgba8.data(), nullptr, nullptr);
+  if (rc == OCR_ERR_OVERFLOW) {
+    // the work lists grew; render the same view again
+    rc = ocr_render(dat.scene, &last_view, dat.frame, nullptr);
+    if (rc == OCR_OK) rc = ocr_frame_read(dat.frame, dat.depth.data(), dat.tri_id.data(), dat.rgba8.data(), nullptr, nullptr);
+  }
+  report("ocr_frame_read", rc);
+  last_status = rc;
+}
+
+void engine::render_block() { (void)hipDeviceSynchronize(); }

@@ -179,35 +179,46 @@ __device__ __forceinline__ f3 depth_project(f3 r, float width, float height, flo
   return mk3(fmaf(r.x, f, width / 2.f), fmaf(r.y, f, height / 2.f), r.z);
 }
 
-// generate_new_triangles, cl2.cl:577-663: near/far clip into 0..2 triangles.
-__device__ __forceinline__ int clip_near(const f3 pts[3], f3 out[2][3]) {
-  int idv = 0, behind[3] = {0, 0, 0}, nb = 0;
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    if (pts[i].z <= (float)kIcutoff || pts[i].z > kDepthFar) behind[nb++] = i;
-    else idv = i;
-  }
+__device__ __forceinline__ f3 sel3(int k, f3 a, f3 b, f3 c) {
+  return k == 0 ? a : (k == 1 ? b : c);
+}
+
+// The point of edge a -> b at z = depth_icutoff (cl2.cl:636-637).
+__device__ __forceinline__ f3 clip_edge(f3 a, f3 b) {
+  return a + ((b - a) * ((float)kIcutoff - a.z)) / (b.z - a.z);
+}
+
+// generate_new_triangles, cl2.cl:577-663: near/far clip into 0..2
+// triangles.  Vertex choices are selects, not array indexing, so the three
+// vertices stay in registers.
+__device__ __forceinline__ int clip_near(f3 p0, f3 p1, f3 p2, f3 out[2][3]) {
+  const bool b0 = p0.z <= (float)kIcutoff || p0.z > kDepthFar;
+  const bool b1 = p1.z <= (float)kIcutoff || p1.z > kDepthFar;
+  const bool b2 = p2.z <= (float)kIcutoff || p2.z > kDepthFar;
+  const int nb = (int)b0 + (int)b1 + (int)b2;
   if (nb > 2) return 0;
-  if (nb == 0) { out[0][0] = pts[0]; out[0][1] = pts[1]; out[0][2] = pts[2]; return 1; }
-  int g1, g2, g3;
+  if (nb == 0) { out[0][0] = p0; out[0][1] = p1; out[0][2] = p2; return 1; }
   if (nb == 1) {
-    int id = behind[0];
-    g1 = id;
-    g2 = (id + 1) >= 3 ? id - 2 : id + 1;
-    g3 = (id + 2) >= 3 ? id - 1 : id + 2;
-  } else {
-    g2 = behind[0]; g3 = behind[1]; g1 = idv;
-  }
-  f3 p1 = pts[g2] + ((pts[g1] - pts[g2]) * ((float)kIcutoff - pts[g2].z)) / (pts[g1].z - pts[g2].z);
-  f3 p2 = pts[g3] + ((pts[g1] - pts[g3]) * ((float)kIcutoff - pts[g3].z)) / (pts[g1].z - pts[g3].z);
-  if (nb == 1) {
-    out[0][0] = p1; out[0][1] = pts[g2]; out[0][2] = pts[g3];
-    out[1][0] = p1; out[1][1] = pts[g3]; out[1][2] = p2;
+    // g1 = the vertex behind, g2 / g3 the next two in cyclic order
+    const int id = b0 ? 0 : (b1 ? 1 : 2);
+    const f3 G1 = sel3(id, p0, p1, p2);
+    const f3 G2 = sel3(id, p1, p2, p0);
+    const f3 G3 = sel3(id, p2, p0, p1);
+    const f3 c1 = clip_edge(G2, G1), c2 = clip_edge(G3, G1);
+    out[0][0] = c1; out[0][1] = G2; out[0][2] = G3;
+    out[1][0] = c1; out[1][1] = G3; out[1][2] = c2;
     return 2;
   }
-  out[0][behind[0]] = p1;
-  out[0][behind[1]] = p2;
-  out[0][idv] = pts[g1];
+  // two behind: g2 < g3 behind, g1 valid; each behind vertex is replaced
+  const int v = !b0 ? 0 : (!b1 ? 1 : 2);
+  const int first = b0 ? 0 : 1;                 // lower behind index
+  const f3 G1 = sel3(v, p0, p1, p2);
+  const f3 G2 = sel3(first, p0, p1, p2);
+  const f3 G3 = b2 ? p2 : p1;                   // higher behind index
+  const f3 c1 = clip_edge(G2, G1), c2 = clip_edge(G3, G1);
+  out[0][0] = v == 0 ? G1 : c1;
+  out[0][1] = v == 1 ? G1 : (first == 1 ? c1 : c2);
+  out[0][2] = v == 2 ? G1 : c2;
   return 1;
 }
 

@@ -137,8 +137,11 @@ __device__ __forceinline__ int transform_clip_project(const TriPos& T, const Obj
     pr[i] = rot(rot_quat(v * G.scale, q) + wp, cpos, sc);
   }
   f3 tris[2][3];
-  int n = clip_near(pr, tris);
-  for (int t = 0; t < n; t++)
+  int n = clip_near(pr[0], pr[1], pr[2], tris);
+  // both slots are projected unconditionally (static indices keep the
+  // arrays in registers); slot 1 is only read when n == 2
+#pragma unroll
+  for (int t = 0; t < 2; t++)
 #pragma unroll
     for (int i = 0; i < 3; i++) out[t][i] = depth_project(tris[t][i], ew, eh, fovc);
   return n;
@@ -203,7 +206,9 @@ __global__ void __launch_bounds__(256) k_prearrange(const TriPos* __restrict__ t
       if (!(fast_length(mk3(G.wp[0], G.wp[1], G.wp[2]) - cpos) > kDepthFar)) {
         int num = transform_clip_project(T, G, cpos, v.cam, v.fov, ew, eh, tp);
         bool two_sided = (G.flags & OCR_FEATURE_TWO_SIDED) > 0;
-        for (int i = 0; i < num; i++) nf[i] = cull_count(tp[i], ew, eh, two_sided, kOpSize, rc[i]);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+          if (i < num) nf[i] = cull_count(tp[i], ew, eh, two_sided, kOpSize, rc[i]);
       }
     }
   }
@@ -236,60 +241,162 @@ hipError_t launch_prearrange(const Scene* sc, const ViewParams& v, Frame* f, hip
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------- depth pass
-__global__ void __launch_bounds__(256) k_depth(const CutTri* __restrict__ cuts,
-                                               const uint2* __restrict__ frags,
-                                               const uint32_t* __restrict__ counters, uint32_t cap_frag,
-                                               uint32_t W, uint32_t H, uint32_t* __restrict__ depth) {
+// ---------------------------------------------- load-balanced raster passes
+//
+// A fragment is the reference's unit of work: the samples q = pa .. pa + 500
+// (pa = op_size * chunk) of its cut triangle's bounding box, visited by a
+// running cursor (cl2.cl:5046-5126).  Fragment sizes range from 1 to 501
+// samples, so one thread per fragment leaves most lanes of a wavefront idle.
+// Here a wavefront loads 64 fragments, keeps their setup in LDS, prefix-sums
+// their sample counts and sweeps all of their samples with all 64 lanes.
+//
+// Each sample is evaluated on its own with the cursor's closed form:
+//   y(q) = floor(fma(q, iwidth, min_y))                  (the reference's row)
+//   the walk stops at the first q with y(q) >= max_y      (y is monotone in q)
+//   x(q) = (r % width) + min_x + (q - r)
+// where r is the later of the fragment start pa and the first index of q's
+// row run (the cursor is reset to running_width_mod + min_x exactly when y
+// changes, and starts at pa % width + min_x).  Row-run starts are found from
+// the nominal (Y - min_y) * width by stepping while y disagrees, so rows whose
+// fma rounding lags or leads by a sample are reproduced exactly.
+
+struct FragLds {
+  float mm[4][kWave];      // min_x, max_x, min_y, max_y
+  float pl[4][kWave];      // A, B, C (1/z plane), iwidth
+  float ek[8][kWave];      // edge test: s0, s1, s2, t0, t1, t2, sign, 2.0001*A*sign
+  int w[kWave];
+  int pa[kWave];
+  uint32_t aux[kWave];     // cut index (id pass) or cube face (shadow pass)
+  uint32_t key[kWave];     // 2*tri + sub (id pass)
+  uint32_t pre[kWave + 1]; // exclusive prefix of sample counts
+};
+
+__device__ __forceinline__ float row_of(int q, float iw, float min_y) {
+  return floorf(fmaf((float)q, iw, min_y));
+}
+
+// First q >= 0 with row_of(q) >= Y.
+__device__ __forceinline__ int run_start(float Y, int w, float iw, float min_y) {
+  long long n0 = (long long)(Y - min_y) * (long long)w;
+  int q = n0 < 0 ? 0 : (int)n0;
+  for (int g = 0; g < 64 && q > 0 && row_of(q - 1, iw, min_y) >= Y; g++) q--;
+  for (int g = 0; g < 64 && row_of(q, iw, min_y) < Y; g++) q++;
+  return q;
+}
+
+enum class RasterMode { Depth, Id, Shadow };
+
+template <RasterMode kMode>
+__global__ void __launch_bounds__(256) k_raster(const CutTri* __restrict__ cuts, const uint2* __restrict__ frags,
+                                                const uint32_t* __restrict__ counters, uint32_t cap_frag,
+                                                uint32_t W, uint32_t H, int op_size,
+                                                uint32_t* __restrict__ depth,
+                                                unsigned long long* __restrict__ idkey) {
+  __shared__ FragLds lds_all[256 / kWave];
   if (counters[kOverflow]) return;
   const uint32_t n = min(counters[kFragCount], cap_frag);
+  const int lane = threadIdx.x & (kWave - 1);
+  FragLds& L = lds_all[threadIdx.x / kWave];
   const float ew = (float)W, eh = (float)H;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint2 fr = frags[i];
-    Setup st;
-    raster_setup(cuts[fr.x], ew, eh, st);
-    walk(st, fr.y, kOpSize, [&](int x, int y, uint32_t d) {
-      atomicMin(&depth[(size_t)y * W + (size_t)x], d);
-    });
+  const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  for (uint32_t base = wave0 * kWave; base < n; base += nwaves * kWave) {
+    // ---- per-fragment setup, one lane per fragment
+    const uint32_t fi = base + lane;
+    uint32_t cnt = 0;
+    if (fi < n) {
+      const uint2 fr = frags[fi];
+      const CutTri c = cuts[fr.x];
+      Setup st;
+      raster_setup(c, ew, eh, st);
+      const int pa = op_size * (int)fr.y;
+      if (st.width > 0) {
+        // samples of this fragment: q in [pa, pa + op_size] with y(q) < max_y
+        const int q_end = run_start(st.mm[3], st.width, st.iwidth, st.mm[2]);
+        int c2 = q_end - pa;
+        c2 = c2 < 0 ? 0 : (c2 > op_size + 1 ? op_size + 1 : c2);
+        cnt = (uint32_t)c2;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) L.mm[k][lane] = st.mm[k];
+      L.pl[0][lane] = st.A; L.pl[1][lane] = st.B; L.pl[2][lane] = st.C; L.pl[3][lane] = st.iwidth;
+      // point_in_tri's loop-invariant sub-expressions (same operation order)
+      const float p0x = st.xp[0], p0y = st.yp[0], p1x = st.xp[1], p1y = st.yp[1], p2x = st.xp[2], p2y = st.yp[2];
+      const float A = 0.5f * (-p1y * p2x + p0y * (-p1x + p2x) + p0x * (p1y - p2y) + p1x * p2y);
+      const float sign = A < 0 ? -1.f : 1.f;
+      L.ek[0][lane] = p0y * p2x - p0x * p2y;
+      L.ek[1][lane] = p2y - p0y;
+      L.ek[2][lane] = p0x - p2x;
+      L.ek[3][lane] = p0x * p1y - p0y * p1x;
+      L.ek[4][lane] = p0y - p1y;
+      L.ek[5][lane] = p1x - p0x;
+      L.ek[6][lane] = sign;
+      L.ek[7][lane] = 2.0001f * A * sign;
+      L.w[lane] = st.width;
+      L.pa[lane] = pa;
+      // id pass: cut index + triangle key; shadow pass: cube face
+      L.aux[lane] = kMode == RasterMode::Id ? fr.x : c.key;
+      L.key[lane] = c.key;
+    }
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t total = __shfl(incl, kWave - 1, kWave);
+    L.pre[lane] = incl - cnt;
+    if (lane == kWave - 1) L.pre[kWave] = total;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // ---- sweep every sample of the 64 fragments with all lanes
+    for (uint32_t j = lane; j < total; j += kWave) {
+      int k = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1)
+        if (L.pre[k + step] <= j) k += step;
+      const int w = L.w[k];
+      const float min_x = L.mm[0][k], max_x = L.mm[1][k], min_y = L.mm[2][k];
+      const float iw = L.pl[3][k];
+      const int q = L.pa[k] + (int)(j - L.pre[k]);
+      const float y = row_of(q, iw, min_y);
+      int r = run_start(y, w, iw, min_y);
+      r = r > L.pa[k] ? r : L.pa[k];
+      const float x = (float)((uint32_t)r % (uint32_t)w) + min_x + (float)(q - r);
+      if (x >= max_x) continue;
+      const float sign = L.ek[6][k];
+      const float s = (L.ek[0][k] + L.ek[1][k] * x + L.ek[2][k] * y) * sign;
+      const float t = (L.ek[3][k] + L.ek[4][k] * x + L.ek[5][k] * y) * sign;
+      if (!(s > -0.0001f && t > -0.0001f && (s + t) < L.ek[7][k])) continue;
+      const float fd = fmaf(L.pl[0][k], x, fmaf(L.pl[1][k], y, L.pl[2][k]));
+      const uint32_t d = sat_u32(kMulint / fd);
+      const size_t p = (size_t)(int)y * W + (size_t)(int)x;
+      if constexpr (kMode == RasterMode::Depth) {
+        atomicMin(&depth[p], d);
+      } else if constexpr (kMode == RasterMode::Shadow) {
+        atomicMin(&depth[(size_t)L.aux[k] * W * H + p], d);
+      } else {
+        const uint32_t val = depth[p];
+        if (d > val - kBufError && d < val + kBufError) {
+          const unsigned long long key = ((unsigned long long)(L.key[k] + 1u) << 32) | L.aux[k];
+          atomicMax(&idkey[p], key);
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
 hipError_t launch_depth(const ViewParams& v, Frame* f, hipStream_t s) {
   int ncu = f->ctx->prop.multiProcessorCount > 0 ? f->ctx->prop.multiProcessorCount : 256;
-  hipLaunchKernelGGL(k_depth, dim3(ncu * 8), dim3(256), 0, s, f->cuts, f->frags, f->counters,
-                     f->cap_frag, v.width, v.height, f->depth);
+  hipLaunchKernelGGL(k_raster<RasterMode::Depth>, dim3(ncu * 8), dim3(256), 0, s, f->cuts, f->frags,
+                     f->counters, f->cap_frag, v.width, v.height, kOpSize, f->depth, f->idkey);
   return hipGetLastError();
 }
 
-// ---------------------------------------------------------------- id pass
 // kernel2's "last writer wins" is made deterministic: among fragments inside
 // the BUF_ERROR window the pixel keeps the largest (2*tri + sub) key, which
 // is the fragment a sequential in-order execution writes last.
-__global__ void __launch_bounds__(256) k_id(const CutTri* __restrict__ cuts, const uint2* __restrict__ frags,
-                                            const uint32_t* __restrict__ counters, uint32_t cap_frag,
-                                            uint32_t W, uint32_t H, const uint32_t* __restrict__ depth,
-                                            unsigned long long* __restrict__ idkey) {
-  if (counters[kOverflow]) return;
-  const uint32_t n = min(counters[kFragCount], cap_frag);
-  const float ew = (float)W, eh = (float)H;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint2 fr = frags[i];
-    const CutTri c = cuts[fr.x];
-    Setup st;
-    raster_setup(c, ew, eh, st);
-    const unsigned long long key = ((unsigned long long)(c.key + 1u) << 32) | fr.x;
-    walk(st, fr.y, kOpSize, [&](int x, int y, uint32_t mydepth) {
-      size_t p = (size_t)y * W + (size_t)x;
-      uint32_t val = depth[p];
-      if (mydepth > val - kBufError && mydepth < val + kBufError) atomicMax(&idkey[p], key);
-    });
-  }
-}
-
 hipError_t launch_id(const ViewParams& v, Frame* f, hipStream_t s) {
   int ncu = f->ctx->prop.multiProcessorCount > 0 ? f->ctx->prop.multiProcessorCount : 256;
-  hipLaunchKernelGGL(k_id, dim3(ncu * 8), dim3(256), 0, s, f->cuts, f->frags, f->counters, f->cap_frag,
-                     v.width, v.height, f->depth, f->idkey);
+  hipLaunchKernelGGL(k_raster<RasterMode::Id>, dim3(ncu * 8), dim3(256), 0, s, f->cuts, f->frags,
+                     f->counters, f->cap_frag, v.width, v.height, kOpSize, f->depth, f->idkey);
   return hipGetLastError();
 }
 
@@ -331,11 +438,14 @@ __global__ void __launch_bounds__(256) k_prearrange_shadow(const TriPos* __restr
   // Count pass over the touched faces, then allocate, then write.
   uint32_t ncut = 0, nfrag = 0;
   const bool two_sided = live && (G.flags & OCR_FEATURE_TWO_SIDED) > 0;
+#pragma unroll
   for (int kk = 0; kk < 6; kk++) {
     if (!(touched >> kk & 1)) continue;
     f3 tp[2][3];
     int num = transform_clip_project(T, G, lpos, ft.f[kk], efov, ew, eh, tp);
-    for (int i = 0; i < num; i++) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      if (i >= num) continue;
       float rc;
       uint32_t n = cull_count(tp[i], ew, eh, two_sided, kOpSizeLight, rc);
       if (n) { ncut++; nfrag += n; }
@@ -348,11 +458,14 @@ __global__ void __launch_bounds__(256) k_prearrange_shadow(const TriPos* __restr
     if ((threadIdx.x & (kWave - 1)) == 0) atomicOr(&counters[kOverflow], 1u);
     return;
   }
+#pragma unroll
   for (int kk = 0; kk < 6; kk++) {
     if (!(touched >> kk & 1)) continue;
     f3 tp[2][3];
     int num = transform_clip_project(T, G, lpos, ft.f[kk], efov, ew, eh, tp);
-    for (int i = 0; i < num; i++) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      if (i >= num) continue;
       float rc;
       uint32_t n = cull_count(tp[i], ew, eh, two_sided, kOpSizeLight, rc);
       if (!n) continue;
@@ -361,26 +474,6 @@ __global__ void __launch_bounds__(256) k_prearrange_shadow(const TriPos* __restr
       f += n;
       c++;
     }
-  }
-}
-
-__global__ void __launch_bounds__(256) k_depth_shadow(const CutTri* __restrict__ cuts,
-                                                      const uint2* __restrict__ frags,
-                                                      const uint32_t* __restrict__ counters,
-                                                      uint32_t cap_frag, uint32_t L,
-                                                      uint32_t* __restrict__ map) {
-  if (counters[kOverflow]) return;
-  const uint32_t n = min(counters[kFragCount], cap_frag);
-  const float ew = (float)L;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint2 fr = frags[i];
-    const CutTri c = cuts[fr.x];
-    Setup st;
-    raster_setup(c, ew, ew, st);
-    uint32_t* face = map + (size_t)c.key * L * L;
-    walk(st, fr.y, kOpSizeLight, [&](int x, int y, uint32_t d) {
-      atomicMin(&face[(size_t)y * L + (size_t)x], d);
-    });
   }
 }
 
@@ -398,8 +491,9 @@ hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, 
                        sc->ntri, sc->objs, sc->nobj, lpos[0], lpos[1], lpos[2], sc->lbuf_dim, only_static,
                        ft, sc->sh_cuts, sc->sh_cap_cut, sc->sh_frags, sc->sh_cap_frag, sc->sh_counters);
   int ncu = sc->ctx->prop.multiProcessorCount > 0 ? sc->ctx->prop.multiProcessorCount : 256;
-  hipLaunchKernelGGL(k_depth_shadow, dim3(ncu * 8), dim3(256), 0, s, sc->sh_cuts, sc->sh_frags,
-                     sc->sh_counters, sc->sh_cap_frag, sc->lbuf_dim, map);
+  hipLaunchKernelGGL(k_raster<RasterMode::Shadow>, dim3(ncu * 8), dim3(256), 0, s, sc->sh_cuts, sc->sh_frags,
+                     sc->sh_counters, sc->sh_cap_frag, sc->lbuf_dim, sc->lbuf_dim, kOpSizeLight, map,
+                     (unsigned long long*)nullptr);
   return hipGetLastError();
 }
 

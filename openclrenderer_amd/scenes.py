@@ -497,3 +497,30 @@ def random_scene(seed, n_tris=60, width=64, height=48, n_objs=4, near_cross=True
     sc = Scene(tris, objs, lights, [tex], 64, f"random{seed}")
     view = make_view(width, height, (0.0, 0.0, -10.0), c_rot=tuple(rng.uniform(-0.2, 0.2, 3)), hfov=70.0)
     return sc, view
+
+
+def dump_for_facade(sc, view, hfov, path):
+    """Writes a scene for ocr_facade_demo: one container per object, the
+    objects' triangles in object order (object ids must be contiguous and
+    ascending so the triangle ids are unchanged), textures, lights, view and
+    horizontal FOV."""
+    import struct
+    oids = sc.tris["v"]["object_id"][:, 0]
+    if np.any(np.diff(oids.astype(np.int64)) < 0):
+        raise ValueError("triangles must be grouped by ascending object id")
+    with open(path, "wb") as f:
+        f.write(b"OCRSCN01")
+        f.write(struct.pack("<I", sc.objs.shape[0]))
+        for o in range(sc.objs.shape[0]):
+            sel = np.ascontiguousarray(sc.tris[oids == o])
+            f.write(struct.pack("<III", 1, sel.shape[0], int(sc.objs["tid"][o])))
+            f.write(np.ascontiguousarray(sc.objs[o:o + 1]).tobytes())
+            f.write(sel.tobytes())
+        f.write(struct.pack("<I", len(sc.textures)))
+        for t in sc.textures:
+            f.write(struct.pack("<II", t.width, t.height))
+            f.write(np.ascontiguousarray(t.rgba, dtype=np.uint8).tobytes())
+        f.write(struct.pack("<I", sc.lights.shape[0]))
+        f.write(np.ascontiguousarray(sc.lights).tobytes())
+        f.write(np.ascontiguousarray(view).tobytes())
+        f.write(struct.pack("<f", hfov))
