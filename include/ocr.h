@@ -227,6 +227,42 @@ int  ocr_frame_count_texels(ocr_scene* s, const ocr_view* view, ocr_frame* f,
                             uint64_t* atlas_texels, uint64_t* shadow_texels,
                             void* stream);
 
+/* ---- batched form: many views of several resident scenes ------------- */
+
+/* One render of a batch covers every view with one launch per pass; per-view
+ * screen size, FOV and camera are runtime values (the reference bakes them
+ * into its program, ocl.h:227-240, and draws one view per call,
+ * engine.cpp:1794-2161). */
+typedef struct ocr_batch ocr_batch;
+typedef struct ocr_batch_tuning {
+    uint32_t raster_blocks_per_cu;   /* persistent depth / id grid: blocks per CU (1..32) */
+    uint32_t shade_tile_w_log2;      /* shading tile: 2^k wide, 256 >> k high (0..8) */
+    uint32_t _pad[2];
+} ocr_batch_tuning;
+
+int  ocr_batch_create(ocr_ctx* ctx, uint32_t max_views, ocr_batch** out);
+void ocr_batch_destroy(ocr_batch* b);
+/* View i renders scenes[i] into frames[i] (distinct frames, frame size =
+ * view size).  Uploads the view and scene tables and sizes the shared work
+ * lists; enqueues no rendering. */
+int  ocr_batch_set_views(ocr_batch* b, ocr_scene* const* scenes, const ocr_view* views,
+                         ocr_frame* const* frames, uint32_t n);
+int  ocr_batch_set_tuning(ocr_batch* b, const ocr_batch_tuning* t);
+int  ocr_batch_get_tuning(ocr_batch* b, ocr_batch_tuning* t);
+/* hipEvent timing around every render / graph launch (average in
+ * ocr_batch_get_stats ms_total). */
+int  ocr_batch_set_profiling(ocr_batch* b, int enable);
+/* clear -> prearrange -> depth -> id -> shade for all views on `stream`. */
+int  ocr_batch_render(ocr_batch* b, void* stream);
+/* The same five launches captured once into a hipGraph and replayed. */
+int  ocr_batch_launch_graph(ocr_batch* b, void* stream);
+/* Synchronises `stream`; counters of the last render (OCR_ERR_OVERFLOW if the
+ * shared lists overflowed: they grow on the next render). */
+int  ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out);
+/* Times raster grid sizes x shading tile shapes on the current views and
+ * keeps the fastest (reps graph launches each). */
+int  ocr_batch_autotune(ocr_batch* b, void* stream, uint32_t reps, ocr_batch_tuning* best);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,352 @@
+// batch.hip — many views of several resident scenes in one render: one
+// launch per pass (clear, prearrange, depth, id, shade) covers every view,
+// with per-view work ranges found from device view tables, shared work lists,
+// hipGraph capture of the whole multi-view frame, and an autotuner for the
+// raster grid size and the shading tile shape.
+//
+// The reference renders one view per draw call with screen size, FOV and
+// light-map size baked into the program (ocl.h:227-240, engine.cpp:1794-2161);
+// here they are per-view runtime values so views of different sizes share a
+// launch.
+#pragma clang fp contract(off)
+
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <new>
+#include <vector>
+
+#include "ocr_internal.h"
+
+using namespace ocr;
+
+struct ocr_batch {
+  Ctx* ctx = nullptr;
+  uint32_t max_views = 0, nviews = 0;
+  std::vector<Scene*> scenes;        // distinct scenes, table order
+  std::vector<DevView> host_views;
+  DevView* d_views = nullptr;
+  DevScene* d_scenes = nullptr;
+  uint32_t tri_slots = 0, pix_slots = 0, tiles = 0;
+  uint64_t sum_tris = 0, sum_pix = 0;
+  CutTri* cuts = nullptr;
+  uint2* frags = nullptr;
+  uint32_t* counters = nullptr;
+  uint32_t cap_cut = 0, cap_frag = 0, want_frag = 0;
+  ocr_batch_tuning tune{8, 4, {0, 0}};
+  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool profiling = false;
+  std::vector<hipEvent_t> ev;        // kRing x 2 (begin, end of a batch)
+  int n_profiled = 0;
+};
+
+static constexpr int kBatchRing = 1024;
+
+static void drop_graph(ocr_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+static PassArgs batch_pass(const ocr_batch* b) {
+  PassArgs a;
+  memset(&a, 0, sizeof(a));
+  a.views = b->d_views;
+  a.scenes = b->d_scenes;
+  a.nviews = b->nviews;
+  a.tri_slots = b->tri_slots;
+  a.pix_slots = b->pix_slots;
+  a.tiles = b->tiles;
+  a.tile_w_log2 = b->tune.shade_tile_w_log2;
+  a.lists = Lists{b->cuts, b->frags, b->counters, b->cap_cut, b->cap_frag};
+  return a;
+}
+
+// Recompute the per-view ranges (they depend on the tile shape) and upload
+// the view table.
+static int upload_views(ocr_batch* b) {
+  uint32_t tri = 0, pix = 0, tiles = 0;
+  for (auto& v : b->host_views) {
+    const Scene* s = b->scenes[v.scene];
+    v.tri_begin = tri;
+    v.pix_begin = pix;
+    v.tile_begin = tiles;
+    tri += (s->ntri + 255) / 256 * 256;
+    pix += (v.v.width * v.v.height + 255) / 256 * 256;
+    tiles += view_tiles(v.v.width, v.v.height, b->tune.shade_tile_w_log2);
+  }
+  b->tri_slots = tri;
+  b->pix_slots = pix;
+  b->tiles = tiles;
+  OCR_HIP_CHECK(hipMemcpy(b->d_views, b->host_views.data(), sizeof(DevView) * b->nviews, hipMemcpyHostToDevice));
+  return OCR_OK;
+}
+
+static int ensure_lists(ocr_batch* b) {
+  uint64_t want_cut = std::min<uint64_t>(b->sum_tris * 2 + 64, 0xFFFFFFF0ull);
+  uint64_t want_frag = std::min<uint64_t>(
+      std::max<uint64_t>(b->sum_tris * 2 + b->sum_pix / 64 + 65536, b->want_frag), 0xFFFFFFF0ull);
+  if (b->cuts && b->cap_cut >= want_cut && b->frags && b->cap_frag >= want_frag) return OCR_OK;
+  OCR_HIP_CHECK(hipDeviceSynchronize());
+  drop_graph(b);
+  if (b->cuts) (void)hipFree(b->cuts);
+  if (b->frags) (void)hipFree(b->frags);
+  b->cuts = nullptr;
+  b->frags = nullptr;
+  b->cap_cut = b->cap_frag = 0;
+  hipError_t e = hipMalloc((void**)&b->cuts, sizeof(CutTri) * want_cut);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->frags, sizeof(uint2) * want_frag);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(OCR_ERR_OOM, std::string("batch work lists: ") + hipGetErrorString(e));
+  }
+  b->cap_cut = (uint32_t)want_cut;
+  b->cap_frag = (uint32_t)want_frag;
+  return OCR_OK;
+}
+
+static hipError_t enqueue_batch(ocr_batch* b, hipStream_t st) {
+  const PassArgs pa = batch_pass(b);
+  hipError_t e = launch_clear(pa, st);
+  if (e == hipSuccess) e = launch_prearrange(pa, st);
+  if (e == hipSuccess) e = launch_depth(pa, (int)b->tune.raster_blocks_per_cu, st);
+  if (e == hipSuccess) e = launch_id(pa, (int)b->tune.raster_blocks_per_cu, st);
+  if (e == hipSuccess) e = launch_shade(pa, b->ctx->face_sc, st);
+  return e;
+}
+
+extern "C" {
+
+int ocr_batch_create(ocr_ctx* ctx, uint32_t max_views, ocr_batch** out) {
+  if (!ctx || !out || max_views == 0 || max_views > kMaxViews)
+    return fail(OCR_ERR_INVALID, "ocr_batch_create: bad argument");
+  *out = nullptr;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  OCR_HIP_CHECK(hipSetDevice(c->device));
+  ocr_batch* b = new (std::nothrow) ocr_batch();
+  if (!b) return fail(OCR_ERR_OOM, "ocr_batch_create: host allocation");
+  b->ctx = c;
+  b->max_views = max_views;
+  hipError_t e = hipMalloc((void**)&b->d_views, sizeof(DevView) * max_views);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_scenes, sizeof(DevScene) * max_views);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->counters, 4 * kCounterWords);
+  if (e == hipSuccess) e = hipMemset(b->counters, 0, 4 * kCounterWords);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    ocr_batch_destroy(b);
+    return fail(OCR_ERR_HIP, std::string("ocr_batch_create: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return OCR_OK;
+}
+
+void ocr_batch_destroy(ocr_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipDeviceSynchronize();
+  drop_graph(b);
+  if (b->d_views) (void)hipFree(b->d_views);
+  if (b->d_scenes) (void)hipFree(b->d_scenes);
+  if (b->cuts) (void)hipFree(b->cuts);
+  if (b->frags) (void)hipFree(b->frags);
+  if (b->counters) (void)hipFree(b->counters);
+  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  for (hipEvent_t e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete b;
+}
+
+int ocr_batch_set_views(ocr_batch* b, ocr_scene* const* scenes, const ocr_view* views, ocr_frame* const* frames,
+                        uint32_t n) {
+  if (!b || !scenes || !views || !frames || n == 0 || n > b->max_views)
+    return fail(OCR_ERR_INVALID, "ocr_batch_set_views: bad argument");
+  OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
+  std::vector<Scene*> uniq;
+  std::vector<DevView> dv(n);
+  uint64_t sum_tris = 0, sum_pix = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    Scene* s = static_cast<Scene*>(scenes[i]);
+    Frame* f = static_cast<Frame*>(frames[i]);
+    if (!s || !f) return fail(OCR_ERR_INVALID, "ocr_batch_set_views: null scene or frame");
+    if (views[i].width != f->width || views[i].height != f->height)
+      return fail(OCR_ERR_INVALID, "ocr_batch_set_views: view size differs from its frame");
+    if (!(views[i].fov_const > 0.f)) return fail(OCR_ERR_INVALID, "ocr_batch_set_views: fov_const must be > 0");
+    if (!s->atlas.pages || (s->nlights && !s->lights))
+      return fail(OCR_ERR_INVALID, "ocr_batch_set_views: scene needs its atlas and lights uploaded");
+    if (s->nobj && s->max_tid >= std::max<uint32_t>(s->atlas.mip_start, 1))
+      return fail(OCR_ERR_INVALID, "ocr_batch_set_views: an object's texture id is outside the atlas");
+    for (uint32_t j = 0; j < i; j++)
+      if (frames[j] == frames[i]) return fail(OCR_ERR_INVALID, "ocr_batch_set_views: a frame appears twice");
+    uint32_t si = 0;
+    while (si < uniq.size() && uniq[si] != s) si++;
+    if (si == uniq.size()) uniq.push_back(s);
+    dv[i] = dev_view(&views[i], si, f);
+    sum_tris += s->ntri;
+    sum_pix += (uint64_t)views[i].width * views[i].height;
+  }
+  std::vector<DevScene> ds(uniq.size());
+  for (size_t i = 0; i < uniq.size(); i++) {
+    if (!uniq[i]->lights) {
+      int rc = ocr_lights_upload(static_cast<ocr_scene*>(uniq[i]), nullptr, 0, uniq[i]->lbuf_dim);
+      if (rc) return rc;
+    }
+    ds[i] = dev_scene(uniq[i]);
+  }
+  drop_graph(b);
+  b->scenes = uniq;
+  b->host_views = dv;
+  b->nviews = n;
+  b->sum_tris = sum_tris;
+  b->sum_pix = sum_pix;
+  OCR_HIP_CHECK(hipMemcpy(b->d_scenes, ds.data(), sizeof(DevScene) * ds.size(), hipMemcpyHostToDevice));
+  int rc = upload_views(b);
+  if (rc) return rc;
+  return ensure_lists(b);
+}
+
+int ocr_batch_set_tuning(ocr_batch* b, const ocr_batch_tuning* t) {
+  if (!b || !t || t->raster_blocks_per_cu == 0 || t->raster_blocks_per_cu > 32 || t->shade_tile_w_log2 > 8)
+    return fail(OCR_ERR_INVALID, "ocr_batch_set_tuning: bad tuning");
+  b->tune = *t;
+  drop_graph(b);
+  if (b->nviews) return upload_views(b);
+  return OCR_OK;
+}
+
+int ocr_batch_get_tuning(ocr_batch* b, ocr_batch_tuning* t) {
+  if (!b || !t) return fail(OCR_ERR_INVALID, "ocr_batch_get_tuning: bad argument");
+  *t = b->tune;
+  return OCR_OK;
+}
+
+int ocr_batch_set_profiling(ocr_batch* b, int enable) {
+  if (!b) return fail(OCR_ERR_INVALID, "ocr_batch_set_profiling: null batch");
+  b->profiling = enable != 0;
+  b->n_profiled = 0;
+  if (b->profiling && b->ev.empty()) {
+    OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
+    b->ev.assign(2 * kBatchRing, nullptr);
+    for (auto& e : b->ev) OCR_HIP_CHECK(hipEventCreate(&e));
+  }
+  return OCR_OK;
+}
+
+int ocr_batch_render(ocr_batch* b, void* stream) {
+  if (!b || b->nviews == 0) return fail(OCR_ERR_INVALID, "ocr_batch_render: no views");
+  OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
+  int rc = ensure_lists(b);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const bool prof = b->profiling && b->n_profiled < kBatchRing;
+  if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled], st));
+  OCR_HIP_CHECK(enqueue_batch(b, st));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled++ + 1], st));
+  return OCR_OK;
+}
+
+int ocr_batch_launch_graph(ocr_batch* b, void* stream) {
+  if (!b || b->nviews == 0) return fail(OCR_ERR_INVALID, "ocr_batch_launch_graph: no views");
+  OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
+  int rc = ensure_lists(b);
+  if (rc) return rc;
+  if (!b->exec) {
+    // Capture on the batch's own stream (the legacy stream cannot be
+    // captured); the passes use fill kernels, never memset nodes.
+    OCR_HIP_CHECK(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_batch(b, b->capture_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->capture_stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(OCR_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    b->graph = g;
+    OCR_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const bool prof = b->profiling && b->n_profiled < kBatchRing;
+  if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled], st));
+  OCR_HIP_CHECK(hipGraphLaunch(b->exec, st));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled++ + 1], st));
+  return OCR_OK;
+}
+
+int ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out) {
+  if (!b || !out) return fail(OCR_ERR_INVALID, "ocr_batch_get_stats: bad argument");
+  OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
+  OCR_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  uint32_t c[kCounterWords];
+  OCR_HIP_CHECK(hipMemcpy(c, b->counters, sizeof(c), hipMemcpyDeviceToHost));
+  memset(out, 0, sizeof(*out));
+  out->n_cut = c[kCutCount];
+  out->n_frag = c[kFragCount];
+  out->overflow = c[kOverflow] ? 1u : 0u;
+  out->n_profiled = (uint32_t)b->n_profiled;
+  if (b->n_profiled) {
+    double sum = 0;
+    for (int r = 0; r < b->n_profiled; r++) {
+      float ms = 0.f;
+      OCR_HIP_CHECK(hipEventElapsedTime(&ms, b->ev[2 * r], b->ev[2 * r + 1]));
+      sum += ms;
+    }
+    out->ms_total = (float)(sum / b->n_profiled);
+  }
+  if (c[kOverflow]) {
+    b->want_frag = (uint32_t)std::min<uint64_t>((uint64_t)c[kFragCount] + c[kFragCount] / 4 + 1024, 0xFFFFFFF0ull);
+    return fail(OCR_ERR_OVERFLOW, "ocr_batch_get_stats: fragment list overflowed; render again");
+  }
+  return OCR_OK;
+}
+
+// Times every candidate tuning on the current views (graph launches, after
+// one warm-up each) and keeps the fastest.
+int ocr_batch_autotune(ocr_batch* b, void* stream, uint32_t reps, ocr_batch_tuning* best_out) {
+  if (!b || b->nviews == 0) return fail(OCR_ERR_INVALID, "ocr_batch_autotune: no views");
+  OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
+  if (reps == 0) reps = 5;
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  OCR_HIP_CHECK(hipEventCreate(&e0));
+  OCR_HIP_CHECK(hipEventCreate(&e1));
+  const uint32_t blocks[] = {2, 4, 8, 16};
+  const uint32_t tiles[] = {3, 4, 5, 6};     // tile widths 8, 16, 32, 64
+  ocr_batch_tuning best = b->tune;
+  float best_ms = 1e30f;
+  const bool prof = b->profiling;
+  b->profiling = false;
+  int status = OCR_OK;
+  for (uint32_t bl : blocks)
+    for (uint32_t tl : tiles) {
+      ocr_batch_tuning t{bl, tl, {0, 0}};
+      if ((status = ocr_batch_set_tuning(b, &t))) break;
+      if ((status = ocr_batch_launch_graph(b, stream))) break;
+      hipError_t e = hipEventRecord(e0, st);
+      for (uint32_t r = 0; r < reps && e == hipSuccess; r++)
+        e = hipGraphLaunch(b->exec, st);
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        status = fail(OCR_ERR_HIP, std::string("ocr_batch_autotune: ") + hipGetErrorString(e));
+        break;
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+  b->profiling = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (status) return status;
+  if (best_out) *best_out = best;
+  return ocr_batch_set_tuning(b, &best);
+}
+
+}  // extern "C"

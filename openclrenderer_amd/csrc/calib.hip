@@ -24,3 +24,67 @@ extern "C" int ocr_calib_copy(size_t bytes, int iters) {
   (void)hipFree(b);
   return e == hipSuccess ? 0 : -3;
 }
+
+// Atomic throughput probes: n u32 atomicMin (or u64 atomicMax) on addresses
+// drawn from a hash of the work-item index over `words` targets; `cluster`
+// consecutive work items share a 64-address window (1 = uniform random).
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return x;
+}
+
+__global__ void __launch_bounds__(256) k_calib_atomic_min(uint32_t* __restrict__ buf, uint32_t words, size_t n,
+                                                          uint32_t cluster) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    uint32_t g = (uint32_t)(i / cluster);
+    uint32_t a = cluster == 1 ? mix32((uint32_t)i) % words
+                              : (mix32(g) % (words / 64)) * 64 + (uint32_t)(i % 64);
+    atomicMin(&buf[a], mix32((uint32_t)i ^ 0x9e3779b9u));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_calib_atomic_max64(unsigned long long* __restrict__ buf, uint32_t words,
+                                                            size_t n, uint32_t cluster) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    uint32_t g = (uint32_t)(i / cluster);
+    uint32_t a = cluster == 1 ? mix32((uint32_t)i) % words
+                              : (mix32(g) % (words / 64)) * 64 + (uint32_t)(i % 64);
+    atomicMax(&buf[a], (unsigned long long)mix32((uint32_t)i));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_calib_scatter_store(uint32_t* __restrict__ buf, uint32_t words, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) buf[mix32((uint32_t)i) % words] = (uint32_t)i;
+}
+
+// Returns the average kernel time in microseconds of `iters` launches
+// (mode 0: u32 atomicMin, 1: u64 atomicMax, 2: plain scattered store).
+extern "C" double ocr_calib_atomics(int mode, uint32_t words, size_t n, uint32_t cluster, int iters) {
+  void* buf = nullptr;
+  if (hipMalloc(&buf, (size_t)words * 8) != hipSuccess) return -1.0;
+  if (hipMemset(buf, 0xFF, (size_t)words * 8) != hipSuccess) return -2.0;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  dim3 grid(4096), block(256);
+  for (int w = 0; w < 2 + iters; w++) {
+    if (w == 2) (void)hipEventRecord(e0, nullptr);
+    if (mode == 0) hipLaunchKernelGGL(k_calib_atomic_min, grid, block, 0, nullptr, (uint32_t*)buf, words, n, cluster);
+    else if (mode == 1) hipLaunchKernelGGL(k_calib_atomic_max64, grid, block, 0, nullptr, (unsigned long long*)buf, words, n, cluster);
+    else hipLaunchKernelGGL(k_calib_scatter_store, grid, block, 0, nullptr, (uint32_t*)buf, words, n);
+  }
+  (void)hipEventRecord(e1, nullptr);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(buf);
+  return e == hipSuccess ? (double)ms * 1e3 / iters : -3.0;
+}

@@ -70,6 +70,65 @@ struct ViewParams {
   float clear[4];
 };
 
+// One scene as the kernels see it.
+struct DevScene {
+  const TriPos* tri_pos;
+  const TriAttr* tri_attr;
+  const ObjDesc* objs;
+  const uint8_t* pages;
+  const uint32_t* nums;
+  const uint32_t* sizes;
+  const ocr_light* lights;
+  const uint32_t* shadow_maps;
+  const uint32_t* static_maps;
+  uint32_t ntri, nobj, mip_start, nlights, L, _pad;
+};
+
+// One view of a render.  Per-view work ranges start on 256-slot boundaries
+// so every 256-thread block of the clear, prearrange and shade launches
+// belongs to exactly one view.
+struct DevView {
+  ViewParams v;
+  uint32_t scene;          // index into the scene table
+  uint32_t tri_begin;      // first triangle slot of this view (multiple of 256)
+  uint32_t pix_begin;      // first pixel slot (multiple of 256)
+  uint32_t tile_begin;     // first 16x16 shading tile
+  uint32_t* depth;
+  unsigned long long* idkey;
+  uint32_t* rgba;
+  uint32_t* normals;
+};
+
+// Work lists and counters shared by the views of one render.
+struct Lists {
+  CutTri* cuts;
+  uint2* frags;            // (cut index, chunk | view << kViewShift)
+  uint32_t* counters;
+  uint32_t cap_cut, cap_frag;
+};
+constexpr uint32_t kViewShift = 20;
+constexpr uint32_t kChunkMask = (1u << kViewShift) - 1;
+constexpr uint32_t kMaxViews = 1u << (32 - kViewShift);
+
+// Everything the camera passes need.  A single view travels inline in the
+// kernel arguments (views == nullptr); a batch reads device tables.
+struct PassArgs {
+  DevView view0;
+  DevScene scene0;
+  const DevView* views;
+  const DevScene* scenes;
+  uint32_t nviews;
+  uint32_t tri_slots, pix_slots, tiles;
+  uint32_t tile_w_log2;    // shading tile = (1 << tile_w_log2) x (256 >> tile_w_log2)
+  Lists lists;
+};
+
+// Shading tiles of one view for a tile width of 1 << log2 (256 pixels each).
+__host__ __device__ inline uint32_t view_tiles(uint32_t w, uint32_t h, uint32_t log2w) {
+  const uint32_t tw = 1u << log2w, th = 256u >> log2w;
+  return ((w + tw - 1) / tw) * ((h + th - 1) / th);
+}
+
 // Device counters of one frame's work lists.
 enum Counter : int {
   kCutCount = 0,
@@ -146,13 +205,17 @@ int fail(int code, const std::string& msg);
   } while (0)
 
 // ---- kernel launchers (defined next to their kernels) --------------------
-hipError_t launch_clear_frame(Frame* f, hipStream_t s);
-hipError_t launch_prearrange(const Scene* sc, const ViewParams& v, Frame* f, hipStream_t s);
-hipError_t launch_depth(const ViewParams& v, Frame* f, hipStream_t s);
-hipError_t launch_id(const ViewParams& v, Frame* f, hipStream_t s);
-hipError_t launch_shade(const Scene* sc, const ViewParams& v, Frame* f, hipStream_t s);
-hipError_t launch_shade_count(const Scene* sc, const ViewParams& v, Frame* f, uint32_t* tex_bits,
+hipError_t launch_clear(const PassArgs& a, hipStream_t s);
+hipError_t launch_prearrange(const PassArgs& a, hipStream_t s);
+hipError_t launch_depth(const PassArgs& a, int blocks_per_cu, hipStream_t s);
+hipError_t launch_id(const PassArgs& a, int blocks_per_cu, hipStream_t s);
+hipError_t launch_shade(const PassArgs& a, const SinCos face[6], hipStream_t s);
+hipError_t launch_shade_count(const PassArgs& a, const SinCos face[6], uint32_t* tex_bits,
                               uint32_t* shadow_bits, uint32_t* static_bits, hipStream_t s);
+// helpers shared by the plain and batched renders (runtime.hip)
+DevScene dev_scene(const Scene* sc);
+DevView dev_view(const ocr_view* v, uint32_t scene, Frame* f);
+int device_cu_count();
 hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, uint32_t* map,
                                hipStream_t s);
 hipError_t launch_atlas_upload(const Atlas& a, uint32_t tid, const uint8_t* src, uint32_t w,
@@ -164,3 +227,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
 constexpr int kWave = 64;
 
 }  // namespace ocr
+
+// The C ABI's opaque handles are the internal records.
+struct ocr_ctx : public ocr::Ctx {};
+struct ocr_scene : public ocr::Scene {};
+struct ocr_frame : public ocr::Frame {};

@@ -16,6 +16,8 @@
 // touching the lists.
 #pragma clang fp contract(off)
 
+#include <string.h>
+
 #include "ocr_internal.h"
 
 namespace ocr {
@@ -147,17 +149,39 @@ __device__ __forceinline__ int transform_clip_project(const TriPos& T, const Obj
   return n;
 }
 
-// ------------------------------------------------------------------ clear
-__global__ void __launch_bounds__(256) k_clear_frame(uint32_t* __restrict__ depth,
-                                                     unsigned long long* __restrict__ idkey,
-                                                     uint32_t* __restrict__ counters, size_t npix) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * blockDim.x;
-  if (i < kCounterWords) counters[i] = 0u;
-  for (; i < npix; i += stride) {
-    depth[i] = 0xFFFFFFFFu;
-    idkey[i] = 0ull;
+// ------------------------------------------------------------------ views
+__device__ __forceinline__ const DevView& view_at(const PassArgs& a, uint32_t i) {
+  return a.views ? a.views[i] : a.view0;
+}
+__device__ __forceinline__ const DevScene& scene_at(const PassArgs& a, uint32_t i) {
+  return a.scenes ? a.scenes[i] : a.scene0;
+}
+
+// The view owning a 256-slot block: the last view whose range starts at or
+// before `slot` (ranges are 256-aligned, so the answer is block-uniform).
+enum class Range { Tri, Pix, Tile };
+template <Range kR>
+__device__ __forceinline__ uint32_t view_of(const PassArgs& a, uint32_t slot) {
+  if (!a.views) return 0;
+  uint32_t lo = 0, hi = a.nviews;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    const DevView& m = a.views[mid];
+    uint32_t b = kR == Range::Tri ? m.tri_begin : (kR == Range::Pix ? m.pix_begin : m.tile_begin);
+    if (b <= slot) lo = mid; else hi = mid;
   }
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
+}
+
+// ------------------------------------------------------------------ clear
+__global__ void __launch_bounds__(256) k_clear(PassArgs a) {
+  const uint32_t slot = blockIdx.x * 256u;
+  if (blockIdx.x == 0 && threadIdx.x < kCounterWords) a.lists.counters[threadIdx.x] = 0u;
+  const DevView& V = view_at(a, view_of<Range::Pix>(a, slot));
+  const uint32_t i = slot - V.pix_begin + threadIdx.x;
+  if (i >= V.v.width * V.v.height) return;
+  V.depth[i] = 0xFFFFFFFFu;
+  V.idkey[i] = 0ull;
 }
 
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
@@ -173,10 +197,8 @@ static unsigned grid_for(size_t n, unsigned block, unsigned max_blocks) {
   return (unsigned)b;
 }
 
-hipError_t launch_clear_frame(Frame* f, hipStream_t s) {
-  size_t npix = (size_t)f->width * f->height;
-  hipLaunchKernelGGL(k_clear_frame, dim3(grid_for(npix, 256, 4096)), dim3(256), 0, s, f->depth,
-                     f->idkey, f->counters, npix);
+hipError_t launch_clear(const PassArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear, dim3(a.pix_slots / 256 > 0 ? a.pix_slots / 256 : 1), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -186,22 +208,23 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
 }
 
 // ------------------------------------------------------------- prearrange
-__global__ void __launch_bounds__(256) k_prearrange(const TriPos* __restrict__ tris, uint32_t ntri,
-                                                    const ObjDesc* __restrict__ objs, uint32_t nobj,
-                                                    ViewParams v, CutTri* __restrict__ cuts,
-                                                    uint32_t cap_cut, uint2* __restrict__ frags,
-                                                    uint32_t cap_frag, uint32_t* __restrict__ counters) {
-  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_prearrange(PassArgs a) {
+  const uint32_t slot = blockIdx.x * 256u;
+  const uint32_t vi = view_of<Range::Tri>(a, slot);
+  const DevView& V = view_at(a, vi);
+  const DevScene& S = scene_at(a, V.scene);
+  const ViewParams& v = V.v;
+  const uint32_t id = slot - V.tri_begin + threadIdx.x;
   const float ew = (float)v.width, eh = (float)v.height;
   f3 tp[2][3];
   float rc[2] = {0.f, 0.f};
   uint32_t nf[2] = {0u, 0u};
   uint32_t oid = 0;
-  if (id < ntri) {
-    const TriPos T = tris[id];
+  if (id < S.ntri) {
+    const TriPos T = S.tri_pos[id];
     oid = T.oid;
-    if (oid < nobj) {
-      const ObjDesc G = objs[oid];
+    if (oid < S.nobj) {
+      const ObjDesc G = S.objs[oid];
       f3 cpos = mk3(v.cpos[0], v.cpos[1], v.cpos[2]);
       if (!(fast_length(mk3(G.wp[0], G.wp[1], G.wp[2]) - cpos) > kDepthFar)) {
         int num = transform_clip_project(T, G, cpos, v.cam, v.fov, ew, eh, tp);
@@ -216,28 +239,28 @@ __global__ void __launch_bounds__(256) k_prearrange(const TriPos* __restrict__ t
   // write a pixel, so it is not stored either.
   const uint32_t k0 = nf[0] > 0, k1 = nf[1] > 0;
   uint32_t cut_end, frag_end;
-  uint32_t cut0 = wave_alloc(&counters[kCutCount], k0 + k1, cut_end);
-  uint32_t frag0 = wave_alloc(&counters[kFragCount], nf[0] + nf[1], frag_end);
-  if (cut_end > cap_cut || frag_end > cap_frag) {
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicOr(&counters[kOverflow], 1u);
+  const Lists& L = a.lists;
+  uint32_t cut0 = wave_alloc(&L.counters[kCutCount], k0 + k1, cut_end);
+  uint32_t frag0 = wave_alloc(&L.counters[kFragCount], nf[0] + nf[1], frag_end);
+  if (cut_end > L.cap_cut || frag_end > L.cap_frag) {
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicOr(&L.counters[kOverflow], 1u);
     return;
   }
   uint32_t c = cut0, f = frag0;
+  const uint32_t vbits = vi << kViewShift;
 #pragma unroll
   for (int i = 0; i < 2; i++) {
     if (!nf[i]) continue;
-    store_cut(&cuts[c], tp[i], rc[i], 2u * id + (uint32_t)i, oid);
-    for (uint32_t a = 0; a < nf[i]; a++) frags[f + a] = make_uint2(c, a);
+    store_cut(&L.cuts[c], tp[i], rc[i], 2u * id + (uint32_t)i, oid);
+    for (uint32_t q = 0; q < nf[i]; q++) L.frags[f + q] = make_uint2(c, q | vbits);
     f += nf[i];
     c++;
   }
 }
 
-hipError_t launch_prearrange(const Scene* sc, const ViewParams& v, Frame* f, hipStream_t s) {
-  if (sc->ntri == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_prearrange, dim3((sc->ntri + 255) / 256), dim3(256), 0, s, sc->tri_pos,
-                     sc->ntri, sc->objs, sc->nobj, v, f->cuts, f->cap_cut, f->frags, f->cap_frag,
-                     f->counters);
+hipError_t launch_prearrange(const PassArgs& a, hipStream_t s) {
+  if (a.tri_slots == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prearrange, dim3(a.tri_slots / 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -266,8 +289,10 @@ struct FragLds {
   float ek[8][kWave];      // edge test: s0, s1, s2, t0, t1, t2, sign, 2.0001*A*sign
   int w[kWave];
   int pa[kWave];
-  uint32_t aux[kWave];     // cut index (id pass) or cube face (shadow pass)
-  uint32_t key[kWave];     // 2*tri + sub (id pass)
+  uint32_t pitch[kWave];   // row pitch of the target buffer
+  uint32_t* dst[kWave];    // depth buffer / cube face
+  unsigned long long* ids[kWave];
+  unsigned long long key[kWave];
   uint32_t pre[kWave + 1]; // exclusive prefix of sample counts
 };
 
@@ -286,18 +311,19 @@ __device__ __forceinline__ int run_start(float Y, int w, float iw, float min_y) 
 
 enum class RasterMode { Depth, Id, Shadow };
 
+struct ShadowTarget {
+  uint32_t* map;   // 6 faces of L x L
+  uint32_t L;
+};
+
 template <RasterMode kMode>
-__global__ void __launch_bounds__(256) k_raster(const CutTri* __restrict__ cuts, const uint2* __restrict__ frags,
-                                                const uint32_t* __restrict__ counters, uint32_t cap_frag,
-                                                uint32_t W, uint32_t H, int op_size,
-                                                uint32_t* __restrict__ depth,
-                                                unsigned long long* __restrict__ idkey) {
+__global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh, int op_size) {
   __shared__ FragLds lds_all[256 / kWave];
-  if (counters[kOverflow]) return;
-  const uint32_t n = min(counters[kFragCount], cap_frag);
+  const Lists& Ls = a.lists;
+  if (Ls.counters[kOverflow]) return;
+  const uint32_t n = min(Ls.counters[kFragCount], Ls.cap_frag);
   const int lane = threadIdx.x & (kWave - 1);
   FragLds& L = lds_all[threadIdx.x / kWave];
-  const float ew = (float)W, eh = (float)H;
   const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
   for (uint32_t base = wave0 * kWave; base < n; base += nwaves * kWave) {
@@ -305,11 +331,24 @@ __global__ void __launch_bounds__(256) k_raster(const CutTri* __restrict__ cuts,
     const uint32_t fi = base + lane;
     uint32_t cnt = 0;
     if (fi < n) {
-      const uint2 fr = frags[fi];
-      const CutTri c = cuts[fr.x];
+      const uint2 fr = Ls.frags[fi];
+      const CutTri c = Ls.cuts[fr.x];
+      uint32_t W, H;
+      if constexpr (kMode == RasterMode::Shadow) {
+        W = H = sh.L;
+        L.dst[lane] = sh.map + (size_t)c.key * sh.L * sh.L;
+      } else {
+        const DevView& V = view_at(a, fr.y >> kViewShift);
+        W = V.v.width;
+        H = V.v.height;
+        L.dst[lane] = V.depth;
+        L.ids[lane] = V.idkey;
+        L.key[lane] = ((unsigned long long)(c.key + 1u) << 32) | fr.x;
+      }
+      L.pitch[lane] = W;
       Setup st;
-      raster_setup(c, ew, eh, st);
-      const int pa = op_size * (int)fr.y;
+      raster_setup(c, (float)W, (float)H, st);
+      const int pa = op_size * (int)(fr.y & kChunkMask);
       if (st.width > 0) {
         // samples of this fragment: q in [pa, pa + op_size] with y(q) < max_y
         const int q_end = run_start(st.mm[3], st.width, st.iwidth, st.mm[2]);
@@ -334,9 +373,6 @@ __global__ void __launch_bounds__(256) k_raster(const CutTri* __restrict__ cuts,
       L.ek[7][lane] = 2.0001f * A * sign;
       L.w[lane] = st.width;
       L.pa[lane] = pa;
-      // id pass: cut index + triangle key; shadow pass: cube face
-      L.aux[lane] = kMode == RasterMode::Id ? fr.x : c.key;
-      L.key[lane] = c.key;
     }
     const uint32_t incl = wave_incl_scan(cnt);
     const uint32_t total = __shfl(incl, kWave - 1, kWave);
@@ -366,37 +402,32 @@ __global__ void __launch_bounds__(256) k_raster(const CutTri* __restrict__ cuts,
       if (!(s > -0.0001f && t > -0.0001f && (s + t) < L.ek[7][k])) continue;
       const float fd = fmaf(L.pl[0][k], x, fmaf(L.pl[1][k], y, L.pl[2][k]));
       const uint32_t d = sat_u32(kMulint / fd);
-      const size_t p = (size_t)(int)y * W + (size_t)(int)x;
-      if constexpr (kMode == RasterMode::Depth) {
-        atomicMin(&depth[p], d);
-      } else if constexpr (kMode == RasterMode::Shadow) {
-        atomicMin(&depth[(size_t)L.aux[k] * W * H + p], d);
+      const size_t p = (size_t)(int)y * L.pitch[k] + (size_t)(int)x;
+      if constexpr (kMode == RasterMode::Id) {
+        const uint32_t val = L.dst[k][p];
+        if (d > val - kBufError && d < val + kBufError) atomicMax(&L.ids[k][p], L.key[k]);
       } else {
-        const uint32_t val = depth[p];
-        if (d > val - kBufError && d < val + kBufError) {
-          const unsigned long long key = ((unsigned long long)(L.key[k] + 1u) << 32) | L.aux[k];
-          atomicMax(&idkey[p], key);
-        }
+        atomicMin(&L.dst[k][p], d);
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
 }
 
-hipError_t launch_depth(const ViewParams& v, Frame* f, hipStream_t s) {
-  int ncu = f->ctx->prop.multiProcessorCount > 0 ? f->ctx->prop.multiProcessorCount : 256;
-  hipLaunchKernelGGL(k_raster<RasterMode::Depth>, dim3(ncu * 8), dim3(256), 0, s, f->cuts, f->frags,
-                     f->counters, f->cap_frag, v.width, v.height, kOpSize, f->depth, f->idkey);
+int device_cu_count();
+
+hipError_t launch_depth(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
+  hipLaunchKernelGGL(k_raster<RasterMode::Depth>, dim3(device_cu_count() * blocks_per_cu), dim3(256), 0, s, a,
+                     ShadowTarget{nullptr, 0}, kOpSize);
   return hipGetLastError();
 }
 
 // kernel2's "last writer wins" is made deterministic: among fragments inside
 // the BUF_ERROR window the pixel keeps the largest (2*tri + sub) key, which
 // is the fragment a sequential in-order execution writes last.
-hipError_t launch_id(const ViewParams& v, Frame* f, hipStream_t s) {
-  int ncu = f->ctx->prop.multiProcessorCount > 0 ? f->ctx->prop.multiProcessorCount : 256;
-  hipLaunchKernelGGL(k_raster<RasterMode::Id>, dim3(ncu * 8), dim3(256), 0, s, f->cuts, f->frags,
-                     f->counters, f->cap_frag, v.width, v.height, kOpSize, f->depth, f->idkey);
+hipError_t launch_id(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
+  hipLaunchKernelGGL(k_raster<RasterMode::Id>, dim3(device_cu_count() * blocks_per_cu), dim3(256), 0, s, a,
+                     ShadowTarget{nullptr, 0}, kOpSize);
   return hipGetLastError();
 }
 
@@ -490,10 +521,11 @@ hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, 
     hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + 255) / 256), dim3(256), 0, s, sc->tri_pos,
                        sc->ntri, sc->objs, sc->nobj, lpos[0], lpos[1], lpos[2], sc->lbuf_dim, only_static,
                        ft, sc->sh_cuts, sc->sh_cap_cut, sc->sh_frags, sc->sh_cap_frag, sc->sh_counters);
-  int ncu = sc->ctx->prop.multiProcessorCount > 0 ? sc->ctx->prop.multiProcessorCount : 256;
-  hipLaunchKernelGGL(k_raster<RasterMode::Shadow>, dim3(ncu * 8), dim3(256), 0, s, sc->sh_cuts, sc->sh_frags,
-                     sc->sh_counters, sc->sh_cap_frag, sc->lbuf_dim, sc->lbuf_dim, kOpSizeLight, map,
-                     (unsigned long long*)nullptr);
+  PassArgs a;
+  memset(&a, 0, sizeof(a));
+  a.lists = Lists{sc->sh_cuts, sc->sh_frags, sc->sh_counters, sc->sh_cap_cut, sc->sh_cap_frag};
+  hipLaunchKernelGGL(k_raster<RasterMode::Shadow>, dim3(device_cu_count() * 8), dim3(256), 0, s, a,
+                     ShadowTarget{map, sc->lbuf_dim}, kOpSizeLight);
   return hipGetLastError();
 }
 

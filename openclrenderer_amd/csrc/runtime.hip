@@ -64,6 +64,8 @@ static void face_table(SinCos out[6]) {
   for (int i = 0; i < 6; i++) sincos_of(ang[i], out[i]);
 }
 
+constexpr int kRasterBlocksPerCu = 8;
+
 static ViewParams make_view_params(const ocr_view* v) {
   ViewParams p;
   memset(&p, 0, sizeof(p));
@@ -77,13 +79,63 @@ static ViewParams make_view_params(const ocr_view* v) {
   return p;
 }
 
+static int g_cu_count = 0;
+
+int device_cu_count() { return g_cu_count > 0 ? g_cu_count : 256; }
+
+DevScene dev_scene(const Scene* sc) {
+  DevScene d;
+  memset(&d, 0, sizeof(d));
+  d.tri_pos = sc->tri_pos;
+  d.tri_attr = sc->tri_attr;
+  d.objs = sc->objs;
+  d.pages = sc->atlas.pages;
+  d.nums = sc->atlas.nums;
+  d.sizes = sc->atlas.sizes;
+  d.lights = sc->lights;
+  d.shadow_maps = sc->shadow_maps;
+  d.static_maps = sc->static_maps;
+  d.ntri = sc->ntri;
+  d.nobj = sc->nobj;
+  d.mip_start = sc->atlas.mip_start;
+  d.nlights = sc->nlights;
+  d.L = sc->lbuf_dim;
+  return d;
+}
+
+DevView dev_view(const ocr_view* v, uint32_t scene, Frame* f) {
+  DevView d;
+  memset(&d, 0, sizeof(d));
+  d.v = make_view_params(v);
+  d.scene = scene;
+  d.depth = f->depth;
+  d.idkey = f->idkey;
+  d.rgba = f->rgba;
+  d.normals = f->normals;
+  return d;
+}
+
+static uint32_t round256(uint64_t n) { return (uint32_t)((n + 255) / 256 * 256); }
+
+// Pass arguments of a single view rendered into its own frame.
+static PassArgs single_pass(const Scene* s, const ocr_view* view, Frame* f) {
+  PassArgs a;
+  memset(&a, 0, sizeof(a));
+  a.view0 = dev_view(view, 0, f);
+  a.scene0 = dev_scene(s);
+  a.nviews = 1;
+  a.tri_slots = round256(s->ntri);
+  a.pix_slots = round256((uint64_t)view->width * view->height);
+  a.tile_w_log2 = 4;
+  a.tiles = view_tiles(view->width, view->height, a.tile_w_log2);
+  a.lists = Lists{f->cuts, f->frags, f->counters, f->cap_cut, f->cap_frag};
+  return a;
+}
+
 }  // namespace ocr
 
 using namespace ocr;
 
-struct ocr_ctx : public Ctx {};
-struct ocr_scene : public Scene {};
-struct ocr_frame : public Frame {};
 
 extern "C" {
 
@@ -105,6 +157,7 @@ int ocr_create(int device, ocr_ctx** out) {
     return fail(OCR_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
   }
   face_table(c->face_sc);
+  g_cu_count = c->prop.multiProcessorCount;
   *out = c;
   return OCR_OK;
 }
@@ -541,7 +594,6 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   int rc = ensure_frame_lists(f, s->ntri);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
-  ViewParams vp = make_view_params(view);
   if (!s->lights) {
     // a scene with no lights still needs valid (empty) light and map buffers
     rc = ocr_lights_upload(s, nullptr, 0, s->lbuf_dim);
@@ -549,16 +601,17 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   }
   const bool prof = f->profiling && f->n_profiled < Frame::kRing;
   hipEvent_t* ev = prof ? &f->ev[(size_t)f->n_profiled * Frame::kEvents] : nullptr;
+  const PassArgs pa = single_pass(s, view, f);
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[0], st));
-  OCR_HIP_CHECK(launch_clear_frame(f, st));
+  OCR_HIP_CHECK(launch_clear(pa, st));
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[1], st));
-  OCR_HIP_CHECK(launch_prearrange(s, vp, f, st));
+  OCR_HIP_CHECK(launch_prearrange(pa, st));
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[2], st));
-  OCR_HIP_CHECK(launch_depth(vp, f, st));
+  OCR_HIP_CHECK(launch_depth(pa, kRasterBlocksPerCu, st));
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], st));
-  OCR_HIP_CHECK(launch_id(vp, f, st));
+  OCR_HIP_CHECK(launch_id(pa, kRasterBlocksPerCu, st));
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], st));
-  OCR_HIP_CHECK(launch_shade(s, vp, f, st));
+  OCR_HIP_CHECK(launch_shade(pa, s->ctx->face_sc, st));
   if (prof) {
     OCR_HIP_CHECK(hipEventRecord(ev[5], st));
     f->n_profiled++;
@@ -628,8 +681,8 @@ int ocr_frame_count_texels(ocr_scene* s, const ocr_view* view, ocr_frame* f, uin
   hipError_t e = launch_fill_u32(tb, 0u, tex_words, st);
   if (e == hipSuccess) e = launch_fill_u32(sb, 0u, sh_words, st);
   if (e == hipSuccess) e = launch_fill_u32(tt, 0u, st_words, st);
-  ViewParams vp = make_view_params(view);
-  if (e == hipSuccess) e = launch_shade_count(s, vp, f, tb, sb, tt, st);
+  const PassArgs pa = single_pass(s, view, f);
+  if (e == hipSuccess) e = launch_shade_count(pa, s->ctx->face_sc, tb, sb, tt, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   std::vector<uint32_t> h1, h2, h3;
   if (e == hipSuccess) {

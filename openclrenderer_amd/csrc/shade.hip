@@ -4,15 +4,15 @@
 // lights with cube-map shadows (generate_hard_occlusion) and Cook-Torrance
 // specular; writes RGBA8 and the encoded screen normal.
 //
-// Pixels are processed in 16x16 tiles (the reference's 16x16 work groups,
-// engine.cpp:2018-2022) so the SSAO neighbourhood and the tile's triangles
-// stay in the XCD's L2.
+// Pixels are processed in 256-pixel tiles, 16x16 by default (the reference's
+// 16x16 work groups, engine.cpp:2018-2022); the tile shape is a tuning knob.
 #pragma clang fp contract(off)
 
 #include "ocr_internal.h"
 
 namespace ocr {
 
+// What one shading block reads: its view, its scene and the frame's lists.
 struct ShadeArgs {
   ViewParams v;
   const uint32_t* depth;
@@ -33,10 +33,19 @@ struct ShadeArgs {
   const uint32_t* counters;
   uint32_t* rgba;
   uint32_t* normals;
-  SinCos face[6];
+  const SinCos* face;
   uint32_t* tex_touch;      // count mode: 1 bit per atlas texel
   uint32_t* shadow_touch;   // count mode: 1 bit per dynamic cube-map texel
   uint32_t* static_touch;   // count mode: 1 bit per static cube-map texel
+};
+
+// Kernel arguments of the shading launch.
+struct ShadeLaunch {
+  PassArgs p;
+  SinCos face[6];
+  uint32_t* tex_touch;
+  uint32_t* shadow_touch;
+  uint32_t* static_touch;
 };
 
 template <bool kCount>
@@ -194,13 +203,55 @@ __device__ __forceinline__ f4 vcol(uint32_t c) {
             (float)(c & 0xFF) / 255.f};
 }
 
+__device__ __forceinline__ const DevView& shade_view(const PassArgs& p, uint32_t& vi) {
+  vi = 0;
+  if (p.views) {
+    uint32_t lo = 0, hi = p.nviews;
+    while (hi - lo > 1) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (p.views[mid].tile_begin <= blockIdx.x) lo = mid; else hi = mid;
+    }
+    vi = (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
+    return p.views[vi];
+  }
+  return p.view0;
+}
+
 template <bool kCount>
-__global__ void __launch_bounds__(256) k_shade(ShadeArgs a) {
+__global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
+  uint32_t vi;
+  const DevView& V = shade_view(sl.p, vi);
+  const DevScene& S = sl.p.scenes ? sl.p.scenes[V.scene] : sl.p.scene0;
+  ShadeArgs a;
+  a.v = V.v;
+  a.depth = V.depth;
+  a.idkey = V.idkey;
+  a.cuts = sl.p.lists.cuts;
+  a.tri_pos = S.tri_pos;
+  a.tri_attr = S.tri_attr;
+  a.objs = S.objs;
+  a.pages = S.pages;
+  a.nums = S.nums;
+  a.sizes = S.sizes;
+  a.mip_start = S.mip_start;
+  a.lights = S.lights;
+  a.nlights = S.nlights;
+  a.shadow_maps = S.shadow_maps;
+  a.static_maps = S.static_maps;
+  a.L = S.L;
+  a.counters = sl.p.lists.counters;
+  a.rgba = V.rgba;
+  a.normals = V.normals;
+  a.face = sl.face;
+  a.tex_touch = sl.tex_touch;
+  a.shadow_touch = sl.shadow_touch;
+  a.static_touch = sl.static_touch;
   const uint32_t W = a.v.width, H = a.v.height;
-  const uint32_t tiles_x = (W + 15) / 16;
-  const uint32_t tile = blockIdx.x;
-  const int x = (int)((tile % tiles_x) * 16 + (threadIdx.x & 15));
-  const int y = (int)((tile / tiles_x) * 16 + (threadIdx.x >> 4));
+  const uint32_t lw = sl.p.tile_w_log2, tw = 1u << lw, th = 256u >> lw;
+  const uint32_t tiles_x = (W + tw - 1) / tw;
+  const uint32_t tile = blockIdx.x - V.tile_begin;
+  const int x = (int)((tile % tiles_x) * tw + (threadIdx.x & (tw - 1)));
+  const int y = (int)((tile / tiles_x) * th + (threadIdx.x >> lw));
   if (x >= (int)W || y >= (int)H) return;
   if (a.counters[kOverflow]) return;
   const size_t pix = (size_t)y * W + x;
@@ -365,47 +416,24 @@ __global__ void __launch_bounds__(256) k_shade(ShadeArgs a) {
   a.normals[pix] = encode_normal(normal);
 }
 
-static ShadeArgs make_args(const Scene* sc, const ViewParams& v, Frame* f) {
-  ShadeArgs a;
-  a.v = v;
-  a.depth = f->depth;
-  a.idkey = f->idkey;
-  a.cuts = f->cuts;
-  a.tri_pos = sc->tri_pos;
-  a.tri_attr = sc->tri_attr;
-  a.objs = sc->objs;
-  a.pages = sc->atlas.pages;
-  a.nums = sc->atlas.nums;
-  a.sizes = sc->atlas.sizes;
-  a.mip_start = sc->atlas.mip_start;
-  a.lights = sc->lights;
-  a.nlights = sc->nlights;
-  a.shadow_maps = sc->shadow_maps;
-  a.static_maps = sc->static_maps;
-  a.L = sc->lbuf_dim;
-  a.counters = f->counters;
-  a.rgba = f->rgba;
-  a.normals = f->normals;
-  for (int i = 0; i < 6; i++) a.face[i] = sc->ctx->face_sc[i];
-  a.tex_touch = a.shadow_touch = a.static_touch = nullptr;
-  return a;
-}
-
-hipError_t launch_shade(const Scene* sc, const ViewParams& v, Frame* f, hipStream_t s) {
-  ShadeArgs a = make_args(sc, v, f);
-  unsigned tiles = ((v.width + 15) / 16) * ((v.height + 15) / 16);
-  hipLaunchKernelGGL(k_shade<false>, dim3(tiles), dim3(256), 0, s, a);
+hipError_t launch_shade(const PassArgs& p, const SinCos face[6], hipStream_t s) {
+  ShadeLaunch sl;
+  sl.p = p;
+  for (int i = 0; i < 6; i++) sl.face[i] = face[i];
+  sl.tex_touch = sl.shadow_touch = sl.static_touch = nullptr;
+  hipLaunchKernelGGL(k_shade<false>, dim3(p.tiles > 0 ? p.tiles : 1), dim3(256), 0, s, sl);
   return hipGetLastError();
 }
 
-hipError_t launch_shade_count(const Scene* sc, const ViewParams& v, Frame* f, uint32_t* tex_bits,
+hipError_t launch_shade_count(const PassArgs& p, const SinCos face[6], uint32_t* tex_bits,
                               uint32_t* shadow_bits, uint32_t* static_bits, hipStream_t s) {
-  ShadeArgs a = make_args(sc, v, f);
-  a.tex_touch = tex_bits;
-  a.shadow_touch = shadow_bits;
-  a.static_touch = static_bits;
-  unsigned tiles = ((v.width + 15) / 16) * ((v.height + 15) / 16);
-  hipLaunchKernelGGL(k_shade<true>, dim3(tiles), dim3(256), 0, s, a);
+  ShadeLaunch sl;
+  sl.p = p;
+  for (int i = 0; i < 6; i++) sl.face[i] = face[i];
+  sl.tex_touch = tex_bits;
+  sl.shadow_touch = shadow_bits;
+  sl.static_touch = static_bits;
+  hipLaunchKernelGGL(k_shade<true>, dim3(p.tiles > 0 ? p.tiles : 1), dim3(256), 0, s, sl);
   return hipGetLastError();
 }
 

@@ -22,6 +22,9 @@ EXPORTS = [
     "ocr_scene_read_shadow_maps", "ocr_frame_create", "ocr_frame_destroy",
     "ocr_frame_set_profiling", "ocr_render", "ocr_frame_read", "ocr_frame_device_buffers",
     "ocr_frame_get_stats", "ocr_frame_count_texels", "ocr_frame_reserve",
+    "ocr_batch_create", "ocr_batch_destroy", "ocr_batch_set_views", "ocr_batch_set_tuning",
+    "ocr_batch_get_tuning", "ocr_batch_set_profiling", "ocr_batch_render", "ocr_batch_launch_graph",
+    "ocr_batch_get_stats", "ocr_batch_autotune",
 ]
 
 
@@ -33,6 +36,11 @@ class OcrError(RuntimeError):
 
 class TexSrc(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.c_void_p)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("raster_blocks_per_cu", C.c_uint32), ("shade_tile_w_log2", C.c_uint32),
+                ("_pad", C.c_uint32 * 2)]
 
 
 class FrameStats(C.Structure):
@@ -80,6 +88,17 @@ def load_library(path=LIB_PATH):
     L.ocr_frame_device_buffers.argtypes = [vp, pp, pp, pp, pp]
     L.ocr_frame_get_stats.argtypes = [vp, vp, C.POINTER(FrameStats)]
     L.ocr_frame_count_texels.argtypes = [vp, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp]
+    L.ocr_batch_create.argtypes = [vp, u32, pp]
+    L.ocr_batch_destroy.argtypes = [vp]
+    L.ocr_batch_destroy.restype = None
+    L.ocr_batch_set_views.argtypes = [vp, vp, vp, vp, u32]
+    L.ocr_batch_set_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+    L.ocr_batch_get_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+    L.ocr_batch_set_profiling.argtypes = [vp, i32]
+    L.ocr_batch_render.argtypes = [vp, vp]
+    L.ocr_batch_launch_graph.argtypes = [vp, vp]
+    L.ocr_batch_get_stats.argtypes = [vp, vp, C.POINTER(FrameStats)]
+    L.ocr_batch_autotune.argtypes = [vp, vp, u32, C.POINTER(BatchTuning)]
     _lib = L
     return L
 
@@ -235,4 +254,69 @@ class Frame:
     def close(self):
         if self.h:
             self.lib.ocr_frame_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    """Many views of several resident scenes rendered together (one launch
+    per pass for the whole batch, optionally replayed from a hipGraph)."""
+
+    def __init__(self, ctx, max_views):
+        self.ctx, self.lib = ctx, ctx.lib
+        h = C.c_void_p()
+        check(self.lib.ocr_batch_create(ctx.h, max_views, C.byref(h)))
+        self.h = h
+        self.frames = []
+
+    def set_views(self, gscenes, views, frames):
+        n = len(views)
+        sc = (C.c_void_p * n)(*[g.h.value for g in gscenes])
+        fr = (C.c_void_p * n)(*[f.h.value for f in frames])
+        vv = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=abi.VIEW).reshape(1) for v in views]))
+        self._keep = (sc, fr, vv)
+        self.frames = frames
+        check(self.lib.ocr_batch_set_views(self.h, C.cast(sc, C.c_void_p), _p(vv), C.cast(fr, C.c_void_p), n))
+
+    def set_tuning(self, raster_blocks_per_cu, shade_tile_w_log2):
+        t = BatchTuning(raster_blocks_per_cu, shade_tile_w_log2)
+        check(self.lib.ocr_batch_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self):
+        t = BatchTuning()
+        check(self.lib.ocr_batch_get_tuning(self.h, C.byref(t)))
+        return t.raster_blocks_per_cu, t.shade_tile_w_log2
+
+    def set_profiling(self, on=True):
+        check(self.lib.ocr_batch_set_profiling(self.h, 1 if on else 0))
+
+    def render(self, stream=None):
+        check(self.lib.ocr_batch_render(self.h, stream))
+
+    def launch_graph(self, stream=None):
+        check(self.lib.ocr_batch_launch_graph(self.h, stream))
+
+    def stats(self, stream=None):
+        s = FrameStats()
+        check(self.lib.ocr_batch_get_stats(self.h, stream, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in FrameStats._fields_}
+
+    def autotune(self, reps=5, stream=None):
+        t = BatchTuning()
+        check(self.lib.ocr_batch_autotune(self.h, stream, reps, C.byref(t)))
+        return t.raster_blocks_per_cu, t.shade_tile_w_log2
+
+    def render_checked(self, stream=None, graph=False):
+        """Render, and once more if the shared lists overflowed (they grow)."""
+        for attempt in range(3):
+            (self.launch_graph if graph else self.render)(stream)
+            try:
+                return self.stats(stream)
+            except OcrError as e:
+                if e.code != abi.OCR_ERR_OVERFLOW or attempt == 2:
+                    raise
+        raise AssertionError("unreachable")
+
+    def close(self):
+        if self.h:
+            self.lib.ocr_batch_destroy(self.h)
             self.h = None
