@@ -26,8 +26,8 @@ extern "C" {
 #define OCR_ERR_INVALID     -1  /* bad handle, size or argument            */
 #define OCR_ERR_HIP         -2  /* a HIP runtime call failed               */
 #define OCR_ERR_OOM         -3  /* device allocation failed                */
-#define OCR_ERR_OVERFLOW    -4  /* a work list exceeded its capacity; the
-                                   frame was grown and must be re-rendered */
+#define OCR_ERR_OVERFLOW    -4  /* a reserved cut list was too small; it
+                                   was grown and the frame must be re-rendered */
 #define OCR_ERR_UNSUPPORTED -5  /* input outside what the path supports     */
 #define OCR_ERR_BUSY        -6  /* admission refused (serving)              */
 
@@ -173,7 +173,7 @@ int  ocr_scene_update_objs(ocr_scene* s, uint32_t first, uint32_t n,
  * array is descriptor tid == i. */
 int  ocr_atlas_upload(ocr_scene* s, const ocr_tex_src* texs, uint32_t ntex);
 /* light::build (light.cpp:145-276): uploads the light list and sizes the
- * cube shadow maps (n_shadow x 6 x dim^2 u32, dim = LIGHTBUFFERDIM). */
+ * cube shadow maps (n_shadow x 6 x dim^2 u32, dim = LIGHTBUFFERDIM, 8..4096). */
 int  ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n,
                        uint32_t light_buffer_dim);
 /* engine::generate_realtime_shadowing (engine.cpp:1601-1790): clears and
@@ -192,13 +192,15 @@ int  ocr_scene_read_shadow_maps(ocr_scene* s, uint32_t* dynamic_maps,
 
 /* ---- frames: offscreen framebuffer (replaces cl_gl_interop_texture) ----- */
 
+/* Frames are 4x4 up to 2^24 pixels (4096 x 4096, 3840 x 2160 ...). */
 int  ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** out);
 void ocr_frame_destroy(ocr_frame* f);
-/* Sets the fragment-list capacity of the frame's next render (the
- * reference's fixed g_tid_buf_max_len, engine.cpp:600-618).  A render that
- * needs more sets the overflow word; ocr_frame_read then reports
+/* Fixes the capacity of the frame's cut-triangle list (the reference's
+ * fixed g_tid_buf_max_len, engine.cpp:600-618).  By default the list holds
+ * 2 pieces per scene triangle and never overflows; a reserved list that is
+ * too small sets the overflow word, ocr_frame_read then reports
  * OCR_ERR_OVERFLOW and the next render grows the list to fit. */
-int  ocr_frame_reserve(ocr_frame* f, uint32_t max_fragments);
+int  ocr_frame_reserve(ocr_frame* f, uint32_t max_cut_triangles);
 /* Record per-pass hipEvent timings on every render of this frame (up to
  * 1024 renders are averaged; enabling again restarts the average). */
 int  ocr_frame_set_profiling(ocr_frame* f, int enable);
@@ -213,7 +215,8 @@ int  ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream);
  * then copies any non-NULL output: depth u32 (UINT_MAX = background),
  * triangle id u32 (OCR_NO_TRIANGLE where nothing is visible), RGBA8 and the
  * encoded screen normals (ushort2 per pixel, cl2.cl:5617-5630).  Returns
- * OCR_ERR_OVERFLOW if the last render overflowed a work list. */
+ * OCR_ERR_OVERFLOW if the last render overflowed a reserved cut list, and
+ * OCR_ERR_UNSUPPORTED if it needed more than 2^30 raster samples. */
 int  ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id,
                     uint8_t* rgba8, uint16_t* normals, void* stream);
 /* Device pointers of the frame's buffers (zero-copy consumers). */
@@ -256,8 +259,8 @@ int  ocr_batch_set_profiling(ocr_batch* b, int enable);
 int  ocr_batch_render(ocr_batch* b, void* stream);
 /* The same five launches captured once into a hipGraph and replayed. */
 int  ocr_batch_launch_graph(ocr_batch* b, void* stream);
-/* Synchronises `stream`; counters of the last render (OCR_ERR_OVERFLOW if the
- * shared lists overflowed: they grow on the next render). */
+/* Synchronises `stream`; counters of the last render (OCR_ERR_UNSUPPORTED if
+ * the views needed more than 2^30 raster samples together). */
 int  ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out);
 /* Times raster grid sizes x shading tile shapes on the current views and
  * keeps the fastest (reps graph launches each). */

@@ -31,9 +31,8 @@ struct ocr_batch {
   uint32_t tri_slots = 0, pix_slots = 0, tiles = 0;
   uint64_t sum_tris = 0, sum_pix = 0;
   CutTri* cuts = nullptr;
-  uint2* frags = nullptr;
   uint32_t* counters = nullptr;
-  uint32_t cap_cut = 0, cap_frag = 0, want_frag = 0;
+  uint32_t cap_cut = 0;
   ocr_batch_tuning tune{8, 4, {0, 0}};
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -62,7 +61,7 @@ static PassArgs batch_pass(const ocr_batch* b) {
   a.pix_slots = b->pix_slots;
   a.tiles = b->tiles;
   a.tile_w_log2 = b->tune.shade_tile_w_log2;
-  a.lists = Lists{b->cuts, b->frags, b->counters, b->cap_cut, b->cap_frag};
+  a.lists = Lists{b->cuts, b->counters, b->cap_cut};
   return a;
 }
 
@@ -86,26 +85,22 @@ static int upload_views(ocr_batch* b) {
   return OCR_OK;
 }
 
+// At most 2 cut pieces per triangle of every view, so the list never
+// overflows.
 static int ensure_lists(ocr_batch* b) {
   uint64_t want_cut = std::min<uint64_t>(b->sum_tris * 2 + 64, 0xFFFFFFF0ull);
-  uint64_t want_frag = std::min<uint64_t>(
-      std::max<uint64_t>(b->sum_tris * 2 + b->sum_pix / 64 + 65536, b->want_frag), 0xFFFFFFF0ull);
-  if (b->cuts && b->cap_cut >= want_cut && b->frags && b->cap_frag >= want_frag) return OCR_OK;
+  if (b->cuts && b->cap_cut >= want_cut) return OCR_OK;
   OCR_HIP_CHECK(hipDeviceSynchronize());
   drop_graph(b);
   if (b->cuts) (void)hipFree(b->cuts);
-  if (b->frags) (void)hipFree(b->frags);
   b->cuts = nullptr;
-  b->frags = nullptr;
-  b->cap_cut = b->cap_frag = 0;
+  b->cap_cut = 0;
   hipError_t e = hipMalloc((void**)&b->cuts, sizeof(CutTri) * want_cut);
-  if (e == hipSuccess) e = hipMalloc((void**)&b->frags, sizeof(uint2) * want_frag);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    return fail(OCR_ERR_OOM, std::string("batch work lists: ") + hipGetErrorString(e));
+    return fail(OCR_ERR_OOM, std::string("batch cut list: ") + hipGetErrorString(e));
   }
   b->cap_cut = (uint32_t)want_cut;
-  b->cap_frag = (uint32_t)want_frag;
   return OCR_OK;
 }
 
@@ -153,7 +148,6 @@ void ocr_batch_destroy(ocr_batch* b) {
   if (b->d_views) (void)hipFree(b->d_views);
   if (b->d_scenes) (void)hipFree(b->d_scenes);
   if (b->cuts) (void)hipFree(b->cuts);
-  if (b->frags) (void)hipFree(b->frags);
   if (b->counters) (void)hipFree(b->counters);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
   for (hipEvent_t e : b->ev)
@@ -283,7 +277,7 @@ int ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out) {
   uint32_t c[kCounterWords];
   OCR_HIP_CHECK(hipMemcpy(c, b->counters, sizeof(c), hipMemcpyDeviceToHost));
   memset(out, 0, sizeof(*out));
-  out->n_cut = c[kCutCount];
+  out->n_cut = c[kPacked + 1];
   out->n_frag = c[kFragCount];
   out->overflow = c[kOverflow] ? 1u : 0u;
   out->n_profiled = (uint32_t)b->n_profiled;
@@ -296,10 +290,9 @@ int ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out) {
     }
     out->ms_total = (float)(sum / b->n_profiled);
   }
-  if (c[kOverflow]) {
-    b->want_frag = (uint32_t)std::min<uint64_t>((uint64_t)c[kFragCount] + c[kFragCount] / 4 + 1024, 0xFFFFFFF0ull);
-    return fail(OCR_ERR_OVERFLOW, "ocr_batch_get_stats: fragment list overflowed; render again");
-  }
+  if (c[kOverflow] & kOverflowSamples)
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_batch_get_stats: the batch needs more than 2^30 raster samples");
+  if (c[kOverflow]) return fail(OCR_ERR_OVERFLOW, "ocr_batch_get_stats: cut list overflowed");
   return OCR_OK;
 }
 

@@ -47,8 +47,10 @@ struct alignas(16) ObjDesc {
 static_assert(sizeof(ObjDesc) == 64, "ObjDesc layout");
 
 // A clipped, culled, projected triangle: screen x, y and camera z of its
-// three vertices, its 1/det constant and its sort key (2*tri + sub for a
-// camera view, cube face for a shadow map).
+// three vertices, its 1/det constant, its sort key (2*tri + sub for a camera
+// view, cube face for a shadow map), and the range of raster samples its
+// fragments cover.  The list is sorted by sample_begin: prearrange allocates
+// records and sample ranges together with one 64-bit atomic per wavefront.
 struct alignas(16) CutTri {
   float x[3];
   float y[3];
@@ -56,8 +58,12 @@ struct alignas(16) CutTri {
   float rconst;
   uint32_t key;
   uint32_t oid;
+  uint32_t sample_begin;
+  uint32_t nsamples;
+  uint32_t view;
+  uint32_t _pad;
 };
-static_assert(sizeof(CutTri) == 48, "CutTri layout");
+static_assert(sizeof(CutTri) == 64, "CutTri layout");
 
 // Everything a kernel needs to know about one view.
 struct ViewParams {
@@ -99,16 +105,13 @@ struct DevView {
   uint32_t* normals;
 };
 
-// Work lists and counters shared by the views of one render.
+// The cut list and counters shared by the views of one render.
 struct Lists {
   CutTri* cuts;
-  uint2* frags;            // (cut index, chunk | view << kViewShift)
   uint32_t* counters;
-  uint32_t cap_cut, cap_frag;
+  uint32_t cap_cut;
 };
-constexpr uint32_t kViewShift = 20;
-constexpr uint32_t kChunkMask = (1u << kViewShift) - 1;
-constexpr uint32_t kMaxViews = 1u << (32 - kViewShift);
+constexpr uint32_t kMaxViews = 4096;
 
 // Everything the camera passes need.  A single view travels inline in the
 // kernel arguments (views == nullptr); a batch reads device tables.
@@ -129,13 +132,24 @@ __host__ __device__ inline uint32_t view_tiles(uint32_t w, uint32_t h, uint32_t 
   return ((w + tw - 1) / tw) * ((h + th - 1) / th);
 }
 
-// Device counters of one frame's work lists.
+// Device counters of one render: words 0-1 hold a u64 whose high half is the
+// cut-triangle count and low half the raster sample count.
 enum Counter : int {
-  kCutCount = 0,
-  kFragCount = 1,
+  kPacked = 0,
   kOverflow = 2,
+  kFragCount = 3,          // reference fragments (statistics)
   kCounterWords = 8,
 };
+// Bits of the overflow word: the cut list is full / the sample space is full.
+constexpr uint32_t kOverflowCuts = 1u, kOverflowSamples = 2u;
+// Raster samples per render, and per cut triangle.  Frames and shadow faces
+// hold at most 2^24 pixels, so a triangle has at most 2^24 + 2^16 samples; a
+// wavefront then adds less than 2^31 to the low half of the packed counter,
+// which therefore reaches the overflow threshold before it can carry into the
+// high half.
+constexpr uint32_t kMaxSamples = 1u << 30;
+constexpr uint32_t kMaxTriSamples = (1u << 24) + (1u << 16);
+constexpr uint64_t kMaxTargetPixels = 1ull << 24;
 
 struct Ctx {
   int device = 0;
@@ -164,11 +178,10 @@ struct Scene {
   uint32_t* shadow_maps = nullptr;   // n_shadow x 6 x dim^2
   uint32_t* static_maps = nullptr;   // n_static x 6 x dim^2
   bool static_dirty = false;
-  // work lists of the shadow passes
+  // cut list of the shadow passes
   CutTri* sh_cuts = nullptr;
-  uint2* sh_frags = nullptr;
   uint32_t* sh_counters = nullptr;
-  uint32_t sh_cap_cut = 0, sh_cap_frag = 0;
+  uint32_t sh_cap_cut = 0;
   hipEvent_t ev_sh0 = nullptr, ev_sh1 = nullptr;
   float ms_shadow = 0.f;
 };
@@ -181,11 +194,9 @@ struct Frame {
   uint32_t* rgba = nullptr;
   uint32_t* normals = nullptr;
   CutTri* cuts = nullptr;
-  uint2* frags = nullptr;
   uint32_t* counters = nullptr;
-  uint32_t cap_cut = 0, cap_frag = 0;
-  uint32_t want_frag = 0;            // capacity asked for after an overflow
-  bool exact_frag = false;           // want_frag set by ocr_frame_reserve
+  uint32_t cap_cut = 0;
+  uint32_t want_cut = 0;             // capacity set by ocr_frame_reserve (0 = automatic)
   bool profiling = false;
   static constexpr int kEvents = 6;       // clear, prearrange, depth, id, shade, end
   static constexpr int kRing = 1024;      // renders averaged per profiling window

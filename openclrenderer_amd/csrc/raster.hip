@@ -1,4 +1,4 @@
-// raster.hip — prearrange (transform, near clip, cull, fragment count), the
+// raster.hip — prearrange (transform, near clip, cull, sample count), the
 // depth pass (u32 atomicMin) and the id pass (u64 atomicMax on a packed key),
 // for camera views and for cube shadow maps.
 //
@@ -6,14 +6,19 @@
 // kernel2 cl2.cl:5390-5546, prearrange_realtime_shadowing cl2.cl:4419-4636,
 // kernel1_realtime_shadowing cl2.cl:5129-5246.
 //
-// Work lists.  prearrange appends each surviving cut triangle (48 B CutTri)
-// and its ceil(bbox_area / op_size) fragment records (8 B: cut index, chunk)
-// to two device lists.  Offsets come from a wavefront-wide exclusive scan of
-// the per-lane counts plus ONE atomicAdd per wavefront and list (the reference
-// does two global atomics per triangle, cl2.cl:4342, :4388).  Capacity is
-// checked per wavefront; on overflow nothing of that wavefront is written and
-// the overflow word is set, which every later pass of the frame checks before
-// touching the lists.
+// Work decomposition.  The reference cuts every triangle's bounding box into
+// fragments of op_size pixels and runs one work item per fragment; a fragment
+// walks the samples q = pa .. pa + op_size (pa = op_size * chunk) with a
+// running cursor and stops at the first row >= max_y.  Fragments hold from 1
+// to op_size + 1 samples, so one work item per fragment leaves most lanes of
+// a wavefront idle.  Here prearrange computes each cut triangle's exact
+// sample count over all of its fragments and appends the triangle with its
+// range in one global sample space: one packed 64-bit atomicAdd per
+// wavefront returns both the record index and the sample offset, so the cut
+// list is sorted by sample offset and its ranges tile [0, total).  The raster
+// passes split that sample space into equal chunks: a wavefront finds the
+// chunk's first triangle with a 64-ary search, stages 64 triangles' setup in
+// LDS and sweeps the chunk with all lanes.  No per-fragment records exist.
 #pragma clang fp contract(off)
 
 #include <string.h>
@@ -33,17 +38,24 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
-// Allocate `mine` slots per lane from a list counter with one atomic per
-// wavefront.  Returns this lane's first slot and the wavefront's range end.
-__device__ __forceinline__ uint32_t wave_alloc(uint32_t* counter, uint32_t mine, uint32_t& wave_end) {
+// Allocates `ncut` records and `nsmp` samples per lane with one 64-bit atomic
+// per wavefront on the packed counter (cuts << 32 | samples).  Returns this
+// lane's first record and first sample; `end` receives the wavefront's end.
+__device__ __forceinline__ unsigned long long wave_alloc_packed(unsigned long long* counter, uint32_t ncut,
+                                                                uint32_t nsmp, unsigned long long& end) {
   const int lane = threadIdx.x & (kWave - 1);
-  uint32_t incl = wave_incl_scan(mine);
-  uint32_t total = __shfl(incl, kWave - 1, kWave);
-  uint32_t base = 0;
+  const uint32_t ic = wave_incl_scan(ncut);
+  const uint32_t is = wave_incl_scan(nsmp);
+  const uint32_t tc = __shfl(ic, kWave - 1, kWave);
+  const uint32_t ts = __shfl(is, kWave - 1, kWave);
+  const unsigned long long total = ((unsigned long long)tc << 32) + ts;
+  unsigned long long base = 0;
   if (lane == 0 && total) base = atomicAdd(counter, total);
-  base = __shfl(base, 0, kWave);
-  wave_end = base + total;
-  return base + incl - mine;
+  const uint32_t blo = __shfl((uint32_t)base, 0, kWave);
+  const uint32_t bhi = __shfl((uint32_t)(base >> 32), 0, kWave);
+  base = ((unsigned long long)bhi << 32) | blo;
+  end = base + total;
+  return base + ((unsigned long long)(ic - ncut) << 32) + (is - nsmp);
 }
 
 // ---------------------------------------------------------- setup helpers
@@ -65,36 +77,45 @@ __device__ __forceinline__ void raster_setup(const CutTri& c, float ew, float eh
   s.iwidth = 1.f / (float)s.width;
 }
 
-// The fragment walk of kernel1 / kernel2 (cl2.cl:5046-5126): 501 steps of a
-// running (x, y) cursor from pixel op_size*chunk of the bounding box, with
-// the reference's fma row computation and running-modulo column.
-template <typename Visit>
-__device__ __forceinline__ void walk(const Setup& s, uint32_t chunk, int op_size, Visit&& visit) {
-  const int width = s.width;
-  if (width <= 0) return;
-  const int pixel_along = op_size * (int)chunk;
-  int pcount = -1;
-  float x = (float)((pixel_along + 0) % width) + s.mm[0] - 1;
-  float y = floorf((float)(pixel_along + pcount) / (float)width) + s.mm[2];
-  float rwm = (float)((pixel_along + pcount) % width);
-  while (pcount < op_size) {
-    pcount++;
-    x += 1;
-    rwm += 1.f;
-    if (rwm >= (float)width) rwm = 0;
-    float ty = y;
-    y = floorf(fmaf((float)(pixel_along + pcount), s.iwidth, s.mm[2]));
-    x = y != ty ? rwm + s.mm[0] : x;
-    if (y >= s.mm[3]) break;
-    if (x >= s.mm[1]) continue;
-    if (point_in_tri(x, y, s.xp[0], s.yp[0], s.xp[1], s.yp[1], s.xp[2], s.yp[2])) {
-      float fd = fmaf(s.A, x, fmaf(s.B, y, s.C));
-      visit((int)x, (int)y, sat_u32(kMulint / fd));
-    }
-  }
+// The walk's row of sample q: floor(mad(q, 1/width, min_y)) (cl2.cl:5083).
+__device__ __forceinline__ float row_of(int q, float iw, float min_y) {
+  return floorf(fmaf((float)q, iw, min_y));
 }
 
-// Clip/cull/count of one projected triangle pair (cl2.cl:4344-4380).
+// First q >= 0 with row_of(q) >= Y, stepping from the nominal (Y - min_y) * w.
+__device__ __forceinline__ int run_start(float Y, int w, float iw, float min_y) {
+  long long n0 = (long long)(Y - min_y) * (long long)w;
+  int q = n0 < 0 ? 0 : (int)n0;
+  for (int g = 0; g < 64 && q > 0 && row_of(q - 1, iw, min_y) >= Y; g++) q--;
+  for (int g = 0; g < 64 && row_of(q, iw, min_y) < Y; g++) q++;
+  return q;
+}
+
+// Samples of all fragments of a cut triangle.  Fragment a visits
+// q in [op*a, op*a + op] up to the first q whose row is >= max_y (q_end), so
+// it holds clamp(q_end - op*a, 0, op + 1) samples; the full fragments come
+// first, then at most one partial one.
+__device__ __forceinline__ uint32_t triangle_samples(const f3 p[3], float ew, float eh, int op, uint32_t nfrag) {
+  if (nfrag == 0) return 0;
+  const float xs[3] = {p[0].x, p[1].x, p[2].x}, ys[3] = {p[0].y, p[1].y, p[2].y};
+  float mm[4];
+  calc_min_max(xs, ys, ew, eh, mm);
+  const int w = (int)(mm[1] - mm[0]);
+  if (w <= 0) return 0;
+  const float iw = 1.f / (float)w;
+  const long long q_end = run_start(mm[3], w, iw, mm[2]);
+  long long m = q_end >= op + 1 ? (q_end - (op + 1)) / op + 1 : 0;
+  if (m > (long long)nfrag) m = nfrag;
+  long long s = (long long)(op + 1) * m;
+  if (m < (long long)nfrag) {
+    const long long r = q_end - (long long)op * m;
+    s += r < 0 ? 0 : (r > op + 1 ? op + 1 : r);
+  }
+  return s > (long long)kMaxTriSamples ? kMaxTriSamples : (uint32_t)s;
+}
+
+// Clip/cull/count of one projected triangle (cl2.cl:4344-4380): returns the
+// reference's fragment count ceil(bbox_area / op_size), 0 if culled.
 __device__ __forceinline__ uint32_t cull_count(const f3 p[3], float ew, float eh, bool two_sided,
                                                int op_size, float& rconst) {
   bool valid = two_sided || front_facing(p[0], p[1], p[2]);
@@ -114,13 +135,18 @@ __device__ __forceinline__ uint32_t cull_count(const f3 p[3], float ew, float eh
   return n > 0.f ? (uint32_t)n : 0u;
 }
 
-__device__ __forceinline__ void store_cut(CutTri* dst, const f3 p[3], float rconst, uint32_t key, uint32_t oid) {
+__device__ __forceinline__ void store_cut(CutTri* dst, const f3 p[3], float rconst, uint32_t key, uint32_t oid,
+                                          uint32_t sample_begin, uint32_t nsamples, uint32_t view) {
   CutTri c;
 #pragma unroll
   for (int i = 0; i < 3; i++) { c.x[i] = p[i].x; c.y[i] = p[i].y; c.z[i] = p[i].z; }
   c.rconst = rconst;
   c.key = key;
   c.oid = oid;
+  c.sample_begin = sample_begin;
+  c.nsamples = nsamples;
+  c.view = view;
+  c._pad = 0;
   *dst = c;
 }
 
@@ -149,6 +175,35 @@ __device__ __forceinline__ int transform_clip_project(const TriPos& T, const Obj
   return n;
 }
 
+// Appends this lane's cut triangles that hold samples (0..2) with their
+// sample ranges.  A wavefront that would pass the list capacity or the
+// sample limit writes nothing and sets the overflow word, which every later
+// pass of the render checks before reading the list.
+__device__ __forceinline__ void append_cuts(const Lists& L, const f3 tp[2][3], const float rc[2],
+                                            const uint32_t nf[2], const uint32_t ns[2], uint32_t key0,
+                                            uint32_t key_step, uint32_t oid, uint32_t view) {
+  const uint32_t k0 = ns[0] > 0, k1 = ns[1] > 0;
+  unsigned long long end;
+  const unsigned long long b = wave_alloc_packed(reinterpret_cast<unsigned long long*>(L.counters + kPacked),
+                                                 k0 + k1, ns[0] + ns[1], end);
+  const uint32_t frags = wave_incl_scan(nf[0] * k0 + nf[1] * k1);
+  const int lane = threadIdx.x & (kWave - 1);
+  if (lane == kWave - 1 && frags) atomicAdd(&L.counters[kFragCount], frags);
+  const bool over_cut = (end >> 32) > L.cap_cut, over_smp = (uint32_t)end > kMaxSamples;
+  if (over_cut || over_smp) {
+    if (lane == 0) atomicOr(&L.counters[kOverflow], (over_cut ? kOverflowCuts : 0u) | (over_smp ? kOverflowSamples : 0u));
+    return;
+  }
+  uint32_t c = (uint32_t)(b >> 32), smp = (uint32_t)b;
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    if (!ns[i]) continue;
+    store_cut(&L.cuts[c], tp[i], rc[i], key0 + key_step * (uint32_t)i, oid, smp, ns[i], view);
+    smp += ns[i];
+    c++;
+  }
+}
+
 // ------------------------------------------------------------------ views
 __device__ __forceinline__ const DevView& view_at(const PassArgs& a, uint32_t i) {
   return a.views ? a.views[i] : a.view0;
@@ -159,7 +214,7 @@ __device__ __forceinline__ const DevScene& scene_at(const PassArgs& a, uint32_t 
 
 // The view owning a 256-slot block: the last view whose range starts at or
 // before `slot` (ranges are 256-aligned, so the answer is block-uniform).
-enum class Range { Tri, Pix, Tile };
+enum class Range { Tri, Pix };
 template <Range kR>
 __device__ __forceinline__ uint32_t view_of(const PassArgs& a, uint32_t slot) {
   if (!a.views) return 0;
@@ -167,7 +222,7 @@ __device__ __forceinline__ uint32_t view_of(const PassArgs& a, uint32_t slot) {
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
     const DevView& m = a.views[mid];
-    uint32_t b = kR == Range::Tri ? m.tri_begin : (kR == Range::Pix ? m.pix_begin : m.tile_begin);
+    uint32_t b = kR == Range::Tri ? m.tri_begin : m.pix_begin;
     if (b <= slot) lo = mid; else hi = mid;
   }
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
@@ -218,7 +273,7 @@ __global__ void __launch_bounds__(256) k_prearrange(PassArgs a) {
   const float ew = (float)v.width, eh = (float)v.height;
   f3 tp[2][3];
   float rc[2] = {0.f, 0.f};
-  uint32_t nf[2] = {0u, 0u};
+  uint32_t nf[2] = {0u, 0u}, ns[2] = {0u, 0u};
   uint32_t oid = 0;
   if (id < S.ntri) {
     const TriPos T = S.tri_pos[id];
@@ -231,31 +286,14 @@ __global__ void __launch_bounds__(256) k_prearrange(PassArgs a) {
         bool two_sided = (G.flags & OCR_FEATURE_TWO_SIDED) > 0;
 #pragma unroll
         for (int i = 0; i < 2; i++)
-          if (i < num) nf[i] = cull_count(tp[i], ew, eh, two_sided, kOpSize, rc[i]);
+          if (i < num) {
+            nf[i] = cull_count(tp[i], ew, eh, two_sided, kOpSize, rc[i]);
+            ns[i] = triangle_samples(tp[i], ew, eh, kOpSize, nf[i]);
+          }
       }
     }
   }
-  // A cut triangle whose bbox has zero area emits no fragment and can never
-  // write a pixel, so it is not stored either.
-  const uint32_t k0 = nf[0] > 0, k1 = nf[1] > 0;
-  uint32_t cut_end, frag_end;
-  const Lists& L = a.lists;
-  uint32_t cut0 = wave_alloc(&L.counters[kCutCount], k0 + k1, cut_end);
-  uint32_t frag0 = wave_alloc(&L.counters[kFragCount], nf[0] + nf[1], frag_end);
-  if (cut_end > L.cap_cut || frag_end > L.cap_frag) {
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicOr(&L.counters[kOverflow], 1u);
-    return;
-  }
-  uint32_t c = cut0, f = frag0;
-  const uint32_t vbits = vi << kViewShift;
-#pragma unroll
-  for (int i = 0; i < 2; i++) {
-    if (!nf[i]) continue;
-    store_cut(&L.cuts[c], tp[i], rc[i], 2u * id + (uint32_t)i, oid);
-    for (uint32_t q = 0; q < nf[i]; q++) L.frags[f + q] = make_uint2(c, q | vbits);
-    f += nf[i];
-    c++;
-  }
+  append_cuts(a.lists, tp, rc, nf, ns, 2u * id, 1u, oid, vi);
 }
 
 hipError_t launch_prearrange(const PassArgs& a, hipStream_t s) {
@@ -264,50 +302,20 @@ hipError_t launch_prearrange(const PassArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// ---------------------------------------------- load-balanced raster passes
+// ------------------------------------------------- sample-chunk raster passes
 //
-// A fragment is the reference's unit of work: the samples q = pa .. pa + 500
-// (pa = op_size * chunk) of its cut triangle's bounding box, visited by a
-// running cursor (cl2.cl:5046-5126).  Fragment sizes range from 1 to 501
-// samples, so one thread per fragment leaves most lanes of a wavefront idle.
-// Here a wavefront loads 64 fragments, keeps their setup in LDS, prefix-sums
-// their sample counts and sweeps all of their samples with all 64 lanes.
-//
-// Each sample is evaluated on its own with the cursor's closed form:
-//   y(q) = floor(fma(q, iwidth, min_y))                  (the reference's row)
-//   the walk stops at the first q with y(q) >= max_y      (y is monotone in q)
+// Sample s of a cut triangle (0 <= s < nsamples) is sample o = s % (op + 1)
+// of fragment a = s / (op + 1), i.e. q = op*a + o with pa = op*a.  Its cursor
+// position is the closed form of the reference's running (x, y):
+//   y(q) = floor(fma(q, 1/width, min_y))
 //   x(q) = (r % width) + min_x + (q - r)
-// where r is the later of the fragment start pa and the first index of q's
-// row run (the cursor is reset to running_width_mod + min_x exactly when y
-// changes, and starts at pa % width + min_x).  Row-run starts are found from
-// the nominal (Y - min_y) * width by stepping while y disagrees, so rows whose
-// fma rounding lags or leads by a sample are reproduced exactly.
+// where r is the later of pa and the first index of q's row run (the cursor
+// is reset to running_width_mod + min_x exactly when y changes, and starts at
+// pa % width + min_x).  Row-run starts come from the nominal
+// (Y - min_y) * width by stepping while y disagrees, so rows whose fma
+// rounding lags or leads by a sample are reproduced exactly.
 
-struct FragLds {
-  float mm[4][kWave];      // min_x, max_x, min_y, max_y
-  float pl[4][kWave];      // A, B, C (1/z plane), iwidth
-  float ek[8][kWave];      // edge test: s0, s1, s2, t0, t1, t2, sign, 2.0001*A*sign
-  int w[kWave];
-  int pa[kWave];
-  uint32_t pitch[kWave];   // row pitch of the target buffer
-  uint32_t* dst[kWave];    // depth buffer / cube face
-  unsigned long long* ids[kWave];
-  unsigned long long key[kWave];
-  uint32_t pre[kWave + 1]; // exclusive prefix of sample counts
-};
-
-__device__ __forceinline__ float row_of(int q, float iw, float min_y) {
-  return floorf(fmaf((float)q, iw, min_y));
-}
-
-// First q >= 0 with row_of(q) >= Y.
-__device__ __forceinline__ int run_start(float Y, int w, float iw, float min_y) {
-  long long n0 = (long long)(Y - min_y) * (long long)w;
-  int q = n0 < 0 ? 0 : (int)n0;
-  for (int g = 0; g < 64 && q > 0 && row_of(q - 1, iw, min_y) >= Y; g++) q--;
-  for (int g = 0; g < 64 && row_of(q, iw, min_y) < Y; g++) q++;
-  return q;
-}
+constexpr uint32_t kChunk = kWave * 8;   // samples per wavefront work item
 
 enum class RasterMode { Depth, Id, Shadow };
 
@@ -316,109 +324,153 @@ struct ShadowTarget {
   uint32_t L;
 };
 
+// Setup of 64 staged triangles, one column per lane.  point_in_tri's
+// loop-invariant sub-expressions are kept with the edge sign folded in
+// (negation is exact, so (a + b*x + c*y) * sign has the same bits as
+// a*sign + (b*sign)*x + (c*sign)*y).
 template <RasterMode kMode>
-__global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh, int op_size) {
-  __shared__ FragLds lds_all[256 / kWave];
+struct TriLds {
+  float min_x[kWave], max_x[kWave], min_y[kWave];
+  float pa[kWave], pb[kWave], pc[kWave], iw[kWave];   // 1/z plane, 1/width
+  float ek[7][kWave];   // s0, s1, s2, t0, t1, t2 (signed), 2.0001 * |area|
+  int w[kWave];
+  uint32_t key[kWave];
+  uint32_t begin[kWave + 1];
+  uint32_t pitch[kMode == RasterMode::Shadow ? 1 : kWave];
+  uint32_t* dst[kWave];                                // depth buffer / cube face
+  unsigned long long* ids[kMode == RasterMode::Id ? kWave : 1];
+};
+
+// x mod w for 0 <= x < 2^25, 1 <= w: a float quotient estimate (|error| <= 1
+// for these ranges) corrected by one step each way.
+__device__ __forceinline__ int mod_small(int x, int w, float invw) {
+  const int qq = (int)((float)x * invw);
+  int r = x - qq * w;
+  if (r < 0) r += w;
+  if (r >= w) r -= w;
+  return r;
+}
+
+template <RasterMode kMode, int kOp>
+__global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
+  __shared__ TriLds<kMode> lds_all[256 / kWave];
   const Lists& Ls = a.lists;
   if (Ls.counters[kOverflow]) return;
-  const uint32_t n = min(Ls.counters[kFragCount], Ls.cap_frag);
+  const unsigned long long packed = *reinterpret_cast<const unsigned long long*>(Ls.counters + kPacked);
+  const uint32_t ncut = (uint32_t)(packed >> 32);
+  const uint32_t total = (uint32_t)packed;
   const int lane = threadIdx.x & (kWave - 1);
-  FragLds& L = lds_all[threadIdx.x / kWave];
+  TriLds<kMode>& L = lds_all[threadIdx.x / kWave];
   const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  for (uint32_t base = wave0 * kWave; base < n; base += nwaves * kWave) {
-    // ---- per-fragment setup, one lane per fragment
-    const uint32_t fi = base + lane;
-    uint32_t cnt = 0;
-    if (fi < n) {
-      const uint2 fr = Ls.frags[fi];
-      const CutTri c = Ls.cuts[fr.x];
-      uint32_t W, H;
-      if constexpr (kMode == RasterMode::Shadow) {
-        W = H = sh.L;
-        L.dst[lane] = sh.map + (size_t)c.key * sh.L * sh.L;
-      } else {
-        const DevView& V = view_at(a, fr.y >> kViewShift);
-        W = V.v.width;
-        H = V.v.height;
-        L.dst[lane] = V.depth;
-        L.ids[lane] = V.idkey;
-        L.key[lane] = ((unsigned long long)(c.key + 1u) << 32) | fr.x;
-      }
-      L.pitch[lane] = W;
-      Setup st;
-      raster_setup(c, (float)W, (float)H, st);
-      const int pa = op_size * (int)(fr.y & kChunkMask);
-      if (st.width > 0) {
-        // samples of this fragment: q in [pa, pa + op_size] with y(q) < max_y
-        const int q_end = run_start(st.mm[3], st.width, st.iwidth, st.mm[2]);
-        int c2 = q_end - pa;
-        c2 = c2 < 0 ? 0 : (c2 > op_size + 1 ? op_size + 1 : c2);
-        cnt = (uint32_t)c2;
-      }
-#pragma unroll
-      for (int k = 0; k < 4; k++) L.mm[k][lane] = st.mm[k];
-      L.pl[0][lane] = st.A; L.pl[1][lane] = st.B; L.pl[2][lane] = st.C; L.pl[3][lane] = st.iwidth;
-      // point_in_tri's loop-invariant sub-expressions (same operation order)
-      const float p0x = st.xp[0], p0y = st.yp[0], p1x = st.xp[1], p1y = st.yp[1], p2x = st.xp[2], p2y = st.yp[2];
-      const float A = 0.5f * (-p1y * p2x + p0y * (-p1x + p2x) + p0x * (p1y - p2y) + p1x * p2y);
-      const float sign = A < 0 ? -1.f : 1.f;
-      L.ek[0][lane] = p0y * p2x - p0x * p2y;
-      L.ek[1][lane] = p2y - p0y;
-      L.ek[2][lane] = p0x - p2x;
-      L.ek[3][lane] = p0x * p1y - p0y * p1x;
-      L.ek[4][lane] = p0y - p1y;
-      L.ek[5][lane] = p1x - p0x;
-      L.ek[6][lane] = sign;
-      L.ek[7][lane] = 2.0001f * A * sign;
-      L.w[lane] = st.width;
-      L.pa[lane] = pa;
+  for (uint32_t s0 = wave0 * kChunk; s0 < total; s0 += nwaves * kChunk) {
+    const uint32_t s1 = min(s0 + kChunk, total);
+    // ---- the chunk's first triangle: last record with sample_begin <= s0
+    // (record 0 begins at sample 0; 64-ary search, one probe per lane)
+    uint32_t lo = 0, hi = ncut;
+    while (hi - lo > 1) {
+      const uint32_t p = lo + (uint32_t)(((unsigned long long)(hi - lo) * (uint32_t)lane) / kWave);
+      const unsigned long long m = __ballot(Ls.cuts[p].sample_begin <= s0);
+      const int t = 63 - __clzll((long long)m);
+      const uint32_t plo = __shfl(p, t, kWave);
+      const uint32_t phi = __shfl(p, t < kWave - 1 ? t + 1 : t, kWave);
+      lo = plo;
+      hi = t < kWave - 1 ? phi : hi;
     }
-    const uint32_t incl = wave_incl_scan(cnt);
-    const uint32_t total = __shfl(incl, kWave - 1, kWave);
-    L.pre[lane] = incl - cnt;
-    if (lane == kWave - 1) L.pre[kWave] = total;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // ---- sweep every sample of the 64 fragments with all lanes
-    for (uint32_t j = lane; j < total; j += kWave) {
-      int k = 0;
-#pragma unroll
-      for (int step = 32; step >= 1; step >>= 1)
-        if (L.pre[k + step] <= j) k += step;
-      const int w = L.w[k];
-      const float min_x = L.mm[0][k], max_x = L.mm[1][k], min_y = L.mm[2][k];
-      const float iw = L.pl[3][k];
-      const int q = L.pa[k] + (int)(j - L.pre[k]);
-      const float y = row_of(q, iw, min_y);
-      int r = run_start(y, w, iw, min_y);
-      r = r > L.pa[k] ? r : L.pa[k];
-      const float x = (float)((uint32_t)r % (uint32_t)w) + min_x + (float)(q - r);
-      if (x >= max_x) continue;
-      const float sign = L.ek[6][k];
-      const float s = (L.ek[0][k] + L.ek[1][k] * x + L.ek[2][k] * y) * sign;
-      const float t = (L.ek[3][k] + L.ek[4][k] * x + L.ek[5][k] * y) * sign;
-      if (!(s > -0.0001f && t > -0.0001f && (s + t) < L.ek[7][k])) continue;
-      const float fd = fmaf(L.pl[0][k], x, fmaf(L.pl[1][k], y, L.pl[2][k]));
-      const uint32_t d = sat_u32(kMulint / fd);
-      const size_t p = (size_t)(int)y * L.pitch[k] + (size_t)(int)x;
-      if constexpr (kMode == RasterMode::Id) {
-        const uint32_t val = L.dst[k][p];
-        if (d > val - kBufError && d < val + kBufError) atomicMax(&L.ids[k][p], L.key[k]);
+    uint32_t c0 = lo;
+    uint32_t s = s0;
+    while (s < s1) {
+      // ---- stage triangles c0 .. c0 + 63, one per lane
+      const uint32_t ci = c0 + lane;
+      if (ci < ncut) {
+        const CutTri c = Ls.cuts[ci];
+        uint32_t W, H;
+        if constexpr (kMode == RasterMode::Shadow) {
+          W = H = sh.L;
+          L.dst[lane] = sh.map + (size_t)c.key * sh.L * sh.L;
+        } else {
+          const DevView& V = view_at(a, c.view);
+          W = V.v.width;
+          H = V.v.height;
+          L.dst[lane] = V.depth;
+          L.pitch[lane] = W;
+          if constexpr (kMode == RasterMode::Id) L.ids[lane] = V.idkey;
+        }
+        Setup st;
+        raster_setup(c, (float)W, (float)H, st);
+        L.min_x[lane] = st.mm[0];
+        L.max_x[lane] = st.mm[1];
+        L.min_y[lane] = st.mm[2];
+        L.pa[lane] = st.A;
+        L.pb[lane] = st.B;
+        L.pc[lane] = st.C;
+        L.iw[lane] = st.iwidth;
+        const float p0x = st.xp[0], p0y = st.yp[0], p1x = st.xp[1], p1y = st.yp[1], p2x = st.xp[2], p2y = st.yp[2];
+        const float A = 0.5f * (-p1y * p2x + p0y * (-p1x + p2x) + p0x * (p1y - p2y) + p1x * p2y);
+        const float sign = A < 0 ? -1.f : 1.f;
+        L.ek[0][lane] = (p0y * p2x - p0x * p2y) * sign;
+        L.ek[1][lane] = (p2y - p0y) * sign;
+        L.ek[2][lane] = (p0x - p2x) * sign;
+        L.ek[3][lane] = (p0x * p1y - p0y * p1x) * sign;
+        L.ek[4][lane] = (p0y - p1y) * sign;
+        L.ek[5][lane] = (p1x - p0x) * sign;
+        L.ek[6][lane] = 2.0001f * A * sign;
+        L.w[lane] = st.width;
+        L.key[lane] = c.key;
+        L.begin[lane] = c.sample_begin;
+        if (lane == kWave - 1) L.begin[kWave] = c.sample_begin + c.nsamples;
       } else {
-        atomicMin(&L.dst[k][p], d);
+        L.begin[lane] = total;
+        if (lane == kWave - 1) L.begin[kWave] = total;
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const uint32_t stage_end = min(s1, L.begin[kWave]);
+      // ---- every sample of [s, stage_end) with all lanes
+      for (uint32_t j = s + lane; j < stage_end; j += kWave) {
+        int k = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1)
+          if (L.begin[k + step] <= j) k += step;
+        const uint32_t sl = j - L.begin[k];
+        const uint32_t fa = sl / (uint32_t)(kOp + 1);
+        const int pa = kOp * (int)fa;
+        const int q = pa + (int)(sl - fa * (uint32_t)(kOp + 1));
+        const int w = L.w[k];
+        const float min_y = L.min_y[k], iw = L.iw[k];
+        const float y = row_of(q, iw, min_y);
+        int r = run_start(y, w, iw, min_y);
+        r = r > pa ? r : pa;
+        const float x = (float)mod_small(r, w, iw) + L.min_x[k] + (float)(q - r);
+        if (x >= L.max_x[k]) continue;
+        const float s_ = L.ek[0][k] + L.ek[1][k] * x + L.ek[2][k] * y;
+        const float t_ = L.ek[3][k] + L.ek[4][k] * x + L.ek[5][k] * y;
+        if (!(s_ > -0.0001f && t_ > -0.0001f && (s_ + t_) < L.ek[6][k])) continue;
+        const float fd = fmaf(L.pa[k], x, fmaf(L.pb[k], y, L.pc[k]));
+        const uint32_t d = sat_u32(kMulint / fd);
+        const uint32_t pitch = kMode == RasterMode::Shadow ? sh.L : L.pitch[kMode == RasterMode::Shadow ? 0 : k];
+        const size_t pix = (size_t)(int)y * pitch + (size_t)(int)x;
+        if constexpr (kMode == RasterMode::Id) {
+          const uint32_t val = L.dst[k][pix];
+          if (d > val - kBufError && d < val + kBufError)
+            atomicMax(&L.ids[k][pix], ((unsigned long long)(L.key[k] + 1u) << 32) | (c0 + (uint32_t)k));
+        } else {
+          atomicMin(&L.dst[k][pix], d);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      s = stage_end;
+      c0 += kWave;
     }
-    __builtin_amdgcn_wave_barrier();
   }
 }
 
 int device_cu_count();
 
 hipError_t launch_depth(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
-  hipLaunchKernelGGL(k_raster<RasterMode::Depth>, dim3(device_cu_count() * blocks_per_cu), dim3(256), 0, s, a,
-                     ShadowTarget{nullptr, 0}, kOpSize);
+  hipLaunchKernelGGL((k_raster<RasterMode::Depth, kOpSize>), dim3(device_cu_count() * blocks_per_cu), dim3(256), 0,
+                     s, a, ShadowTarget{nullptr, 0});
   return hipGetLastError();
 }
 
@@ -426,8 +478,8 @@ hipError_t launch_depth(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
 // the BUF_ERROR window the pixel keeps the largest (2*tri + sub) key, which
 // is the fragment a sequential in-order execution writes last.
 hipError_t launch_id(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
-  hipLaunchKernelGGL(k_raster<RasterMode::Id>, dim3(device_cu_count() * blocks_per_cu), dim3(256), 0, s, a,
-                     ShadowTarget{nullptr, 0}, kOpSize);
+  hipLaunchKernelGGL((k_raster<RasterMode::Id, kOpSize>), dim3(device_cu_count() * blocks_per_cu), dim3(256), 0,
+                     s, a, ShadowTarget{nullptr, 0});
   return hipGetLastError();
 }
 
@@ -437,10 +489,7 @@ struct FaceTable { SinCos f[6]; };
 __global__ void __launch_bounds__(256) k_prearrange_shadow(const TriPos* __restrict__ tris, uint32_t ntri,
                                                            const ObjDesc* __restrict__ objs, uint32_t nobj,
                                                            float lx, float ly, float lz, uint32_t L,
-                                                           int only_static, FaceTable ft,
-                                                           CutTri* __restrict__ cuts, uint32_t cap_cut,
-                                                           uint2* __restrict__ frags, uint32_t cap_frag,
-                                                           uint32_t* __restrict__ counters) {
+                                                           int only_static, FaceTable ft, Lists lists) {
   const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
   const float ew = (float)L, eh = (float)L, efov = (float)L / 2.0f;
   const f3 lpos = mk3(lx, ly, lz);
@@ -466,45 +515,25 @@ __global__ void __launch_bounds__(256) k_prearrange_shadow(const TriPos* __restr
       }
     }
   }
-  // Count pass over the touched faces, then allocate, then write.
-  uint32_t ncut = 0, nfrag = 0;
   const bool two_sided = live && (G.flags & OCR_FEATURE_TWO_SIDED) > 0;
+  // Faces one at a time; every lane takes part in each face's allocation
+  // (a lane that does not touch the face appends nothing).
 #pragma unroll
   for (int kk = 0; kk < 6; kk++) {
-    if (!(touched >> kk & 1)) continue;
+    if (__ballot((touched >> kk) & 1) == 0ull) continue;
     f3 tp[2][3];
-    int num = transform_clip_project(T, G, lpos, ft.f[kk], efov, ew, eh, tp);
+    float rc[2] = {0.f, 0.f};
+    uint32_t nf[2] = {0u, 0u}, ns[2] = {0u, 0u};
+    if ((touched >> kk) & 1) {
+      int num = transform_clip_project(T, G, lpos, ft.f[kk], efov, ew, eh, tp);
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-      if (i >= num) continue;
-      float rc;
-      uint32_t n = cull_count(tp[i], ew, eh, two_sided, kOpSizeLight, rc);
-      if (n) { ncut++; nfrag += n; }
+      for (int i = 0; i < 2; i++)
+        if (i < num) {
+          nf[i] = cull_count(tp[i], ew, eh, two_sided, kOpSizeLight, rc[i]);
+          ns[i] = triangle_samples(tp[i], ew, eh, kOpSizeLight, nf[i]);
+        }
     }
-  }
-  uint32_t cut_end, frag_end;
-  uint32_t c = wave_alloc(&counters[kCutCount], ncut, cut_end);
-  uint32_t f = wave_alloc(&counters[kFragCount], nfrag, frag_end);
-  if (cut_end > cap_cut || frag_end > cap_frag) {
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicOr(&counters[kOverflow], 1u);
-    return;
-  }
-#pragma unroll
-  for (int kk = 0; kk < 6; kk++) {
-    if (!(touched >> kk & 1)) continue;
-    f3 tp[2][3];
-    int num = transform_clip_project(T, G, lpos, ft.f[kk], efov, ew, eh, tp);
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-      if (i >= num) continue;
-      float rc;
-      uint32_t n = cull_count(tp[i], ew, eh, two_sided, kOpSizeLight, rc);
-      if (!n) continue;
-      store_cut(&cuts[c], tp[i], rc, (uint32_t)kk, T.oid);
-      for (uint32_t a = 0; a < n; a++) frags[f + a] = make_uint2(c, a);
-      f += n;
-      c++;
-    }
+    append_cuts(lists, tp, rc, nf, ns, (uint32_t)kk, 0u, T.oid, 0u);
   }
 }
 
@@ -516,16 +545,17 @@ hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, 
                                hipStream_t s) {
   FaceTable ft;
   for (int i = 0; i < 6; i++) ft.f[i] = sc->ctx->face_sc[i];
+  const Lists lists{sc->sh_cuts, sc->sh_counters, sc->sh_cap_cut};
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, sc->sh_counters);
   if (sc->ntri)
     hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + 255) / 256), dim3(256), 0, s, sc->tri_pos,
                        sc->ntri, sc->objs, sc->nobj, lpos[0], lpos[1], lpos[2], sc->lbuf_dim, only_static,
-                       ft, sc->sh_cuts, sc->sh_cap_cut, sc->sh_frags, sc->sh_cap_frag, sc->sh_counters);
+                       ft, lists);
   PassArgs a;
   memset(&a, 0, sizeof(a));
-  a.lists = Lists{sc->sh_cuts, sc->sh_frags, sc->sh_counters, sc->sh_cap_cut, sc->sh_cap_frag};
-  hipLaunchKernelGGL(k_raster<RasterMode::Shadow>, dim3(device_cu_count() * 8), dim3(256), 0, s, a,
-                     ShadowTarget{map, sc->lbuf_dim}, kOpSizeLight);
+  a.lists = lists;
+  hipLaunchKernelGGL((k_raster<RasterMode::Shadow, kOpSizeLight>), dim3(device_cu_count() * 8), dim3(256), 0, s,
+                     a, ShadowTarget{map, sc->lbuf_dim});
   return hipGetLastError();
 }
 

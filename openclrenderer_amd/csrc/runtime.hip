@@ -128,7 +128,7 @@ static PassArgs single_pass(const Scene* s, const ocr_view* view, Frame* f) {
   a.pix_slots = round256((uint64_t)view->width * view->height);
   a.tile_w_log2 = 4;
   a.tiles = view_tiles(view->width, view->height, a.tile_w_log2);
-  a.lists = Lists{f->cuts, f->frags, f->counters, f->cap_cut, f->cap_frag};
+  a.lists = Lists{f->cuts, f->counters, f->cap_cut};
   return a;
 }
 
@@ -273,7 +273,6 @@ void ocr_scene_destroy(ocr_scene* s) {
   dev_free(s->shadow_maps);
   dev_free(s->static_maps);
   dev_free(s->sh_cuts);
-  dev_free(s->sh_frags);
   dev_free(s->sh_counters);
   if (s->ev_sh0) (void)hipEventDestroy(s->ev_sh0);
   if (s->ev_sh1) (void)hipEventDestroy(s->ev_sh1);
@@ -394,8 +393,8 @@ int ocr_scene_read_atlas(ocr_scene* s, uint8_t* pages, uint32_t* nums, uint32_t*
 // ---------------------------------------------------------------- lights
 int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_t light_buffer_dim) {
   if (!s || (n && !lights)) return fail(OCR_ERR_INVALID, "ocr_lights_upload: bad argument");
-  if (light_buffer_dim < 8 || light_buffer_dim > 8192)
-    return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light buffer dimension must be 8..8192");
+  if (light_buffer_dim < 8 || (uint64_t)light_buffer_dim * light_buffer_dim > kMaxTargetPixels)
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light buffer dimension must be 8..4096");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   dev_free(s->lights);
   dev_free(s->shadow_maps);
@@ -425,16 +424,13 @@ int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_
   return OCR_OK;
 }
 
-static int ensure_shadow_lists(ocr_scene* s, uint32_t cap_cut, uint32_t cap_frag) {
-  if (s->sh_cuts && s->sh_cap_cut >= cap_cut && s->sh_frags && s->sh_cap_frag >= cap_frag) return OCR_OK;
+static int ensure_shadow_lists(ocr_scene* s, uint32_t cap_cut) {
+  if (s->sh_cuts && s->sh_cap_cut >= cap_cut) return OCR_OK;
   dev_free(s->sh_cuts);
-  dev_free(s->sh_frags);
-  s->sh_cap_cut = s->sh_cap_frag = 0;
-  int rc;
-  if ((rc = dev_alloc(&s->sh_cuts, cap_cut, "shadow cut list")) || (rc = dev_alloc(&s->sh_frags, cap_frag, "shadow fragment list")))
-    return rc;
+  s->sh_cap_cut = 0;
+  int rc = dev_alloc(&s->sh_cuts, cap_cut, "shadow cut list");
+  if (rc) return rc;
   s->sh_cap_cut = cap_cut;
-  s->sh_cap_frag = cap_frag;
   return OCR_OK;
 }
 
@@ -443,11 +439,11 @@ int ocr_shadow_pass(ocr_scene* s, void* stream) {
   if (s->n_shadow == 0 && !(s->static_dirty && s->n_static)) return OCR_OK;
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   hipStream_t st = (hipStream_t)stream;
-  if (!s->sh_cuts) {
-    // Each triangle reaches at most 6 faces x 2 clipped pieces.
-    uint64_t cut = std::min<uint64_t>((uint64_t)s->ntri * 12 + 1024, 0xFFFFFFF0ull);
-    uint64_t frag = std::min<uint64_t>((uint64_t)s->ntri * 4 + (1u << 20), 0xFFFFFFF0ull);
-    int rc = ensure_shadow_lists(s, (uint32_t)cut, (uint32_t)frag);
+  {
+    // A triangle reaches the faces its three vertices fall in, each as at
+    // most 2 clipped pieces, so 6 records per triangle always fit.
+    uint64_t cut = std::min<uint64_t>((uint64_t)s->ntri * 6 + 64, 0xFFFFFFF0ull);
+    int rc = ensure_shadow_lists(s, (uint32_t)cut);
     if (rc) return rc;
   }
   const size_t per = (size_t)6 * s->lbuf_dim * s->lbuf_dim;
@@ -483,15 +479,15 @@ int ocr_scene_read_shadow_maps(ocr_scene* s, uint32_t* dynamic_maps, uint32_t* s
     OCR_HIP_CHECK(hipMemcpy(dynamic_maps, s->shadow_maps, per * s->n_shadow, hipMemcpyDeviceToHost));
   if (static_maps && s->n_static)
     OCR_HIP_CHECK(hipMemcpy(static_maps, s->static_maps, per * s->n_static, hipMemcpyDeviceToHost));
-  if (ov) return fail(OCR_ERR_OVERFLOW, "shadow work list overflowed");
+  if (ov) return fail(OCR_ERR_UNSUPPORTED, "shadow pass: more raster samples than one pass can hold");
   return OCR_OK;
 }
 
 // ---------------------------------------------------------------- frames
 int ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** out) {
   if (!ctx || !out || width < 4 || height < 4) return fail(OCR_ERR_INVALID, "ocr_frame_create: bad argument (frames are at least 4x4)");
-  if ((uint64_t)width * height >= (1ull << 31) || width > 32768 || height > 32768)
-    return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_create: frame too large");
+  if ((uint64_t)width * height > kMaxTargetPixels || width > 32768 || height > 32768)
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_create: frame too large (at most 2^24 pixels)");
   *out = nullptr;
   OCR_HIP_CHECK(hipSetDevice(ctx->device));
   ocr_frame* f = new (std::nothrow) ocr_frame();
@@ -525,7 +521,6 @@ void ocr_frame_destroy(ocr_frame* f) {
   dev_free(f->rgba);
   dev_free(f->normals);
   dev_free(f->cuts);
-  dev_free(f->frags);
   dev_free(f->counters);
   for (hipEvent_t e : f->ev)
     if (e) (void)hipEventDestroy(e);
@@ -544,40 +539,29 @@ int ocr_frame_set_profiling(ocr_frame* f, int enable) {
   return OCR_OK;
 }
 
-int ocr_frame_reserve(ocr_frame* f, uint32_t max_fragments) {
-  if (!f || max_fragments == 0) return fail(OCR_ERR_INVALID, "ocr_frame_reserve: bad argument");
+int ocr_frame_reserve(ocr_frame* f, uint32_t max_cut_triangles) {
+  if (!f || max_cut_triangles == 0) return fail(OCR_ERR_INVALID, "ocr_frame_reserve: bad argument");
   OCR_HIP_CHECK(hipSetDevice(f->ctx->device));
   OCR_HIP_CHECK(hipDeviceSynchronize());
-  dev_free(f->frags);
-  f->cap_frag = 0;
-  f->want_frag = max_fragments;
-  f->exact_frag = true;
+  dev_free(f->cuts);
+  f->cap_cut = 0;
+  f->want_cut = max_cut_triangles;
   return OCR_OK;
 }
 
+// The cut list holds at most 2 pieces per scene triangle; unless the caller
+// reserved a capacity it is sized for that, so it never overflows.
 static int ensure_frame_lists(ocr_frame* f, uint32_t ntri) {
-  uint64_t want_cut = (uint64_t)ntri * 2 + 64;
-  uint64_t want_frag = f->exact_frag ? (uint64_t)f->want_frag
-                                     : std::max<uint64_t>((uint64_t)ntri * 2 + (uint64_t)f->width * f->height / 64 + 65536,
-                                                          f->want_frag);
-  want_cut = std::min<uint64_t>(want_cut, 0xFFFFFFF0ull);
-  want_frag = std::min<uint64_t>(want_frag, 0xFFFFFFF0ull);
-  if (f->cuts && f->cap_cut >= want_cut && f->frags && f->cap_frag >= want_frag) return OCR_OK;
+  uint64_t want = f->want_cut ? (uint64_t)f->want_cut : (uint64_t)ntri * 2 + 64;
+  want = std::min<uint64_t>(want, 0xFFFFFFF0ull);
+  if (f->cuts && f->cap_cut == want) return OCR_OK;
+  if (f->cuts && !f->want_cut && f->cap_cut >= want) return OCR_OK;
   OCR_HIP_CHECK(hipDeviceSynchronize());
-  if (!f->cuts || f->cap_cut < want_cut) {
-    dev_free(f->cuts);
-    f->cap_cut = 0;
-    int rc = dev_alloc(&f->cuts, want_cut, "cut list");
-    if (rc) return rc;
-    f->cap_cut = (uint32_t)want_cut;
-  }
-  if (!f->frags || f->cap_frag < want_frag) {
-    dev_free(f->frags);
-    f->cap_frag = 0;
-    int rc = dev_alloc(&f->frags, want_frag, "fragment list");
-    if (rc) return rc;
-    f->cap_frag = (uint32_t)want_frag;
-  }
+  dev_free(f->cuts);
+  f->cap_cut = 0;
+  int rc = dev_alloc(&f->cuts, want, "cut list");
+  if (rc) return rc;
+  f->cap_cut = (uint32_t)want;
   return OCR_OK;
 }
 
@@ -627,7 +611,7 @@ int ocr_frame_get_stats(ocr_frame* f, void* stream, ocr_frame_stats* out) {
   uint32_t c[kCounterWords];
   OCR_HIP_CHECK(hipMemcpy(c, f->counters, sizeof(c), hipMemcpyDeviceToHost));
   memset(out, 0, sizeof(*out));
-  out->n_cut = c[kCutCount];
+  out->n_cut = c[kPacked + 1];
   out->n_frag = c[kFragCount];
   out->overflow = c[kOverflow] ? 1u : 0u;
   out->n_profiled = (uint32_t)f->n_profiled;
@@ -706,11 +690,13 @@ int ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id, uint8_t* rgb
   OCR_HIP_CHECK(hipStreamSynchronize(st));
   uint32_t c[kCounterWords];
   OCR_HIP_CHECK(hipMemcpy(c, f->counters, sizeof(c), hipMemcpyDeviceToHost));
+  if (c[kOverflow] & kOverflowSamples)
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_read: the render needs more than 2^30 raster samples");
   if (c[kOverflow]) {
-    // Grow the lists so the next render of this frame fits.
-    f->want_frag = (uint32_t)std::min<uint64_t>((uint64_t)c[kFragCount] + c[kFragCount] / 4 + 1024, 0xFFFFFFF0ull);
-    f->exact_frag = false;
-    return fail(OCR_ERR_OVERFLOW, "ocr_frame_read: fragment list overflowed; re-render the frame");
+    // A reserved cut list was too small: size the next render's list for
+    // this one's count (high word of the packed counter).
+    f->want_cut = (uint32_t)std::min<uint64_t>((uint64_t)c[kPacked + 1] + c[kPacked + 1] / 4 + 64, 0xFFFFFFF0ull);
+    return fail(OCR_ERR_OVERFLOW, "ocr_frame_read: cut list overflowed; re-render the frame");
   }
   size_t npix = (size_t)f->width * f->height;
   if (depth) OCR_HIP_CHECK(hipMemcpy(depth, f->depth, 4 * npix, hipMemcpyDeviceToHost));

@@ -234,8 +234,10 @@ class Frame:
         check(self.lib.ocr_frame_get_stats(self.h, stream, C.byref(s)))
         return {k: getattr(s, k) for k, _ in FrameStats._fields_}
 
-    def reserve(self, max_fragments):
-        check(self.lib.ocr_frame_reserve(self.h, max_fragments))
+    def reserve(self, max_cut_triangles):
+        """Fixes the cut-list capacity; a render that needs more reports
+        OCR_ERR_OVERFLOW and grows it."""
+        check(self.lib.ocr_frame_reserve(self.h, max_cut_triangles))
 
     def set_profiling(self, on=True):
         check(self.lib.ocr_frame_set_profiling(self.h, 1 if on else 0))
