@@ -63,8 +63,18 @@ __global__ void __launch_bounds__(256) k_calib_scatter_store(uint32_t* __restric
   for (; i < n; i += stride) buf[mix32((uint32_t)i) % words] = (uint32_t)i;
 }
 
+// One 64-bit atomicAdd per wavefront on a single address (n wavefront
+// iterations), the pattern of a global work-list counter.
+__global__ void __launch_bounds__(256) k_calib_atomic_same(unsigned long long* __restrict__ counter, size_t n) {
+  const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const size_t nwaves = (size_t)gridDim.x * blockDim.x / 64;
+  for (size_t i = wave; i < n; i += nwaves)
+    if ((threadIdx.x & 63) == 0) atomicAdd(counter, 1ull);
+}
+
 // Returns the average kernel time in microseconds of `iters` launches
-// (mode 0: u32 atomicMin, 1: u64 atomicMax, 2: plain scattered store).
+// (mode 0: u32 atomicMin, 1: u64 atomicMax, 2: plain scattered store,
+// 3: one same-address u64 atomicAdd per wavefront, n wavefront iterations).
 extern "C" double ocr_calib_atomics(int mode, uint32_t words, size_t n, uint32_t cluster, int iters) {
   void* buf = nullptr;
   if (hipMalloc(&buf, (size_t)words * 8) != hipSuccess) return -1.0;
@@ -77,7 +87,8 @@ extern "C" double ocr_calib_atomics(int mode, uint32_t words, size_t n, uint32_t
     if (w == 2) (void)hipEventRecord(e0, nullptr);
     if (mode == 0) hipLaunchKernelGGL(k_calib_atomic_min, grid, block, 0, nullptr, (uint32_t*)buf, words, n, cluster);
     else if (mode == 1) hipLaunchKernelGGL(k_calib_atomic_max64, grid, block, 0, nullptr, (unsigned long long*)buf, words, n, cluster);
-    else hipLaunchKernelGGL(k_calib_scatter_store, grid, block, 0, nullptr, (uint32_t*)buf, words, n);
+    else if (mode == 2) hipLaunchKernelGGL(k_calib_scatter_store, grid, block, 0, nullptr, (uint32_t*)buf, words, n);
+    else hipLaunchKernelGGL(k_calib_atomic_same, grid, block, 0, nullptr, (unsigned long long*)buf, n);
   }
   (void)hipEventRecord(e1, nullptr);
   hipError_t e = hipEventSynchronize(e1);

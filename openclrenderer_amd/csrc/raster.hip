@@ -38,24 +38,65 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
-// Allocates `ncut` records and `nsmp` samples per lane with one 64-bit atomic
-// per wavefront on the packed counter (cuts << 32 | samples).  Returns this
-// lane's first record and first sample; `end` receives the wavefront's end.
-__device__ __forceinline__ unsigned long long wave_alloc_packed(unsigned long long* counter, uint32_t ncut,
-                                                                uint32_t nsmp, unsigned long long& end) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint32_t ic = wave_incl_scan(ncut);
-  const uint32_t is = wave_incl_scan(nsmp);
-  const uint32_t tc = __shfl(ic, kWave - 1, kWave);
-  const uint32_t ts = __shfl(is, kWave - 1, kWave);
-  const unsigned long long total = ((unsigned long long)tc << 32) + ts;
-  unsigned long long base = 0;
-  if (lane == 0 && total) base = atomicAdd(counter, total);
-  const uint32_t blo = __shfl((uint32_t)base, 0, kWave);
-  const uint32_t bhi = __shfl((uint32_t)(base >> 32), 0, kWave);
-  base = ((unsigned long long)bhi << 32) | blo;
-  end = base + total;
-  return base + ((unsigned long long)(ic - ncut) << 32) + (is - nsmp);
+// Prearrange blocks: 1024 work items, one triangle each.  Cut records and
+// sample ranges are allocated once per block (one 64-bit atomicAdd on the
+// packed counter (cuts << 32 | samples) per 1024 triangles): every wavefront
+// of the render appending to one counter would serialise on that address.
+constexpr int kPreThreads = 1024;
+constexpr int kPreWaves = kPreThreads / kWave;
+
+struct BlockAllocLds {
+  uint32_t cuts[kPreWaves], smps[kPreWaves], frags[kPreWaves];
+  unsigned long long base;
+  uint32_t ok;
+};
+
+// Returns false (for the whole block) when the block's records do not fit the
+// list or the sample space; the overflow word then says which.  The block's
+// sample total is checked before its atomic, and each atomic adds less than
+// kMaxSamples, so the low half reaches the threshold and sets the flag before
+// it could carry into the record count.
+__device__ __forceinline__ bool block_alloc(const Lists& L, BlockAllocLds& sh, uint32_t ncut, uint32_t nsmp,
+                                            uint32_t nfrag, uint32_t& cut0, uint32_t& smp0) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const uint32_t ic = wave_incl_scan(ncut), is = wave_incl_scan(nsmp), ifr = wave_incl_scan(nfrag);
+  if (lane == kWave - 1) {
+    sh.cuts[wave] = ic;
+    sh.smps[wave] = is;
+    sh.frags[wave] = ifr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tc = 0, tf = 0;
+    unsigned long long ts = 0;
+    for (int w = 0; w < kPreWaves; w++) {
+      const uint32_t c = sh.cuts[w], sm = sh.smps[w];
+      sh.cuts[w] = tc;
+      sh.smps[w] = (uint32_t)ts;
+      tc += c;
+      ts += sm;
+      tf += sh.frags[w];
+    }
+    uint32_t bad = ts > kMaxSamples ? kOverflowSamples : 0u;
+    unsigned long long base = 0;
+    if (!bad && tc) {
+      base = atomicAdd(reinterpret_cast<unsigned long long*>(L.counters + kPacked), ((unsigned long long)tc << 32) + ts);
+      const unsigned long long end = base + ((unsigned long long)tc << 32) + ts;
+      if ((end >> 32) > L.cap_cut) bad |= kOverflowCuts;
+      if ((uint32_t)end > kMaxSamples) bad |= kOverflowSamples;
+    }
+    if (tf) atomicAdd(&L.counters[kFragCount], tf);
+    if (bad) atomicOr(&L.counters[kOverflow], bad);
+    sh.base = base;
+    sh.ok = bad == 0;
+  }
+  __syncthreads();
+  const unsigned long long base = sh.base;
+  cut0 = (uint32_t)(base >> 32) + sh.cuts[wave] + ic - ncut;
+  smp0 = (uint32_t)base + sh.smps[wave] + is - nsmp;
+  const bool ok = sh.ok != 0;
+  __syncthreads();   // the LDS record is reused by the block's next allocation
+  return ok;
 }
 
 // ---------------------------------------------------------- setup helpers
@@ -175,26 +216,17 @@ __device__ __forceinline__ int transform_clip_project(const TriPos& T, const Obj
   return n;
 }
 
-// Appends this lane's cut triangles that hold samples (0..2) with their
-// sample ranges.  A wavefront that would pass the list capacity or the
-// sample limit writes nothing and sets the overflow word, which every later
-// pass of the render checks before reading the list.
-__device__ __forceinline__ void append_cuts(const Lists& L, const f3 tp[2][3], const float rc[2],
-                                            const uint32_t nf[2], const uint32_t ns[2], uint32_t key0,
-                                            uint32_t key_step, uint32_t oid, uint32_t view) {
+// Appends this work item's cut triangles that hold samples (0..2) with their
+// sample ranges; called by every work item of the block.  A block that would
+// pass the list capacity or the sample limit writes nothing and sets the
+// overflow word, which every later pass of the render checks before reading
+// the list.
+__device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds& sh, const f3 tp[2][3],
+                                            const float rc[2], const uint32_t nf[2], const uint32_t ns[2],
+                                            uint32_t key0, uint32_t key_step, uint32_t oid, uint32_t view) {
   const uint32_t k0 = ns[0] > 0, k1 = ns[1] > 0;
-  unsigned long long end;
-  const unsigned long long b = wave_alloc_packed(reinterpret_cast<unsigned long long*>(L.counters + kPacked),
-                                                 k0 + k1, ns[0] + ns[1], end);
-  const uint32_t frags = wave_incl_scan(nf[0] * k0 + nf[1] * k1);
-  const int lane = threadIdx.x & (kWave - 1);
-  if (lane == kWave - 1 && frags) atomicAdd(&L.counters[kFragCount], frags);
-  const bool over_cut = (end >> 32) > L.cap_cut, over_smp = (uint32_t)end > kMaxSamples;
-  if (over_cut || over_smp) {
-    if (lane == 0) atomicOr(&L.counters[kOverflow], (over_cut ? kOverflowCuts : 0u) | (over_smp ? kOverflowSamples : 0u));
-    return;
-  }
-  uint32_t c = (uint32_t)(b >> 32), smp = (uint32_t)b;
+  uint32_t c, smp;
+  if (!block_alloc(L, sh, k0 + k1, ns[0] + ns[1], nf[0] * k0 + nf[1] * k1, c, smp)) return;
 #pragma unroll
   for (int i = 0; i < 2; i++) {
     if (!ns[i]) continue;
@@ -263,13 +295,15 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
 }
 
 // ------------------------------------------------------------- prearrange
-__global__ void __launch_bounds__(256) k_prearrange(PassArgs a) {
-  const uint32_t slot = blockIdx.x * 256u;
+__global__ void __launch_bounds__(kPreThreads) k_prearrange(PassArgs a) {
+  __shared__ BlockAllocLds sh;
+  // per-view triangle ranges are 256-aligned: each wavefront has one view
+  const uint32_t slot = blockIdx.x * (uint32_t)kPreThreads + (threadIdx.x & ~(uint32_t)(kWave - 1));
   const uint32_t vi = view_of<Range::Tri>(a, slot);
   const DevView& V = view_at(a, vi);
   const DevScene& S = scene_at(a, V.scene);
   const ViewParams& v = V.v;
-  const uint32_t id = slot - V.tri_begin + threadIdx.x;
+  const uint32_t id = slot - V.tri_begin + (threadIdx.x & (kWave - 1));
   const float ew = (float)v.width, eh = (float)v.height;
   f3 tp[2][3];
   float rc[2] = {0.f, 0.f};
@@ -293,12 +327,12 @@ __global__ void __launch_bounds__(256) k_prearrange(PassArgs a) {
       }
     }
   }
-  append_cuts(a.lists, tp, rc, nf, ns, 2u * id, 1u, oid, vi);
+  append_cuts(a.lists, sh, tp, rc, nf, ns, 2u * id, 1u, oid, vi);
 }
 
 hipError_t launch_prearrange(const PassArgs& a, hipStream_t s) {
   if (a.tri_slots == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_prearrange, dim3(a.tri_slots / 256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_prearrange, dim3((a.tri_slots + kPreThreads - 1) / kPreThreads), dim3(kPreThreads), 0, s, a);
   return hipGetLastError();
 }
 
@@ -486,10 +520,11 @@ hipError_t launch_id(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
 // ---------------------------------------------------------------- shadows
 struct FaceTable { SinCos f[6]; };
 
-__global__ void __launch_bounds__(256) k_prearrange_shadow(const TriPos* __restrict__ tris, uint32_t ntri,
+__global__ void __launch_bounds__(kPreThreads) k_prearrange_shadow(const TriPos* __restrict__ tris, uint32_t ntri,
                                                            const ObjDesc* __restrict__ objs, uint32_t nobj,
                                                            float lx, float ly, float lz, uint32_t L,
                                                            int only_static, FaceTable ft, Lists lists) {
+  __shared__ BlockAllocLds sh;
   const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
   const float ew = (float)L, eh = (float)L, efov = (float)L / 2.0f;
   const f3 lpos = mk3(lx, ly, lz);
@@ -516,11 +551,17 @@ __global__ void __launch_bounds__(256) k_prearrange_shadow(const TriPos* __restr
     }
   }
   const bool two_sided = live && (G.flags & OCR_FEATURE_TWO_SIDED) > 0;
-  // Faces one at a time; every lane takes part in each face's allocation
-  // (a lane that does not touch the face appends nothing).
+  // Faces one at a time; every work item of the block takes part in each
+  // face's allocation (one that does not touch the face appends nothing).
+  __shared__ int faces_touched;
+  if (threadIdx.x == 0) faces_touched = 0;
+  __syncthreads();
+  if (touched) atomicOr(&faces_touched, touched);
+  __syncthreads();
+  const int any_touched = faces_touched;
 #pragma unroll
   for (int kk = 0; kk < 6; kk++) {
-    if (__ballot((touched >> kk) & 1) == 0ull) continue;
+    if (!((any_touched >> kk) & 1)) continue;
     f3 tp[2][3];
     float rc[2] = {0.f, 0.f};
     uint32_t nf[2] = {0u, 0u}, ns[2] = {0u, 0u};
@@ -533,7 +574,7 @@ __global__ void __launch_bounds__(256) k_prearrange_shadow(const TriPos* __restr
           ns[i] = triangle_samples(tp[i], ew, eh, kOpSizeLight, nf[i]);
         }
     }
-    append_cuts(lists, tp, rc, nf, ns, (uint32_t)kk, 0u, T.oid, 0u);
+    append_cuts(lists, sh, tp, rc, nf, ns, (uint32_t)kk, 0u, T.oid, 0u);
   }
 }
 
@@ -548,7 +589,7 @@ hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, 
   const Lists lists{sc->sh_cuts, sc->sh_counters, sc->sh_cap_cut};
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, sc->sh_counters);
   if (sc->ntri)
-    hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + 255) / 256), dim3(256), 0, s, sc->tri_pos,
+    hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + kPreThreads - 1) / kPreThreads), dim3(kPreThreads), 0, s, sc->tri_pos,
                        sc->ntri, sc->objs, sc->nobj, lpos[0], lpos[1], lpos[2], sc->lbuf_dim, only_static,
                        ft, lists);
   PassArgs a;
