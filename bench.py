@@ -1,19 +1,30 @@
-"""Benchmark of the plain HIP path on one MI355X.
+"""Benchmark of the HIP path on one MI355X.
 
-Workload (default): one 1920x1080 frame of a 1,000,024-triangle scene (400
-instanced icospheres / subdivided cubes, 4 point lights, 2048^2 atlas with its
-mip chain), rendered prearrange -> depth -> id -> shade.  A step is one frame
-with the scene, atlas and lights already resident in HBM.
+Workload (default, --config 2): one 1920x1080 frame of a 1,000,024-triangle
+scene (400 instanced icospheres / subdivided cubes, 4 point lights, 2048^2
+atlas with its mip chain), rendered prearrange -> depth -> id -> shade.  A
+step is one frame with the scene, atlas and lights already resident in HBM.
 
 Prints one JSON line: frames/s (value), Mpix/s, per-pass device times from
 HIP events on the render stream, the shading kernel's roofline against
 MI355X's 8 TB/s HBM, and the oracle timed on this host's cores.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Other workloads (one JSON line each):
+  --config 1  800x600, 2,480 triangles, 1 light
+  --config 3  3840x2160, 4M triangles, 16 shadow-casting lights: a step is
+              the shadow pass (16 cube maps of 6 x 1024^2) plus the frame
+  --config 4  64 views of mixed size over 8 resident scenes (50k-2M
+              triangles): one batched render per step vs 64 plain renders
+  --config 5  served requests (Poisson arrivals at several rates) against 12
+              resident scenes: throughput and p50/p99 latency, batched vs one
+              request at a time
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C] [--no-cpu]
 """
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -27,11 +38,45 @@ HBM_PEAK_GBS = 8000.0
 UINT_MAX = 0xFFFFFFFF
 
 
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        _hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        _hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    return _hip
+
+
 def hip_sync():
-    lib = C.CDLL("libamdhip64.so")
-    rc = lib.hipDeviceSynchronize()
+    rc = hip().hipDeviceSynchronize()
     if rc != 0:
         raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
+
+
+class EventPair:
+    """Two HIP events on the null stream (the stream the passes run on)."""
+
+    def __init__(self):
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        for e in (self.a, self.b):
+            if hip().hipEventCreate(C.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+
+    def start(self):
+        hip().hipEventRecord(self.a, None)
+
+    def stop(self):
+        hip().hipEventRecord(self.b, None)
+
+    def ms(self):
+        hip().hipEventSynchronize(self.b)
+        t = C.c_float()
+        hip().hipEventElapsedTime(C.byref(t), self.a, self.b)
+        return t.value
 
 
 def traffic_from_profiles(kernel, workload):
@@ -55,17 +100,22 @@ def shade_algorithmic_bytes(W, H, V, O, x_tex, x_shadow, L):
     return 20 * W * H + 132 * V + 52 * O + 4 * x_tex + 4 * x_shadow + 56 * L
 
 
-def cpu_baseline(sc, view, gpu_out, seconds):
+def cpu_baseline(sc, view, gpu_out, seconds, with_shadows=False):
+    """The oracle on this host's cores over a bounded sample of the same
+    workload: whole frames (with the shadow pass when a step has one)."""
     from oracle.binding import Oracle
     cores = min(os.cpu_count() or 1, 16)
     o = Oracle()
     atlas = o.atlas(sc.textures)
-    maps = o.shadow_maps(sc, cores) if np.any(sc.lights["shadow"]) else None
+    has_shadow = bool(np.any(sc.lights["shadow"]))
+    maps = None if with_shadows or not has_shadow else o.shadow_maps(sc, cores)
     times = []
     ref = None
     t_end = time.perf_counter() + seconds
     while True:
         t0 = time.perf_counter()
+        if with_shadows:
+            maps = o.shadow_maps(sc, cores)
         r = o.render(sc, view, atlas=atlas, maps=maps, nthreads=cores)
         times.append(time.perf_counter() - t0)
         ref = ref or r
@@ -77,47 +127,56 @@ def cpu_baseline(sc, view, gpu_out, seconds):
                       np.array_equal(gpu_out["tri_id"], ref["tri_id"]) and
                       int(np.abs(gpu_out["rgba"].astype(int) - ref["rgba"].astype(int)).max()) <= 1)
     med = float(np.median(times))
+    what = "shadow pass + frame" if with_shadows else "frames"
     return {"value": 1.0 / med, "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": f"{len(times)} full {view['width'][0]}x{view['height'][0]} frames of the same scene, "
+            "sample": f"{len(times)} full {view['width'][0]}x{view['height'][0]} {what} of the same scene, "
                       f"oracle on {cores} OpenMP threads, median {med * 1e3:.0f} ms/frame",
             "ms_per_frame": med * 1e3}, parity
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2])
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--calib", action="store_true", help="also run the copy kernel used to scale PMC counters")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is design-only)", "n_gpus": args.gpus}))
-        sys.exit(2)
+def base_line(args, metric, value, unit, ms_per_step, config, dtype="f32"):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
 
+
+METRIC = "frames/s + Mpix/s at 1080p/1M tris; p50/p99 latency on mixed-size request stream"
+
+
+def run_plain(args):
     from openclrenderer_amd import runtime, scenes
     if args.config == 1:
         sc, view = scenes.config1()
         workload = "800x600 2480 tris (icosphere + 10x10 cubes), 1 light, 256^2 texture"
-    else:
+    elif args.config == 2:
         sc, view = scenes.config2()
         workload = "1920x1080 1M tris (400 instances), 4 point lights, 2048^2 atlas"
+    else:
+        sc, view = scenes.config3()
+        workload = "3840x2160 4M tris (1600 instances), 16 shadow-casting point lights (6x1024^2 maps), 8 1024^2 textures"
     W, H = int(view["width"][0]), int(view["height"][0])
+    shadows = bool(np.any((sc.lights["shadow"] == 1) & (sc.lights["is_static"] == 0)))
 
     ctx = runtime.Context(0)
     g = runtime.GpuScene(ctx, sc)
     fr = runtime.Frame(ctx, W, H)
+    if shadows:
+        g.shadow_pass()
     out = fr.render_read(g, view)          # sizes the work lists, reads the frame once
     for _ in range(args.warmup):
+        if shadows:
+            g.shadow_pass()
         fr.render(g, view)
     hip_sync()
     fr.set_profiling(True)
+    ev = [EventPair() for _ in range(args.steps)] if shadows else []
     hip_sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
+        if shadows:
+            ev[i].start()
+            g.shadow_pass()
+            ev[i].stop()
         fr.render(g, view)
     hip_sync()
     t1 = time.perf_counter()
@@ -126,6 +185,7 @@ def main():
         raise RuntimeError("work list overflow inside the timed region")
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     fps = 1e3 / ms_per_step
+    ms_shadow = float(np.mean([e.ms() for e in ev])) if ev else 0.0
 
     # algorithmic bytes of the shading launch
     tri = out["tri_id"]
@@ -143,24 +203,17 @@ def main():
         if cal.ocr_calib_copy(256 << 20, 3) != 0:
             raise RuntimeError("calibration copy failed")
 
-    cpu, parity = (None, None) if args.no_cpu else cpu_baseline(sc, view, out, args.cpu_seconds)
-    line = {
-        "metric": "frames/s + Mpix/s at 1080p/1M tris; p50/p99 latency on mixed-size request stream",
-        "value": fps,
-        "unit": "frames/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": workload, "width": W, "height": H, "triangles": sc.ntri,
-                   "lights": int(sc.lights.shape[0]), "path": "plain"},
+    cpu, parity = (None, None) if args.no_cpu else cpu_baseline(sc, view, out, args.cpu_seconds, shadows)
+    passes = {k: st[k] for k in ("ms_clear", "ms_prearrange", "ms_depth", "ms_id", "ms_shade", "ms_total")}
+    if shadows:
+        passes["ms_shadow"] = ms_shadow
+    line = base_line(args, METRIC, fps, "frames/s", ms_per_step,
+                     {"workload": workload, "width": W, "height": H, "triangles": sc.ntri,
+                      "lights": int(sc.lights.shape[0]), "shadow_lights": int(np.sum(sc.lights["shadow"] == 1)),
+                      "path": "plain"})
+    line.update({
         "mpix_per_s": W * H / (ms_per_step * 1e-3) / 1e6,
-        "passes_ms": {k: st[k] for k in ("ms_clear", "ms_prearrange", "ms_depth", "ms_id", "ms_shade", "ms_total")},
+        "passes_ms": passes,
         "work": {"cut_tris": st["n_cut"], "fragments": st["n_frag"], "visible_tris": V, "visible_objs": O,
                  "atlas_texels": x_tex, "shadow_texels": x_shadow},
         "roofline": {"bound": "hbm", "kernel": "k_shade", "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -168,10 +221,185 @@ def main():
                      "algorithmic_bytes": B},
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
-    }
+    })
     if cpu:
         line["speedup_vs_cpu"] = fps / cpu["value"]
     print(json.dumps(line), flush=True)
+
+
+def config4_workload():
+    from openclrenderer_amd import scenes
+    sizes = scenes.config4_scene_sizes()
+    scs = [scenes.small_scene(scenes.SEED_BASE + 40 + i, max(20, min(400, n // 5000)), n, n_lights=4,
+                              ntex=1, tex_size=1024, name=f"cfg4-{i}") for i, n in enumerate(sizes)]
+    views = scenes.config4_views()
+    return scs, views
+
+
+def run_batched(args):
+    from openclrenderer_amd import runtime
+    scs, views = config4_workload()
+    ctx = runtime.Context(0)
+    gs = [runtime.GpuScene(ctx, s) for s in scs]
+    frames = [runtime.Frame(ctx, int(v["width"][0]), int(v["height"][0])) for _, v in views]
+    owners = [gs[i] for i, _ in views]
+    vlist = [v for _, v in views]
+    b = runtime.Batch(ctx, len(views))
+    b.set_views(owners, vlist, frames)
+    b.render_checked()
+    tuning = b.autotune(reps=3)
+    b.render_checked(graph=True)
+    for _ in range(args.warmup):
+        b.launch_graph()
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.launch_graph()
+    hip_sync()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    b.stats()
+    # the same 64 views rendered one plain launch sequence at a time
+    plain = [runtime.Frame(ctx, f.width, f.height) for f in frames]
+    for f, g, v in zip(plain, owners, vlist):
+        f.render_read(g, v)
+    hip_sync()
+    seq_steps = max(1, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(seq_steps):
+        for f, g, v in zip(plain, owners, vlist):
+            f.render(g, v)
+    hip_sync()
+    t_seq = (time.perf_counter() - t0) / seq_steps
+    same = all(np.array_equal(a.read()["rgba"], p.read()["rgba"]) for a, p in zip(frames, plain))
+    npix = sum(f.width * f.height for f in frames)
+    line = base_line(args, "views/s, 64 mixed-size views over 8 resident scenes (batched vs sequential plain)",
+                     len(views) / t_batch, "views/s", t_batch * 1e3,
+                     {"workload": "64 views (320x240 .. 2560x1440) over 8 scenes of 50k-2M tris, 4 lights each",
+                      "views": len(views), "scenes": len(scs), "triangles": [s.ntri for s in scs],
+                      "path": "batched (hipGraph, autotuned)"})
+    line.update({"mpix_per_s": npix / t_batch / 1e6, "sequential_views_per_s": len(views) / t_seq,
+                 "sequential_ms_per_step": t_seq * 1e3, "speedup_vs_sequential": t_seq / t_batch,
+                 "tuning": {"raster_blocks_per_cu": tuning[0], "shade_tile_w_log2": tuning[1]},
+                 "batched_equals_sequential": bool(same)})
+    print(json.dumps(line), flush=True)
+
+
+def config5_workload(seed=None):
+    from openclrenderer_amd import scenes
+    rng = scenes.SplitMix64(scenes.SEED_BASE + 5 if seed is None else seed)
+    scs = []
+    for i in range(12):
+        n = int(round(math.exp(math.log(50_000) + (math.log(4_000_000) - math.log(50_000)) * rng.uniform())))
+        nl = 1 + int(rng.uniform() * 16)
+        scs.append(scenes.small_scene(scenes.SEED_BASE + 50 + i, max(20, min(400, n // 5000)), n,
+                                      n_lights=nl, shadow_lights=nl, ntex=1, tex_size=1024, name=f"cfg5-{i}"))
+    return scs, rng
+
+
+def run_served(args):
+    from openclrenderer_amd import runtime, scenes
+    scs, rng = config5_workload()
+    ctx = runtime.Context(0)
+    gs = [runtime.GpuScene(ctx, s) for s in scs]
+    res = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+    cum = np.cumsum([0.4, 0.3, 0.2, 0.1])
+    rates = [int(r) for r in args.rates.split(",")]
+    n_req = args.requests
+
+    def schedule(rate):
+        t, out = 0.0, []
+        for _ in range(n_req):
+            t += -math.log(max(rng.uniform(), 1e-12)) / rate
+            u = rng.uniform()
+            w, h = res[int(np.searchsorted(cum, u, side="right"))] if u < 1.0 else res[-1]
+            k = int(rng.uniform() * len(scs))
+            out.append((t, k, scenes.orbit_view(rng, w, h, 3 + 17 * rng.uniform())))
+        return out
+
+    def drive(srv, slots, sched):
+        lat, qwait, batch = [], [], []
+        t_base = srv.now_ns()
+        pending = 0
+        refused = 0
+        i = 0
+        done = 0
+        while done < len(sched):
+            now = (srv.now_ns() - t_base) * 1e-9
+            while i < len(sched) and sched[i][0] <= now:
+                rid = srv.submit(slots[sched[i][1]], sched[i][2])
+                if rid is None:
+                    refused += 1
+                    break                      # retried after completions are released
+                pending += 1
+                i += 1
+            for c in srv.poll(max_n=512, wait_ms=0):
+                lat.append((c.t_done_ns - c.t_submit_ns) * 1e-6)
+                qwait.append((c.t_start_ns - c.t_submit_ns) * 1e-6)
+                batch.append(c.batch_views)
+                if c.status != 0:
+                    raise RuntimeError(f"request failed with status {c.status}")
+                srv.release(c.request_id)
+                done += 1
+            if i < len(sched) and done < len(sched):
+                gap = sched[i][0] - (srv.now_ns() - t_base) * 1e-9
+                if gap > 2e-4:
+                    time.sleep(min(gap, 1e-3) * 0.5)
+        wall = (srv.now_ns() - t_base) * 1e-9
+        lat = np.asarray(lat)
+        return {"completed_per_s": len(sched) / wall, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                "queue_p50_ms": float(np.percentile(qwait, 50)), "mean_batch": float(np.mean(batch)),
+                "refused_submits": refused}
+
+    modes = {"batched": dict(max_batch_views=32, max_inflight=2),
+             "one_at_a_time": dict(max_batch_views=1, max_inflight=1)}
+    results = {}
+    for name, kw in modes.items():
+        srv = runtime.Server(ctx, max_batch_tris=24_000_000, hbm_fraction=0.25, read_mask=runtime.READ_RGBA, **kw)
+        slots = [srv.add_scene(g) for g in gs]
+        warm = schedule(2000)[:64]
+        drive(srv, slots, [(0.0, k, v) for _, k, v in warm])
+        results[name] = {}
+        for rate in rates:
+            r = drive(srv, slots, schedule(rate))
+            results[name][str(rate)] = r
+            print(json.dumps({"progress": name, "rate": rate, **r}), file=sys.stderr, flush=True)
+        srv.close()
+    top = str(max(rates))
+    best = results["batched"][top]
+    line = base_line(args, "served requests/s with p50/p99 latency (Poisson arrivals, mixed resolutions)",
+                     best["completed_per_s"], "requests/s", 1e3 / best["completed_per_s"],
+                     {"workload": "12 resident scenes (50k-4M tris, 1-16 shadowed lights), resolutions "
+                                  "640x480/1280x720/1920x1080/3840x2160 at .4/.3/.2/.1",
+                      "requests_per_rate": n_req, "rates": rates, "scenes_tris": [s.ntri for s in scs],
+                      "path": "served (continuous batching)"})
+    line.update({"served": results, "p99_ms_at_top_rate": best["p99_ms"],
+                 "one_at_a_time_p99_ms_at_top_rate": results["one_at_a_time"][top]["p99_ms"]})
+    print(json.dumps(line), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calib", action="store_true", help="also run the copy kernel used to scale PMC counters")
+    ap.add_argument("--requests", type=int, default=2000, help="served: requests per arrival rate")
+    ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="served: arrival rates (req/s)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is design-only)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.config in (1, 2, 3):
+        run_plain(args)
+    elif args.config == 4:
+        run_batched(args)
+    else:
+        run_served(args)
 
 
 if __name__ == "__main__":

@@ -173,7 +173,7 @@ int  ocr_scene_update_objs(ocr_scene* s, uint32_t first, uint32_t n,
  * array is descriptor tid == i. */
 int  ocr_atlas_upload(ocr_scene* s, const ocr_tex_src* texs, uint32_t ntex);
 /* light::build (light.cpp:145-276): uploads the light list and sizes the
- * cube shadow maps (n_shadow x 6 x dim^2 u32, dim = LIGHTBUFFERDIM, 8..4096). */
+ * cube shadow maps (n_shadow x 6 x dim^2 u32, dim = LIGHTBUFFERDIM, 8..4000). */
 int  ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n,
                        uint32_t light_buffer_dim);
 /* engine::generate_realtime_shadowing (engine.cpp:1601-1790): clears and
@@ -192,7 +192,7 @@ int  ocr_scene_read_shadow_maps(ocr_scene* s, uint32_t* dynamic_maps,
 
 /* ---- frames: offscreen framebuffer (replaces cl_gl_interop_texture) ----- */
 
-/* Frames are 4x4 up to 2^24 pixels (4096 x 4096, 3840 x 2160 ...). */
+/* Frames are 4x4 up to 16,000,000 pixels (3840 x 2160, 5120 x 2880 ...). */
 int  ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** out);
 void ocr_frame_destroy(ocr_frame* f);
 /* Fixes the capacity of the frame's cut-triangle list (the reference's
@@ -265,6 +265,63 @@ int  ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out);
 /* Times raster grid sizes x shading tile shapes on the current views and
  * keeps the fastest (reps graph launches each). */
 int  ocr_batch_autotune(ocr_batch* b, void* stream, uint32_t reps, ocr_batch_tuning* best);
+
+/* ---- served form ----------------------------------------------------------
+ * No counterpart in the reference, whose main loop renders one view per draw
+ * call (main.cpp:254-342 -> engine::draw_bulk_objs_n, engine.cpp:2356-2445).
+ * A server keeps scenes and their shadow maps resident, admits requests
+ * against a budget of free HBM and renders whatever is pending as one batch
+ * (ocr_batch) each time one of max_inflight batch slots frees. */
+
+typedef struct ocr_server ocr_server;
+
+#define OCR_SERVE_READ_RGBA  1u   /* completion host buffer holds RGBA8      */
+#define OCR_SERVE_READ_DEPTH 2u   /* ... then the u32 depth buffer           */
+#define OCR_SERVE_READ_IDKEY 4u   /* ... then the u64 id keys (see ocr_frame_read) */
+
+typedef struct ocr_server_config {
+    uint32_t max_batch_views;   /* requests per batch, 1..4096             */
+    uint32_t max_inflight;      /* batch slots (streams) in flight, 1..8   */
+    uint64_t max_batch_tris;    /* scene triangles per batch (a larger single
+                                   request still runs alone); sizes each
+                                   slot's cut list                         */
+    float    hbm_fraction;      /* of the free HBM at creation, held by
+                                   admitted requests' frames at most       */
+    uint32_t read_mask;         /* OCR_SERVE_READ_* copied to host per request */
+} ocr_server_config;            /* 24 B */
+
+typedef struct ocr_completion {
+    uint64_t request_id;
+    uint64_t t_submit_ns, t_start_ns, t_done_ns;   /* ocr_server_now_ns clock */
+    void*    host;              /* pinned outputs per read_mask; valid until
+                                   ocr_server_release                      */
+    int32_t  status;            /* OCR_OK or the batch's error             */
+    uint32_t scene_slot, width, height;
+    uint32_t batch_views;       /* requests rendered in the same batch     */
+    uint32_t _pad;
+} ocr_completion;               /* 64 B */
+
+typedef struct ocr_server_info {
+    uint64_t hbm_budget, hbm_held;
+    uint64_t n_batches, n_done, n_refused;
+    uint32_t n_pending, n_frames;
+} ocr_server_info;              /* 48 B */
+
+int  ocr_server_create(ocr_ctx* ctx, const ocr_server_config* cfg, ocr_server** out);
+/* Registers a scene (atlas uploaded) and renders its shadow maps once. */
+int  ocr_server_add_scene(ocr_server* s, ocr_scene* scene, uint32_t* slot);
+/* OCR_ERR_BUSY when the request's frame (20 B per pixel) does not fit the
+ * HBM budget beside the admitted, unreleased requests. */
+int  ocr_server_submit(ocr_server* s, uint32_t scene_slot, const ocr_view* view,
+                       uint64_t* request_id);
+/* Up to `max` completions; waits up to wait_ms for the first one. */
+int  ocr_server_poll(ocr_server* s, ocr_completion* out, uint32_t max, uint32_t* n, int wait_ms);
+/* Returns a completed request's host buffer and HBM budget to the server. */
+int  ocr_server_release(ocr_server* s, uint64_t request_id);
+int  ocr_server_get_info(ocr_server* s, ocr_server_info* info);
+uint64_t ocr_server_now_ns(void);
+/* Waits for in-flight batches, drops requests that never started. */
+void ocr_server_destroy(ocr_server* s);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libocr.so")
 CALIB_LIB = os.path.join(HERE, "libocr_calib.so")
 FACADE_LIB = os.path.join(HERE, "libocr_engine.so")
 FACADE_DEMO = os.path.join(HERE, "ocr_facade_demo")
-SOURCES = ["runtime.hip", "raster.hip", "shade.hip", "atlas.hip", "batch.hip"]
+SOURCES = ["runtime.hip", "raster.hip", "shade.hip", "atlas.hip", "batch.hip", "server.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
@@ -26,7 +26,7 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-pthread",
 ]
 
 
@@ -56,7 +56,7 @@ def build(verbose=False, jobs=4):
     while procs:
         _wait(procs.pop(0))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp], verbose)
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-pthread", "-o", tmp], verbose)
     os.replace(tmp, LIB)
     # the C++ draw API over the C ABI, and its demo program
     inc = os.path.join(os.path.dirname(HERE), "include")

@@ -61,7 +61,7 @@ static PassArgs batch_pass(const ocr_batch* b) {
   a.pix_slots = b->pix_slots;
   a.tiles = b->tiles;
   a.tile_w_log2 = b->tune.shade_tile_w_log2;
-  a.lists = Lists{b->cuts, b->counters, b->cap_cut};
+  a.lists = make_lists(b->cuts, b->counters, b->cap_cut);
   return a;
 }
 
@@ -95,13 +95,22 @@ static int ensure_lists(ocr_batch* b) {
   if (b->cuts) (void)hipFree(b->cuts);
   b->cuts = nullptr;
   b->cap_cut = 0;
-  hipError_t e = hipMalloc((void**)&b->cuts, sizeof(CutTri) * want_cut);
+  hipError_t e = hipMalloc((void**)&b->cuts, sizeof(CutTri) * want_cut * 2);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return fail(OCR_ERR_OOM, std::string("batch cut list: ") + hipGetErrorString(e));
   }
   b->cap_cut = (uint32_t)want_cut;
   return OCR_OK;
+}
+
+int ocr::batch_reserve(ocr_batch* b, uint64_t sum_tris) {
+  OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
+  const uint64_t keep = b->sum_tris;
+  b->sum_tris = sum_tris;
+  const int rc = ensure_lists(b);
+  b->sum_tris = keep;
+  return rc;
 }
 
 static hipError_t enqueue_batch(ocr_batch* b, hipStream_t st) {

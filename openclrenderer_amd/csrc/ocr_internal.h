@@ -50,7 +50,7 @@ static_assert(sizeof(ObjDesc) == 64, "ObjDesc layout");
 // three vertices, its 1/det constant, its sort key (2*tri + sub for a camera
 // view, cube face for a shadow map), and the range of raster samples its
 // fragments cover.  The list is sorted by sample_begin: prearrange allocates
-// records and sample ranges together with one 64-bit atomic per wavefront.
+// records and sample ranges together, once per block.
 struct alignas(16) CutTri {
   float x[3];
   float y[3];
@@ -64,6 +64,23 @@ struct alignas(16) CutTri {
   uint32_t _pad;
 };
 static_assert(sizeof(CutTri) == 64, "CutTri layout");
+
+// The raster setup of a cut triangle, computed once by prearrange and read
+// by every raster pass: bounding-box origin and width, 1/width, the 1/z plane
+// (cl2.cl:5044) and point_in_tri's loop-invariant terms with the edge sign
+// folded in (negation is exact, so a*sign + (b*sign)*x + (c*sign)*y has the
+// bits of (a + b*x + c*y) * sign).
+struct alignas(16) RasterTri {
+  float min_x;
+  uint32_t key;
+  float min_y;
+  float w;          // max_x - min_x
+  float iw;         // 1 / w
+  float A, B, C;    // 1/z plane
+  float ek[7];      // s0, s1, s2, t0, t1, t2 (signed), 2.0001 * |area|
+  uint32_t sample_begin;
+};
+static_assert(sizeof(RasterTri) == 64, "RasterTri layout");
 
 // Everything a kernel needs to know about one view.
 struct ViewParams {
@@ -105,12 +122,18 @@ struct DevView {
   uint32_t* normals;
 };
 
-// The cut list and counters shared by the views of one render.
+// The cut list and counters shared by the views of one render.  One
+// allocation of 2 * cap_cut records holds the CutTri records followed by
+// their RasterTri setups.
 struct Lists {
   CutTri* cuts;
+  RasterTri* rast;
   uint32_t* counters;
   uint32_t cap_cut;
 };
+inline Lists make_lists(CutTri* cuts, uint32_t* counters, uint32_t cap_cut) {
+  return Lists{cuts, reinterpret_cast<RasterTri*>(cuts + cap_cut), counters, cap_cut};
+}
 constexpr uint32_t kMaxViews = 4096;
 
 // Everything the camera passes need.  A single view travels inline in the
@@ -143,13 +166,12 @@ enum Counter : int {
 // Bits of the overflow word: the cut list is full / the sample space is full.
 constexpr uint32_t kOverflowCuts = 1u, kOverflowSamples = 2u;
 // Raster samples per render, and per cut triangle.  Frames and shadow faces
-// hold at most 2^24 pixels, so a triangle has at most 2^24 + 2^16 samples; a
-// wavefront then adds less than 2^31 to the low half of the packed counter,
-// which therefore reaches the overflow threshold before it can carry into the
-// high half.
+// hold at most 16,000,000 pixels, so a triangle has at most
+// area * (1 + 1/op) + op + 1 < 2^24 samples: sample indices, bounding-box
+// products and row starts are exact in binary32.
 constexpr uint32_t kMaxSamples = 1u << 30;
-constexpr uint32_t kMaxTriSamples = (1u << 24) + (1u << 16);
-constexpr uint64_t kMaxTargetPixels = 1ull << 24;
+constexpr uint32_t kMaxTriSamples = (1u << 24) - 1;
+constexpr uint64_t kMaxTargetPixels = 16000000ull;
 
 struct Ctx {
   int device = 0;
@@ -225,6 +247,8 @@ hipError_t launch_shade_count(const PassArgs& a, const SinCos face[6], uint32_t*
                               uint32_t* shadow_bits, uint32_t* static_bits, hipStream_t s);
 // helpers shared by the plain and batched renders (runtime.hip)
 DevScene dev_scene(const Scene* sc);
+// Sizes a batch's cut list for `sum_tris` scene triangles ahead of use.
+int batch_reserve(ocr_batch* b, uint64_t sum_tris);
 DevView dev_view(const ocr_view* v, uint32_t scene, Frame* f);
 int device_cu_count();
 hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, uint32_t* map,

@@ -55,10 +55,11 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fm
 __device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(fminf(a, b), c); }
 __device__ __forceinline__ float max3f(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
 
+// convert_uint_sat: NaN and f <= 0 give 0, f >= 2^32 gives UINT_MAX, others
+// truncate (4294967040 is the largest binary32 below 2^32).
 __device__ __forceinline__ uint32_t sat_u32(float f) {
-  if (!(f > 0.0f)) return 0u;
-  if (f >= 4294967296.0f) return 0xFFFFFFFFu;
-  return (uint32_t)f;
+  const uint32_t v = (uint32_t)fminf(fmaxf(f, 0.0f), 4294967040.0f);
+  return f >= 4294967296.0f ? 0xFFFFFFFFu : v;
 }
 
 // exp in binary64: Cody-Waite reduction by ln2 + degree-13 Taylor polynomial.

@@ -100,51 +100,65 @@ __device__ __forceinline__ bool block_alloc(const Lists& L, BlockAllocLds& sh, u
 }
 
 // ---------------------------------------------------------- setup helpers
-struct Setup {
-  float mm[4];        // min_x, max_x, min_y, max_y
-  int width;
-  float A, B, C;      // 1/z plane, cl2.cl:5044
-  float iwidth;
-  float xp[3], yp[3]; // rounded vertex coordinates
-};
-
-__device__ __forceinline__ void raster_setup(const CutTri& c, float ew, float eh, Setup& s) {
-  calc_min_max(c.x, c.y, ew, eh, s.mm);
-  s.width = (int)(s.mm[1] - s.mm[0]);
-#pragma unroll
-  for (int i = 0; i < 3; i++) { s.xp[i] = roundf(c.x[i]); s.yp[i] = roundf(c.y[i]); }
-  f3 depths = mk3(1.0f / (c.z[0] / kDepthFar), 1.0f / (c.z[1] / kDepthFar), 1.0f / (c.z[2] / kDepthFar));
-  plane_const(depths, mk3(s.xp[0], s.xp[1], s.xp[2]), mk3(s.yp[0], s.yp[1], s.yp[2]), c.rconst, s.A, s.B, s.C);
-  s.iwidth = 1.f / (float)s.width;
-}
-
 // The walk's row of sample q: floor(mad(q, 1/width, min_y)) (cl2.cl:5083).
-__device__ __forceinline__ float row_of(int q, float iw, float min_y) {
-  return floorf(fmaf((float)q, iw, min_y));
+// Sample indices are integers below 2^24, exact in binary32.
+__device__ __forceinline__ float row_of(float q, float iw, float min_y) {
+  return floorf(fmaf(q, iw, min_y));
 }
 
-// First q >= 0 with row_of(q) >= Y, stepping from the nominal (Y - min_y) * w.
-__device__ __forceinline__ int run_start(float Y, int w, float iw, float min_y) {
-  long long n0 = (long long)(Y - min_y) * (long long)w;
-  int q = n0 < 0 ? 0 : (int)n0;
-  for (int g = 0; g < 64 && q > 0 && row_of(q - 1, iw, min_y) >= Y; g++) q--;
-  for (int g = 0; g < 64 && row_of(q, iw, min_y) < Y; g++) q++;
+// First sample q >= 0 whose row is >= Y, from the nominal n0 = (Y - min_y) * w
+// (exact: an integer below 2^24).  The fma rounds across a row boundary for at
+// most a few samples, so the steps below run only on such rows.
+__device__ __forceinline__ float run_start_steps(float Y, float q, float iw, float min_y) {
+#pragma clang loop unroll(disable)
+  for (int g = 0; g < 64 && q > 0.f && row_of(q - 1.f, iw, min_y) >= Y; g++) q -= 1.f;
+#pragma clang loop unroll(disable)
+  for (int g = 0; g < 64 && row_of(q, iw, min_y) < Y; g++) q += 1.f;
   return q;
+}
+
+__device__ __forceinline__ float run_start(float Y, float w, float iw, float min_y) {
+  const float n0 = (Y - min_y) * w;
+  const bool lead = n0 > 0.f && row_of(n0 - 1.f, iw, min_y) >= Y;
+  const bool lag = row_of(n0, iw, min_y) < Y;
+  return (lead || lag) ? run_start_steps(Y, n0, iw, min_y) : n0;
+}
+
+// Setup of one cut triangle for the raster passes (kernel1 cl2.cl:5000-5046).
+__device__ __forceinline__ void raster_setup(const f3 p[3], float rc, float ew, float eh, RasterTri& t,
+                                             float& max_y) {
+  const float xs[3] = {p[0].x, p[1].x, p[2].x}, ys[3] = {p[0].y, p[1].y, p[2].y};
+  float mm[4];
+  calc_min_max(xs, ys, ew, eh, mm);
+  t.min_x = mm[0];
+  t.min_y = mm[2];
+  max_y = mm[3];
+  t.w = mm[1] - mm[0];
+  t.iw = 1.f / (float)(int)t.w;
+  const float xp[3] = {roundf(p[0].x), roundf(p[1].x), roundf(p[2].x)};
+  const float yp[3] = {roundf(p[0].y), roundf(p[1].y), roundf(p[2].y)};
+  const f3 depths = mk3(1.0f / (p[0].z / kDepthFar), 1.0f / (p[1].z / kDepthFar), 1.0f / (p[2].z / kDepthFar));
+  plane_const(depths, mk3(xp[0], xp[1], xp[2]), mk3(yp[0], yp[1], yp[2]), rc, t.A, t.B, t.C);
+  // point_in_tri (cl2.cl:4798-4807), loop-invariant terms in the same order
+  const float p0x = xp[0], p0y = yp[0], p1x = xp[1], p1y = yp[1], p2x = xp[2], p2y = yp[2];
+  const float A = 0.5f * (-p1y * p2x + p0y * (-p1x + p2x) + p0x * (p1y - p2y) + p1x * p2y);
+  const float sign = A < 0 ? -1.f : 1.f;
+  t.ek[0] = (p0y * p2x - p0x * p2y) * sign;
+  t.ek[1] = (p2y - p0y) * sign;
+  t.ek[2] = (p0x - p2x) * sign;
+  t.ek[3] = (p0x * p1y - p0y * p1x) * sign;
+  t.ek[4] = (p0y - p1y) * sign;
+  t.ek[5] = (p1x - p0x) * sign;
+  t.ek[6] = 2.0001f * A * sign;
 }
 
 // Samples of all fragments of a cut triangle.  Fragment a visits
 // q in [op*a, op*a + op] up to the first q whose row is >= max_y (q_end), so
 // it holds clamp(q_end - op*a, 0, op + 1) samples; the full fragments come
 // first, then at most one partial one.
-__device__ __forceinline__ uint32_t triangle_samples(const f3 p[3], float ew, float eh, int op, uint32_t nfrag) {
-  if (nfrag == 0) return 0;
-  const float xs[3] = {p[0].x, p[1].x, p[2].x}, ys[3] = {p[0].y, p[1].y, p[2].y};
-  float mm[4];
-  calc_min_max(xs, ys, ew, eh, mm);
-  const int w = (int)(mm[1] - mm[0]);
-  if (w <= 0) return 0;
-  const float iw = 1.f / (float)w;
-  const long long q_end = run_start(mm[3], w, iw, mm[2]);
+__device__ __forceinline__ uint32_t triangle_samples(const RasterTri& t, float max_y, int op, uint32_t nfrag) {
+  if (nfrag == 0 || !(t.w > 0.f)) return 0;
+  const long long q_end = (long long)run_start(max_y, t.w, t.iw, t.min_y);
   long long m = q_end >= op + 1 ? (q_end - (op + 1)) / op + 1 : 0;
   if (m > (long long)nfrag) m = nfrag;
   long long s = (long long)(op + 1) * m;
@@ -222,17 +236,36 @@ __device__ __forceinline__ int transform_clip_project(const TriPos& T, const Obj
 // overflow word, which every later pass of the render checks before reading
 // the list.
 __device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds& sh, const f3 tp[2][3],
-                                            const float rc[2], const uint32_t nf[2], const uint32_t ns[2],
-                                            uint32_t key0, uint32_t key_step, uint32_t oid, uint32_t view) {
+                                            const float rc[2], RasterTri rt[2], const uint32_t nf[2],
+                                            const uint32_t ns[2], uint32_t key0, uint32_t key_step, uint32_t oid,
+                                            uint32_t view) {
   const uint32_t k0 = ns[0] > 0, k1 = ns[1] > 0;
   uint32_t c, smp;
   if (!block_alloc(L, sh, k0 + k1, ns[0] + ns[1], nf[0] * k0 + nf[1] * k1, c, smp)) return;
 #pragma unroll
   for (int i = 0; i < 2; i++) {
     if (!ns[i]) continue;
-    store_cut(&L.cuts[c], tp[i], rc[i], key0 + key_step * (uint32_t)i, oid, smp, ns[i], view);
+    const uint32_t key = key0 + key_step * (uint32_t)i;
+    store_cut(&L.cuts[c], tp[i], rc[i], key, oid, smp, ns[i], view);
+    rt[i].key = key;
+    rt[i].sample_begin = smp;
+    L.rast[c] = rt[i];
     smp += ns[i];
     c++;
+  }
+}
+
+// Cull, count and set up the (up to 2) clipped pieces of one triangle.
+__device__ __forceinline__ void cut_pieces(const f3 tp[2][3], int num, float ew, float eh, bool two_sided, int op,
+                                           float rc[2], RasterTri rt[2], uint32_t nf[2], uint32_t ns[2]) {
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    if (i >= num) continue;
+    nf[i] = cull_count(tp[i], ew, eh, two_sided, op, rc[i]);
+    if (!nf[i]) continue;
+    float max_y;
+    raster_setup(tp[i], rc[i], ew, eh, rt[i], max_y);
+    ns[i] = triangle_samples(rt[i], max_y, op, nf[i]);
   }
 }
 
@@ -307,6 +340,7 @@ __global__ void __launch_bounds__(kPreThreads) k_prearrange(PassArgs a) {
   const float ew = (float)v.width, eh = (float)v.height;
   f3 tp[2][3];
   float rc[2] = {0.f, 0.f};
+  RasterTri rt[2];
   uint32_t nf[2] = {0u, 0u}, ns[2] = {0u, 0u};
   uint32_t oid = 0;
   if (id < S.ntri) {
@@ -316,18 +350,12 @@ __global__ void __launch_bounds__(kPreThreads) k_prearrange(PassArgs a) {
       const ObjDesc G = S.objs[oid];
       f3 cpos = mk3(v.cpos[0], v.cpos[1], v.cpos[2]);
       if (!(fast_length(mk3(G.wp[0], G.wp[1], G.wp[2]) - cpos) > kDepthFar)) {
-        int num = transform_clip_project(T, G, cpos, v.cam, v.fov, ew, eh, tp);
-        bool two_sided = (G.flags & OCR_FEATURE_TWO_SIDED) > 0;
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-          if (i < num) {
-            nf[i] = cull_count(tp[i], ew, eh, two_sided, kOpSize, rc[i]);
-            ns[i] = triangle_samples(tp[i], ew, eh, kOpSize, nf[i]);
-          }
+        const int num = transform_clip_project(T, G, cpos, v.cam, v.fov, ew, eh, tp);
+        cut_pieces(tp, num, ew, eh, (G.flags & OCR_FEATURE_TWO_SIDED) > 0, kOpSize, rc, rt, nf, ns);
       }
     }
   }
-  append_cuts(a.lists, sh, tp, rc, nf, ns, 2u * id, 1u, oid, vi);
+  append_cuts(a.lists, sh, tp, rc, rt, nf, ns, 2u * id, 1u, oid, vi);
 }
 
 hipError_t launch_prearrange(const PassArgs& a, hipStream_t s) {
@@ -358,43 +386,30 @@ struct ShadowTarget {
   uint32_t L;
 };
 
-// Setup of 64 staged triangles, one column per lane.  point_in_tri's
-// loop-invariant sub-expressions are kept with the edge sign folded in
-// (negation is exact, so (a + b*x + c*y) * sign has the same bits as
-// a*sign + (b*sign)*x + (c*sign)*y).
-template <RasterMode kMode>
+// Setups of 64 staged triangles, one column per lane.  The target buffer
+// travels in the kernel arguments unless views differ per triangle (batch).
+template <RasterMode kMode, bool kBatch>
 struct TriLds {
-  float min_x[kWave], max_x[kWave], min_y[kWave];
-  float pa[kWave], pb[kWave], pc[kWave], iw[kWave];   // 1/z plane, 1/width
-  float ek[7][kWave];   // s0, s1, s2, t0, t1, t2 (signed), 2.0001 * |area|
-  int w[kWave];
+  float min_x[kWave], min_y[kWave], w[kWave], iw[kWave];
+  float pa[kWave], pb[kWave], pc[kWave];   // 1/z plane
+  float ek[7][kWave];
   uint32_t key[kWave];
   uint32_t begin[kWave + 1];
-  uint32_t pitch[kMode == RasterMode::Shadow ? 1 : kWave];
-  uint32_t* dst[kWave];                                // depth buffer / cube face
-  unsigned long long* ids[kMode == RasterMode::Id ? kWave : 1];
+  uint32_t pitch[kBatch ? kWave : 1];
+  uint32_t* dst[kBatch || kMode == RasterMode::Shadow ? kWave : 1];
+  unsigned long long* ids[kBatch && kMode == RasterMode::Id ? kWave : 1];
 };
 
-// x mod w for 0 <= x < 2^25, 1 <= w: a float quotient estimate (|error| <= 1
-// for these ranges) corrected by one step each way.
-__device__ __forceinline__ int mod_small(int x, int w, float invw) {
-  const int qq = (int)((float)x * invw);
-  int r = x - qq * w;
-  if (r < 0) r += w;
-  if (r >= w) r -= w;
-  return r;
-}
-
-template <RasterMode kMode, int kOp>
+template <RasterMode kMode, int kOp, bool kBatch>
 __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
-  __shared__ TriLds<kMode> lds_all[256 / kWave];
+  __shared__ TriLds<kMode, kBatch> lds_all[256 / kWave];
   const Lists& Ls = a.lists;
   if (Ls.counters[kOverflow]) return;
   const unsigned long long packed = *reinterpret_cast<const unsigned long long*>(Ls.counters + kPacked);
   const uint32_t ncut = (uint32_t)(packed >> 32);
   const uint32_t total = (uint32_t)packed;
   const int lane = threadIdx.x & (kWave - 1);
-  TriLds<kMode>& L = lds_all[threadIdx.x / kWave];
+  TriLds<kMode, kBatch>& L = lds_all[threadIdx.x / kWave];
   const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
   for (uint32_t s0 = wave0 * kChunk; s0 < total; s0 += nwaves * kChunk) {
@@ -404,7 +419,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
     uint32_t lo = 0, hi = ncut;
     while (hi - lo > 1) {
       const uint32_t p = lo + (uint32_t)(((unsigned long long)(hi - lo) * (uint32_t)lane) / kWave);
-      const unsigned long long m = __ballot(Ls.cuts[p].sample_begin <= s0);
+      const unsigned long long m = __ballot(Ls.rast[p].sample_begin <= s0);
       const int t = 63 - __clzll((long long)m);
       const uint32_t plo = __shfl(p, t, kWave);
       const uint32_t phi = __shfl(p, t < kWave - 1 ? t + 1 : t, kWave);
@@ -417,42 +432,27 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
       // ---- stage triangles c0 .. c0 + 63, one per lane
       const uint32_t ci = c0 + lane;
       if (ci < ncut) {
-        const CutTri c = Ls.cuts[ci];
-        uint32_t W, H;
+        const RasterTri t = Ls.rast[ci];
+        L.min_x[lane] = t.min_x;
+        L.min_y[lane] = t.min_y;
+        L.w[lane] = t.w;
+        L.iw[lane] = t.iw;
+        L.pa[lane] = t.A;
+        L.pb[lane] = t.B;
+        L.pc[lane] = t.C;
+#pragma unroll
+        for (int e = 0; e < 7; e++) L.ek[e][lane] = t.ek[e];
+        L.key[lane] = t.key;
+        L.begin[lane] = t.sample_begin;
         if constexpr (kMode == RasterMode::Shadow) {
-          W = H = sh.L;
-          L.dst[lane] = sh.map + (size_t)c.key * sh.L * sh.L;
-        } else {
-          const DevView& V = view_at(a, c.view);
-          W = V.v.width;
-          H = V.v.height;
+          L.dst[lane] = sh.map + (size_t)t.key * sh.L * sh.L;
+        } else if constexpr (kBatch) {
+          const DevView& V = a.views[Ls.cuts[ci].view];
           L.dst[lane] = V.depth;
-          L.pitch[lane] = W;
+          L.pitch[lane] = V.v.width;
           if constexpr (kMode == RasterMode::Id) L.ids[lane] = V.idkey;
         }
-        Setup st;
-        raster_setup(c, (float)W, (float)H, st);
-        L.min_x[lane] = st.mm[0];
-        L.max_x[lane] = st.mm[1];
-        L.min_y[lane] = st.mm[2];
-        L.pa[lane] = st.A;
-        L.pb[lane] = st.B;
-        L.pc[lane] = st.C;
-        L.iw[lane] = st.iwidth;
-        const float p0x = st.xp[0], p0y = st.yp[0], p1x = st.xp[1], p1y = st.yp[1], p2x = st.xp[2], p2y = st.yp[2];
-        const float A = 0.5f * (-p1y * p2x + p0y * (-p1x + p2x) + p0x * (p1y - p2y) + p1x * p2y);
-        const float sign = A < 0 ? -1.f : 1.f;
-        L.ek[0][lane] = (p0y * p2x - p0x * p2y) * sign;
-        L.ek[1][lane] = (p2y - p0y) * sign;
-        L.ek[2][lane] = (p0x - p2x) * sign;
-        L.ek[3][lane] = (p0x * p1y - p0y * p1x) * sign;
-        L.ek[4][lane] = (p0y - p1y) * sign;
-        L.ek[5][lane] = (p1x - p0x) * sign;
-        L.ek[6][lane] = 2.0001f * A * sign;
-        L.w[lane] = st.width;
-        L.key[lane] = c.key;
-        L.begin[lane] = c.sample_begin;
-        if (lane == kWave - 1) L.begin[kWave] = c.sample_begin + c.nsamples;
+        if (lane == kWave - 1) L.begin[kWave] = ci + 1 < ncut ? Ls.rast[ci + 1].sample_begin : total;
       } else {
         L.begin[lane] = total;
         if (lane == kWave - 1) L.begin[kWave] = total;
@@ -467,30 +467,48 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
 #pragma unroll
         for (int step = 32; step >= 1; step >>= 1)
           if (L.begin[k + step] <= j) k += step;
+        // sample -> (fragment a, walk index q); pa = op * a
         const uint32_t sl = j - L.begin[k];
         const uint32_t fa = sl / (uint32_t)(kOp + 1);
-        const int pa = kOp * (int)fa;
-        const int q = pa + (int)(sl - fa * (uint32_t)(kOp + 1));
-        const int w = L.w[k];
-        const float min_y = L.min_y[k], iw = L.iw[k];
-        const float y = row_of(q, iw, min_y);
-        int r = run_start(y, w, iw, min_y);
-        r = r > pa ? r : pa;
-        const float x = (float)mod_small(r, w, iw) + L.min_x[k] + (float)(q - r);
-        if (x >= L.max_x[k]) continue;
+        const uint32_t o = sl - fa * (uint32_t)(kOp + 1);
+        const float paf = (float)(fa * (uint32_t)kOp);
+        const float qf = paf + (float)o;
+        const float w = L.w[k], iw = L.iw[k], min_y = L.min_y[k];
+        const float y = row_of(qf, iw, min_y);
+        // cursor column: (r % w) + min_x + (q - r), r = max(pa, row start)
+        const float r = fmaxf(paf, run_start(y, w, iw, min_y));
+        const float m = floorf(r * iw);
+        float e = fmaf(-m, w, r);
+        e = e < 0.f ? e + w : e;
+        e = e >= w ? e - w : e;
+        const float xo = e + (qf - r);
+        if (xo >= w) continue;   // x >= max_x
+        const float x = L.min_x[k] + xo;
         const float s_ = L.ek[0][k] + L.ek[1][k] * x + L.ek[2][k] * y;
         const float t_ = L.ek[3][k] + L.ek[4][k] * x + L.ek[5][k] * y;
         if (!(s_ > -0.0001f && t_ > -0.0001f && (s_ + t_) < L.ek[6][k])) continue;
         const float fd = fmaf(L.pa[k], x, fmaf(L.pb[k], y, L.pc[k]));
         const uint32_t d = sat_u32(kMulint / fd);
-        const uint32_t pitch = kMode == RasterMode::Shadow ? sh.L : L.pitch[kMode == RasterMode::Shadow ? 0 : k];
-        const size_t pix = (size_t)(int)y * pitch + (size_t)(int)x;
-        if constexpr (kMode == RasterMode::Id) {
-          const uint32_t val = L.dst[k][pix];
-          if (d > val - kBufError && d < val + kBufError)
-            atomicMax(&L.ids[k][pix], ((unsigned long long)(L.key[k] + 1u) << 32) | (c0 + (uint32_t)k));
+        uint32_t pitch;
+        uint32_t* dst;
+        if constexpr (kMode == RasterMode::Shadow) {
+          pitch = sh.L;
+          dst = L.dst[k];
+        } else if constexpr (kBatch) {
+          pitch = L.pitch[k];
+          dst = L.dst[k];
         } else {
-          atomicMin(&L.dst[k][pix], d);
+          pitch = a.view0.v.width;
+          dst = a.view0.depth;
+        }
+        const size_t pix = (size_t)((uint32_t)(int)y * pitch + (uint32_t)(int)x);
+        if constexpr (kMode == RasterMode::Id) {
+          unsigned long long* ids = kBatch ? L.ids[kBatch ? k : 0] : a.view0.idkey;
+          const uint32_t val = dst[pix];
+          if (d > val - kBufError && d < val + kBufError)
+            atomicMax(&ids[pix], ((unsigned long long)(L.key[k] + 1u) << 32) | (c0 + (uint32_t)k));
+        } else {
+          atomicMin(&dst[pix], d);
         }
       }
       __builtin_amdgcn_wave_barrier();
@@ -502,19 +520,25 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
 
 int device_cu_count();
 
-hipError_t launch_depth(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
-  hipLaunchKernelGGL((k_raster<RasterMode::Depth, kOpSize>), dim3(device_cu_count() * blocks_per_cu), dim3(256), 0,
-                     s, a, ShadowTarget{nullptr, 0});
+template <RasterMode kMode>
+static hipError_t launch_camera_raster(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
+  const dim3 grid(device_cu_count() * blocks_per_cu), block(256);
+  if (a.views)
+    hipLaunchKernelGGL((k_raster<kMode, kOpSize, true>), grid, block, 0, s, a, ShadowTarget{nullptr, 0});
+  else
+    hipLaunchKernelGGL((k_raster<kMode, kOpSize, false>), grid, block, 0, s, a, ShadowTarget{nullptr, 0});
   return hipGetLastError();
+}
+
+hipError_t launch_depth(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
+  return launch_camera_raster<RasterMode::Depth>(a, blocks_per_cu, s);
 }
 
 // kernel2's "last writer wins" is made deterministic: among fragments inside
 // the BUF_ERROR window the pixel keeps the largest (2*tri + sub) key, which
 // is the fragment a sequential in-order execution writes last.
 hipError_t launch_id(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
-  hipLaunchKernelGGL((k_raster<RasterMode::Id, kOpSize>), dim3(device_cu_count() * blocks_per_cu), dim3(256), 0,
-                     s, a, ShadowTarget{nullptr, 0});
-  return hipGetLastError();
+  return launch_camera_raster<RasterMode::Id>(a, blocks_per_cu, s);
 }
 
 // ---------------------------------------------------------------- shadows
@@ -564,17 +588,13 @@ __global__ void __launch_bounds__(kPreThreads) k_prearrange_shadow(const TriPos*
     if (!((any_touched >> kk) & 1)) continue;
     f3 tp[2][3];
     float rc[2] = {0.f, 0.f};
+    RasterTri rt[2];
     uint32_t nf[2] = {0u, 0u}, ns[2] = {0u, 0u};
     if ((touched >> kk) & 1) {
-      int num = transform_clip_project(T, G, lpos, ft.f[kk], efov, ew, eh, tp);
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-        if (i < num) {
-          nf[i] = cull_count(tp[i], ew, eh, two_sided, kOpSizeLight, rc[i]);
-          ns[i] = triangle_samples(tp[i], ew, eh, kOpSizeLight, nf[i]);
-        }
+      const int num = transform_clip_project(T, G, lpos, ft.f[kk], efov, ew, eh, tp);
+      cut_pieces(tp, num, ew, eh, two_sided, kOpSizeLight, rc, rt, nf, ns);
     }
-    append_cuts(lists, sh, tp, rc, nf, ns, (uint32_t)kk, 0u, T.oid, 0u);
+    append_cuts(lists, sh, tp, rc, rt, nf, ns, (uint32_t)kk, 0u, T.oid, 0u);
   }
 }
 
@@ -586,7 +606,7 @@ hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, 
                                hipStream_t s) {
   FaceTable ft;
   for (int i = 0; i < 6; i++) ft.f[i] = sc->ctx->face_sc[i];
-  const Lists lists{sc->sh_cuts, sc->sh_counters, sc->sh_cap_cut};
+  const Lists lists = make_lists(sc->sh_cuts, sc->sh_counters, sc->sh_cap_cut);
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, sc->sh_counters);
   if (sc->ntri)
     hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + kPreThreads - 1) / kPreThreads), dim3(kPreThreads), 0, s, sc->tri_pos,
@@ -595,7 +615,7 @@ hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, 
   PassArgs a;
   memset(&a, 0, sizeof(a));
   a.lists = lists;
-  hipLaunchKernelGGL((k_raster<RasterMode::Shadow, kOpSizeLight>), dim3(device_cu_count() * 8), dim3(256), 0, s,
+  hipLaunchKernelGGL((k_raster<RasterMode::Shadow, kOpSizeLight, false>), dim3(device_cu_count() * 8), dim3(256), 0, s,
                      a, ShadowTarget{map, sc->lbuf_dim});
   return hipGetLastError();
 }

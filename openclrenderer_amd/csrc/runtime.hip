@@ -128,7 +128,7 @@ static PassArgs single_pass(const Scene* s, const ocr_view* view, Frame* f) {
   a.pix_slots = round256((uint64_t)view->width * view->height);
   a.tile_w_log2 = 4;
   a.tiles = view_tiles(view->width, view->height, a.tile_w_log2);
-  a.lists = Lists{f->cuts, f->counters, f->cap_cut};
+  a.lists = make_lists(f->cuts, f->counters, f->cap_cut);
   return a;
 }
 
@@ -394,7 +394,7 @@ int ocr_scene_read_atlas(ocr_scene* s, uint8_t* pages, uint32_t* nums, uint32_t*
 int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_t light_buffer_dim) {
   if (!s || (n && !lights)) return fail(OCR_ERR_INVALID, "ocr_lights_upload: bad argument");
   if (light_buffer_dim < 8 || (uint64_t)light_buffer_dim * light_buffer_dim > kMaxTargetPixels)
-    return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light buffer dimension must be 8..4096");
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light buffer dimension must be 8..4000");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   dev_free(s->lights);
   dev_free(s->shadow_maps);
@@ -428,7 +428,7 @@ static int ensure_shadow_lists(ocr_scene* s, uint32_t cap_cut) {
   if (s->sh_cuts && s->sh_cap_cut >= cap_cut) return OCR_OK;
   dev_free(s->sh_cuts);
   s->sh_cap_cut = 0;
-  int rc = dev_alloc(&s->sh_cuts, cap_cut, "shadow cut list");
+  int rc = dev_alloc(&s->sh_cuts, (size_t)cap_cut * 2, "shadow cut list");
   if (rc) return rc;
   s->sh_cap_cut = cap_cut;
   return OCR_OK;
@@ -487,7 +487,7 @@ int ocr_scene_read_shadow_maps(ocr_scene* s, uint32_t* dynamic_maps, uint32_t* s
 int ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** out) {
   if (!ctx || !out || width < 4 || height < 4) return fail(OCR_ERR_INVALID, "ocr_frame_create: bad argument (frames are at least 4x4)");
   if ((uint64_t)width * height > kMaxTargetPixels || width > 32768 || height > 32768)
-    return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_create: frame too large (at most 2^24 pixels)");
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_create: frame too large (at most 16,000,000 pixels)");
   *out = nullptr;
   OCR_HIP_CHECK(hipSetDevice(ctx->device));
   ocr_frame* f = new (std::nothrow) ocr_frame();
@@ -559,7 +559,7 @@ static int ensure_frame_lists(ocr_frame* f, uint32_t ntri) {
   OCR_HIP_CHECK(hipDeviceSynchronize());
   dev_free(f->cuts);
   f->cap_cut = 0;
-  int rc = dev_alloc(&f->cuts, want, "cut list");
+  int rc = dev_alloc(&f->cuts, (size_t)want * 2, "cut list");
   if (rc) return rc;
   f->cap_cut = (uint32_t)want;
   return OCR_OK;

@@ -25,6 +25,8 @@ EXPORTS = [
     "ocr_batch_create", "ocr_batch_destroy", "ocr_batch_set_views", "ocr_batch_set_tuning",
     "ocr_batch_get_tuning", "ocr_batch_set_profiling", "ocr_batch_render", "ocr_batch_launch_graph",
     "ocr_batch_get_stats", "ocr_batch_autotune",
+    "ocr_server_create", "ocr_server_add_scene", "ocr_server_submit", "ocr_server_poll",
+    "ocr_server_release", "ocr_server_get_info", "ocr_server_now_ns", "ocr_server_destroy",
 ]
 
 
@@ -49,6 +51,26 @@ class FrameStats(C.Structure):
                 ("ms_depth", C.c_float), ("ms_id", C.c_float), ("ms_shade", C.c_float),
                 ("ms_total", C.c_float)]
 
+
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch_views", C.c_uint32), ("max_inflight", C.c_uint32),
+                ("max_batch_tris", C.c_uint64), ("hbm_fraction", C.c_float), ("read_mask", C.c_uint32)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("t_submit_ns", C.c_uint64), ("t_start_ns", C.c_uint64),
+                ("t_done_ns", C.c_uint64), ("host", C.c_void_p), ("status", C.c_int32),
+                ("scene_slot", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("batch_views", C.c_uint32), ("_pad", C.c_uint32)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("hbm_held", C.c_uint64), ("n_batches", C.c_uint64),
+                ("n_done", C.c_uint64), ("n_refused", C.c_uint64), ("n_pending", C.c_uint32),
+                ("n_frames", C.c_uint32)]
+
+
+READ_RGBA, READ_DEPTH, READ_IDKEY = 1, 2, 4
 
 _lib = None
 
@@ -99,6 +121,16 @@ def load_library(path=LIB_PATH):
     L.ocr_batch_launch_graph.argtypes = [vp, vp]
     L.ocr_batch_get_stats.argtypes = [vp, vp, C.POINTER(FrameStats)]
     L.ocr_batch_autotune.argtypes = [vp, vp, u32, C.POINTER(BatchTuning)]
+    L.ocr_server_create.argtypes = [vp, C.POINTER(ServerConfig), pp]
+    L.ocr_server_add_scene.argtypes = [vp, vp, C.POINTER(C.c_uint32)]
+    L.ocr_server_submit.argtypes = [vp, u32, vp, C.POINTER(C.c_uint64)]
+    L.ocr_server_poll.argtypes = [vp, C.POINTER(Completion), u32, C.POINTER(C.c_uint32), i32]
+    L.ocr_server_release.argtypes = [vp, C.c_uint64]
+    L.ocr_server_get_info.argtypes = [vp, C.POINTER(ServerInfo)]
+    L.ocr_server_now_ns.argtypes = []
+    L.ocr_server_now_ns.restype = C.c_uint64
+    L.ocr_server_destroy.argtypes = [vp]
+    L.ocr_server_destroy.restype = None
     _lib = L
     return L
 
@@ -321,4 +353,81 @@ class Batch:
     def close(self):
         if self.h:
             self.lib.ocr_batch_destroy(self.h)
+            self.h = None
+
+
+def id_keys_to_tri(keys, depth):
+    """The triangle-id image from raw id keys and depth, as ocr_frame_read
+    converts it (background and empty keys give OCR_NO_TRIANGLE)."""
+    keys = np.asarray(keys, dtype=np.uint64)
+    tri = (((keys >> np.uint64(32)) - np.uint64(1)) >> np.uint64(1)).astype(np.uint32)
+    empty = (keys == 0) | (np.asarray(depth) == 0xFFFFFFFF)
+    tri[empty] = 0xFFFFFFFF
+    return tri
+
+
+class Server:
+    """The served form: resident scenes, admission against an HBM budget,
+    continuous batching over `max_inflight` in-flight batch slots."""
+
+    def __init__(self, ctx, max_batch_views=32, max_inflight=2, max_batch_tris=16_000_000,
+                 hbm_fraction=0.5, read_mask=READ_RGBA):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.read_mask = read_mask
+        cfg = ServerConfig(max_batch_views, max_inflight, max_batch_tris, hbm_fraction, read_mask)
+        h = C.c_void_p()
+        check(self.lib.ocr_server_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.scenes = []
+
+    def add_scene(self, gscene):
+        slot = C.c_uint32()
+        check(self.lib.ocr_server_add_scene(self.h, gscene.h, C.byref(slot)))
+        self.scenes.append(gscene)
+        return slot.value
+
+    def submit(self, slot, view):
+        """Returns the request id, or None when admission refuses it (busy)."""
+        v = np.ascontiguousarray(np.asarray(view, dtype=abi.VIEW).reshape(1))
+        rid = C.c_uint64()
+        rc = self.lib.ocr_server_submit(self.h, slot, _p(v), C.byref(rid))
+        if rc == abi.OCR_ERR_BUSY:
+            return None
+        check(rc)
+        return rid.value
+
+    def poll(self, max_n=256, wait_ms=0):
+        buf = (Completion * max_n)()
+        n = C.c_uint32()
+        check(self.lib.ocr_server_poll(self.h, buf, max_n, C.byref(n), wait_ms))
+        return [buf[i] for i in range(n.value)]
+
+    def outputs(self, c):
+        """Copies of a completion's host outputs, keyed rgba / depth / id_key."""
+        npix = c.width * c.height
+        out, off = {}, 0
+        base = c.host
+        for bit, name, nbytes, dt in ((READ_RGBA, "rgba", 4, np.uint8), (READ_DEPTH, "depth", 4, np.uint32),
+                                      (READ_IDKEY, "id_key", 8, np.uint64)):
+            if self.read_mask & bit:
+                raw = (C.c_uint8 * (nbytes * npix)).from_address(base + off)
+                a = np.frombuffer(raw, dtype=dt).copy()
+                out[name] = a.reshape(c.height, c.width, 4) if name == "rgba" else a.reshape(c.height, c.width)
+                off += nbytes * npix
+        return out
+
+    def release(self, request_id):
+        check(self.lib.ocr_server_release(self.h, request_id))
+
+    def info(self):
+        i = ServerInfo()
+        check(self.lib.ocr_server_get_info(self.h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in ServerInfo._fields_}
+
+    def now_ns(self):
+        return int(self.lib.ocr_server_now_ns())
+
+    def close(self):
+        if self.h:
+            self.lib.ocr_server_destroy(self.h)
             self.h = None

@@ -51,3 +51,24 @@ def test_header_constants():
     assert "#define OCR_OP_SIZE_LIGHT    300" in text
     assert "#define OCR_BUF_ERROR        20" in text
     assert np.uint32(abi.NO_TRIANGLE) == 0xFFFFFFFF
+
+
+def test_server_records_match_header(tmp_path):
+    """The ctypes mirrors of the served-form records have the C layout the
+    header gives (sizes and field offsets, measured by compiling the header)."""
+    import subprocess
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "ocr.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(ocr_server_config),"
+        " sizeof(ocr_completion), sizeof(ocr_server_info), offsetof(ocr_completion, host),"
+        " offsetof(ocr_completion, status), offsetof(ocr_server_config, hbm_fraction),"
+        " sizeof(ocr_frame_stats)); return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(runtime.ServerConfig), ctypes.sizeof(runtime.Completion),
+            ctypes.sizeof(runtime.ServerInfo), runtime.Completion.host.offset, runtime.Completion.status.offset,
+            runtime.ServerConfig.hbm_fraction.offset, ctypes.sizeof(runtime.FrameStats)]
+    assert got == want
+    assert got[:3] == [24, 64, 48]

@@ -1,0 +1,118 @@
+"""The served form: requests batched with others give each request the same
+depth, triangle ids and RGBA as rendering its view alone (and as the
+oracle); admission refuses what the HBM budget cannot hold and admits again
+once completions are released."""
+import time
+
+import numpy as np
+import pytest
+
+from openclrenderer_amd import abi, runtime, scenes
+
+pytestmark = pytest.mark.gpu
+
+ALL = runtime.READ_RGBA | runtime.READ_DEPTH | runtime.READ_IDKEY
+
+
+def _wait_all(srv, ids, timeout_s=60.0):
+    got = {}
+    t_end = time.time() + timeout_s
+    while len(got) < len(ids) and time.time() < t_end:
+        for c in srv.poll(wait_ms=50):
+            got[c.request_id] = (c, srv.outputs(c))
+    assert len(got) == len(ids), f"{len(got)} of {len(ids)} requests completed"
+    return got
+
+
+def _scenes(gpu_ctx):
+    sc1, v1 = scenes.config1(shadow=1)
+    out = [(sc1, v1)]
+    for k in range(2):
+        out.append(scenes.random_scene(300 + k, n_tris=80 + 40 * k))
+    gs = [runtime.GpuScene(gpu_ctx, s) for s, _ in out]
+    return out, gs
+
+
+def test_alone_vs_mix_and_oracle(gpu_ctx, oracle):
+    sv, gs = _scenes(gpu_ctx)
+    srv = runtime.Server(gpu_ctx, max_batch_views=8, max_inflight=2, max_batch_tris=1 << 20,
+                         hbm_fraction=0.05, read_mask=ALL)
+    slots = [srv.add_scene(g) for g in gs]
+    rng = np.random.default_rng(7)
+    reqs = []
+    for i in range(14):
+        k = i % 3
+        if k == 0:
+            w, h = 800, 600
+            view = sv[0][1]
+        else:
+            w, h = int(rng.integers(24, 200)), int(rng.integers(16, 160))
+            view = scenes.make_view(w, h, (0.0, 0.0, -10.0), c_rot=tuple(rng.uniform(-0.2, 0.2, 3)),
+                                    hfov=float(rng.uniform(50, 80)))
+        rid = srv.submit(slots[k], view)
+        assert rid is not None
+        reqs.append((rid, k, view))
+    got = _wait_all(srv, [r[0] for r in reqs])
+    assert max(c.batch_views for c, _ in got.values()) > 1, "no request shared a batch"
+    maps = oracle.shadow_maps(sv[0][0])
+    for rid, k, view in reqs:
+        c, o = got[rid]
+        assert c.status == 0
+        alone = runtime.Frame(gpu_ctx, int(view["width"][0]), int(view["height"][0])).render_read(gs[k], view)
+        tri = runtime.id_keys_to_tri(o["id_key"], o["depth"])
+        assert np.array_equal(o["depth"], alone["depth"]), rid
+        assert np.array_equal(tri, alone["tri_id"]), rid
+        assert np.array_equal(o["rgba"], alone["rgba"]), rid
+        if k == 0 and rid == reqs[0][0]:
+            ref = oracle.render(sv[0][0], view, maps=maps)
+            assert np.array_equal(o["depth"], ref["depth"])
+            assert np.array_equal(tri, ref["tri_id"])
+            assert np.abs(o["rgba"].astype(int) - ref["rgba"].astype(int)).max() <= 1
+        srv.release(rid)
+    info = srv.info()
+    assert info["n_done"] == len(reqs) and info["hbm_held"] == 0
+    srv.close()
+
+
+def test_admission_by_hbm_budget(gpu_ctx):
+    sc, view = scenes.random_scene(11, n_tris=60)
+    g = runtime.GpuScene(gpu_ctx, sc)
+    free, _ = gpu_ctx.free_hbm()
+    w, h = int(view["width"][0]), int(view["height"][0])
+    one = 20 * w * h
+    srv = runtime.Server(gpu_ctx, max_batch_views=4, max_inflight=1, max_batch_tris=1 << 16,
+                         hbm_fraction=min(1.0, 1.5 * one / free), read_mask=runtime.READ_RGBA)
+    slot = srv.add_scene(g)
+    assert srv.info()["hbm_budget"] < 2 * one
+    r1 = srv.submit(slot, view)
+    assert r1 is not None
+    assert srv.submit(slot, view) is None          # budget holds one frame
+    _wait_all(srv, [r1])
+    assert srv.submit(slot, view) is None          # completed but not released
+    srv.release(r1)
+    r2 = srv.submit(slot, view)
+    assert r2 is not None
+    _wait_all(srv, [r2])
+    srv.release(r2)
+    assert srv.info()["n_refused"] == 2
+    with pytest.raises(runtime.OcrError) as ei:
+        srv.release(r2)
+    assert ei.value.code == abi.OCR_ERR_INVALID
+    srv.close()
+
+
+def test_one_request_per_batch_matches(gpu_ctx):
+    sv, gs = _scenes(gpu_ctx)
+    srv = runtime.Server(gpu_ctx, max_batch_views=1, max_inflight=1, max_batch_tris=1 << 20,
+                         hbm_fraction=0.05, read_mask=runtime.READ_RGBA)
+    slots = [srv.add_scene(g) for g in gs]
+    ids = [srv.submit(slots[k % 3], sv[k % 3][1]) for k in range(6)]
+    got = _wait_all(srv, ids)
+    for i, rid in enumerate(ids):
+        c, o = got[rid]
+        assert c.batch_views == 1
+        k = i % 3
+        alone = runtime.Frame(gpu_ctx, c.width, c.height).render_read(gs[k], sv[k][1])
+        assert np.array_equal(o["rgba"], alone["rgba"])
+        srv.release(rid)
+    srv.close()
