@@ -195,7 +195,7 @@ def run_plain(args):
     x_tex, x_shadow = fr.count_texels(g, view)
     B = shade_algorithmic_bytes(W, H, V, O, x_tex, x_shadow, int(sc.lights.shape[0]))
     achieved = B / (st["ms_shade"] * 1e-3) / 1e9
-    traffic = traffic_from_profiles("k_shade", workload)
+    traffic = traffic_from_profiles("k_shade<false>", workload)
 
     if args.calib:
         cal = C.CDLL(os.path.join(ROOT, "openclrenderer_amd", "libocr_calib.so"))

@@ -48,9 +48,11 @@ static_assert(sizeof(ObjDesc) == 64, "ObjDesc layout");
 
 // A clipped, culled, projected triangle: screen x, y and camera z of its
 // three vertices, its 1/det constant, its sort key (2*tri + sub for a camera
-// view, cube face for a shadow map), and the range of raster samples its
-// fragments cover.  The list is sorted by sample_begin: prearrange allocates
-// records and sample ranges together, once per block.
+// view, cube face for a shadow map), object id, view, and for camera views
+// the screen-space plane of 1/z (get_vtdiff's interpolate_get_const,
+// cl2.cl:5691-5760), which shading needs once per visible triangle.  Records
+// are in sample order: prearrange allocates records and sample ranges
+// together, once per block.
 struct alignas(16) CutTri {
   float x[3];
   float y[3];
@@ -58,10 +60,8 @@ struct alignas(16) CutTri {
   float rconst;
   uint32_t key;
   uint32_t oid;
-  uint32_t sample_begin;
-  uint32_t nsamples;
   uint32_t view;
-  uint32_t _pad;
+  float da, db, dc;   // 1/z plane (0 for shadow faces)
 };
 static_assert(sizeof(CutTri) == 64, "CutTri layout");
 
@@ -102,6 +102,7 @@ struct DevScene {
   const uint32_t* nums;
   const uint32_t* sizes;
   const ocr_light* lights;
+  const float* light_lin;   // per light: gamma_fwd of its colour (x, y, z, 0)
   const uint32_t* shadow_maps;
   const uint32_t* static_maps;
   uint32_t ntri, nobj, mip_start, nlights, L, _pad;
@@ -195,6 +196,7 @@ struct Scene {
   ObjDesc* objs = nullptr;
   Atlas atlas;
   ocr_light* lights = nullptr;
+  float* light_lin = nullptr;        // 4 floats per light
   uint32_t nlights = 0, n_shadow = 0, n_static = 0, lbuf_dim = 1024;
   std::vector<ocr_light> host_lights;
   uint32_t* shadow_maps = nullptr;   // n_shadow x 6 x dim^2
@@ -258,6 +260,7 @@ hipError_t launch_atlas_upload(const Atlas& a, uint32_t tid, const uint8_t* src,
 hipError_t launch_mip(const Atlas& a, uint32_t src_id, uint32_t dst_id, uint32_t gw, uint32_t gh,
                       hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
+hipError_t launch_light_prep(const ocr_light* lights, uint32_t n, float* lin, hipStream_t s);
 
 constexpr int kWave = 64;
 

@@ -190,18 +190,24 @@ __device__ __forceinline__ uint32_t cull_count(const f3 p[3], float ew, float eh
   return n > 0.f ? (uint32_t)n : 0u;
 }
 
+template <bool kCamera>
 __device__ __forceinline__ void store_cut(CutTri* dst, const f3 p[3], float rconst, uint32_t key, uint32_t oid,
-                                          uint32_t sample_begin, uint32_t nsamples, uint32_t view) {
+                                          uint32_t view) {
   CutTri c;
 #pragma unroll
   for (int i = 0; i < 3; i++) { c.x[i] = p[i].x; c.y[i] = p[i].y; c.z[i] = p[i].z; }
   c.rconst = rconst;
   c.key = key;
   c.oid = oid;
-  c.sample_begin = sample_begin;
-  c.nsamples = nsamples;
   c.view = view;
-  c._pad = 0;
+  c.da = c.db = c.dc = 0.f;
+  if constexpr (kCamera) {
+    // the plane kernel3's get_vtdiff builds per pixel (cl2.cl:5691-5760)
+    const f3 xpv = mk3(roundf(p[0].x), roundf(p[1].x), roundf(p[2].x));
+    const f3 ypv = mk3(roundf(p[0].y), roundf(p[1].y), roundf(p[2].y));
+    const f3 depths = mk3(1.0f / p[0].z, 1.0f / p[1].z, 1.0f / p[2].z);
+    plane_const(depths, xpv, ypv, rconst, c.da, c.db, c.dc);
+  }
   *dst = c;
 }
 
@@ -235,6 +241,7 @@ __device__ __forceinline__ int transform_clip_project(const TriPos& T, const Obj
 // pass the list capacity or the sample limit writes nothing and sets the
 // overflow word, which every later pass of the render checks before reading
 // the list.
+template <bool kCamera>
 __device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds& sh, const f3 tp[2][3],
                                             const float rc[2], RasterTri rt[2], const uint32_t nf[2],
                                             const uint32_t ns[2], uint32_t key0, uint32_t key_step, uint32_t oid,
@@ -246,7 +253,7 @@ __device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds& sh, c
   for (int i = 0; i < 2; i++) {
     if (!ns[i]) continue;
     const uint32_t key = key0 + key_step * (uint32_t)i;
-    store_cut(&L.cuts[c], tp[i], rc[i], key, oid, smp, ns[i], view);
+    store_cut<kCamera>(&L.cuts[c], tp[i], rc[i], key, oid, view);
     rt[i].key = key;
     rt[i].sample_begin = smp;
     L.rast[c] = rt[i];
@@ -355,7 +362,7 @@ __global__ void __launch_bounds__(kPreThreads) k_prearrange(PassArgs a) {
       }
     }
   }
-  append_cuts(a.lists, sh, tp, rc, rt, nf, ns, 2u * id, 1u, oid, vi);
+  append_cuts<true>(a.lists, sh, tp, rc, rt, nf, ns, 2u * id, 1u, oid, vi);
 }
 
 hipError_t launch_prearrange(const PassArgs& a, hipStream_t s) {
@@ -594,7 +601,7 @@ __global__ void __launch_bounds__(kPreThreads) k_prearrange_shadow(const TriPos*
       const int num = transform_clip_project(T, G, lpos, ft.f[kk], efov, ew, eh, tp);
       cut_pieces(tp, num, ew, eh, two_sided, kOpSizeLight, rc, rt, nf, ns);
     }
-    append_cuts(lists, sh, tp, rc, rt, nf, ns, (uint32_t)kk, 0u, T.oid, 0u);
+    append_cuts<false>(lists, sh, tp, rc, rt, nf, ns, (uint32_t)kk, 0u, T.oid, 0u);
   }
 }
 

@@ -93,6 +93,7 @@ DevScene dev_scene(const Scene* sc) {
   d.nums = sc->atlas.nums;
   d.sizes = sc->atlas.sizes;
   d.lights = sc->lights;
+  d.light_lin = sc->light_lin;
   d.shadow_maps = sc->shadow_maps;
   d.static_maps = sc->static_maps;
   d.ntri = sc->ntri;
@@ -270,6 +271,7 @@ void ocr_scene_destroy(ocr_scene* s) {
   dev_free(s->objs);
   free_atlas(s->atlas);
   dev_free(s->lights);
+  dev_free(s->light_lin);
   dev_free(s->shadow_maps);
   dev_free(s->static_maps);
   dev_free(s->sh_cuts);
@@ -397,6 +399,7 @@ int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_
     return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light buffer dimension must be 8..4000");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   dev_free(s->lights);
+  dev_free(s->light_lin);
   dev_free(s->shadow_maps);
   dev_free(s->static_maps);
   for (uint32_t i = 0; i < n; i++)
@@ -412,11 +415,12 @@ int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_
   }
   int rc;
   size_t per = (size_t)6 * light_buffer_dim * light_buffer_dim;
-  if ((rc = dev_alloc(&s->lights, n, "lights")) ||
+  if ((rc = dev_alloc(&s->lights, n, "lights")) || (rc = dev_alloc(&s->light_lin, (size_t)4 * n, "light colours")) ||
       (rc = dev_alloc(&s->shadow_maps, per * std::max<uint32_t>(s->n_shadow, 1), "shadow maps")) ||
       (rc = dev_alloc(&s->static_maps, per * std::max<uint32_t>(s->n_static, 1), "static shadow maps")))
     return rc;
   if (n) OCR_HIP_CHECK(hipMemcpy(s->lights, lights, sizeof(ocr_light) * n, hipMemcpyHostToDevice));
+  if (n) OCR_HIP_CHECK(launch_light_prep(s->lights, n, s->light_lin, nullptr));
   OCR_HIP_CHECK(launch_fill_u32(s->shadow_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_shadow, 1), nullptr));
   OCR_HIP_CHECK(launch_fill_u32(s->static_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_static, 1), nullptr));
   OCR_HIP_CHECK(hipDeviceSynchronize());

@@ -26,6 +26,7 @@ struct ShadeArgs {
   const uint32_t* sizes;
   uint32_t mip_start;
   const ocr_light* lights;
+  const float* light_lin;
   uint32_t nlights;
   const uint32_t* shadow_maps;
   const uint32_t* static_maps;
@@ -130,18 +131,27 @@ __device__ __forceinline__ float ssao(const ShadeArgs& a, int sx, int sy) {
   rad += foffset / 2.f;
   float world_rad = rad * a.v.fov / depth;
   float fsx = (float)sx, fsy = (float)sy;
-  float acc = 0.f;
+  // thresholds depth + z, z = -2..2, as the reference adds them
+  float thr[5];
+#pragma unroll
+  for (int z = 0; z < 5; z++) thr[z] = depth + (float)(z - 2);
+  // the reference sums 1.0f per passing test; the count (<= 125) is exact
+  // in binary32, so it is kept as an integer and converted once
+  uint32_t count = 0;
+#pragma unroll
   for (int y = -2; y <= 2; y++) {
     float oy = roundf((float)y * world_rad);
     float wy = clampf(fsy + oy, 1.f, (float)H - 2.f);
+#pragma unroll
     for (int x = -2; x <= 2; x++) {
       float ox = roundf((float)x * world_rad);
       float wx = clampf(fsx + ox, 1.f, (float)W - 2.f);
       float d2 = ((float)a.depth[(size_t)(int)wy * W + (int)wx] / kMulint) * kDepthFar;
-      for (float z = -2.f; z <= 2.f; z += 1.f)
-        if (d2 > depth + z) acc += 1.f;
+#pragma unroll
+      for (int z = 0; z < 5; z++) count += d2 > thr[z] ? 1u : 0u;
     }
   }
+  float acc = (float)count;
   acc /= 125.f;
   return 1.f - (1.f - acc) / kSsaoDiv;
 }
@@ -235,6 +245,7 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
   a.sizes = S.sizes;
   a.mip_start = S.mip_start;
   a.lights = S.lights;
+  a.light_lin = S.light_lin;
   a.nlights = S.nlights;
   a.shadow_maps = S.shadow_maps;
   a.static_maps = S.static_maps;
@@ -295,11 +306,8 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
   float vdx, vdy;
   {
     const float fx = (float)x, fy = (float)y;
-    f3 xpv = mk3(roundf(ct.x[0]), roundf(ct.x[1]), roundf(ct.x[2]));
-    f3 ypv = mk3(roundf(ct.y[0]), roundf(ct.y[1]), roundf(ct.y[2]));
-    f3 depths = mk3(1.0f / ct.z[0], 1.0f / ct.z[1], 1.0f / ct.z[2]);
-    float DA, DB, DC;
-    plane_const(depths, xpv, ypv, ct.rconst, DA, DB, DC);
+    // 1/z plane of the triangle, built once by prearrange
+    const float DA = ct.da, DB = ct.db, DC = ct.dc;
     float dmx = fmaf(DA, fx + 1, fmaf(DB, fy, DC));
     float dmy = fmaf(DA, fx, fmaf(DB, fy + 1, DC));
     f3 lmx = mk3((fx + 1 - (float)W / 2.f) / fov, (fy - (float)H / 2.f) / fov, 1) / dmx;
@@ -368,7 +376,7 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
     illumination *= 1.f / (1.f - cutoff);
     if (illumination <= 0) continue;
     f3 light_col = mk3(l.col[0], l.col[1], l.col[2]);
-    if (linear) light_col = mk3(gamma_fwd(light_col.x), gamma_fwd(light_col.y), gamma_fwd(light_col.z));
+    if (linear) light_col = mk3(a.light_lin[4 * i], a.light_lin[4 * i + 1], a.light_lin[4 * i + 2]);
     if (l.shadow && recv_dyn) {
       int face = cubeface(global_position, lpos);
       float dyn = 1.f - hard_occlusion<kCount>(a, lpos, normal, ptl, a.shadow_maps, a.shadow_touch, face,
@@ -414,6 +422,22 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
   if (linear) fc = mk3(gamma_inv(fc.x), gamma_inv(fc.y), gamma_inv(fc.z));
   a.rgba[pix] = quant8(fc.x) | (quant8(fc.y) << 8) | (quant8(fc.z) << 16) | (quant8(col.w) << 24);
   a.normals[pix] = encode_normal(normal);
+}
+
+// The linear-space light colours (gamma_fwd of each light's colour, the same
+// for every pixel), once per light upload.
+__global__ void k_light_prep(const ocr_light* __restrict__ lights, uint32_t n, float* __restrict__ lin) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  lin[4 * i] = gamma_fwd(lights[i].col[0]);
+  lin[4 * i + 1] = gamma_fwd(lights[i].col[1]);
+  lin[4 * i + 2] = gamma_fwd(lights[i].col[2]);
+  lin[4 * i + 3] = 0.f;
+}
+
+hipError_t launch_light_prep(const ocr_light* lights, uint32_t n, float* lin, hipStream_t s) {
+  hipLaunchKernelGGL(k_light_prep, dim3((n + 63) / 64), dim3(64), 0, s, lights, n, lin);
+  return hipGetLastError();
 }
 
 hipError_t launch_shade(const PassArgs& p, const SinCos face[6], hipStream_t s) {
