@@ -79,16 +79,20 @@ class EventPair:
         return t.value
 
 
-def traffic_from_profiles(kernel, workload):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE, WRITE_SIZE scaled by the copy-kernel factor)."""
+# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per
+# 4 cycles per SIMD
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+
+
+def profile_entry(kernel, workload):
+    """The committed rocprofv3 --pmc summary of `kernel` on `workload`: HBM
+    bytes per launch (FETCH_SIZE, WRITE_SIZE scaled by the copy-kernel
+    factor) and VALU wave-instructions per launch (SQ_INSTS_VALU)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
-        rec = json.load(open(path))
-        ent = rec[workload][kernel]
-        return float(ent["bytes_per_launch"])
+        return json.load(open(path))[workload][kernel]
     except Exception:
-        return None
+        return {}
 
 
 def shade_algorithmic_bytes(W, H, V, O, x_tex, x_shadow, L):
@@ -195,7 +199,13 @@ def run_plain(args):
     x_tex, x_shadow = fr.count_texels(g, view)
     B = shade_algorithmic_bytes(W, H, V, O, x_tex, x_shadow, int(sc.lights.shape[0]))
     achieved = B / (st["ms_shade"] * 1e-3) / 1e9
-    traffic = traffic_from_profiles("k_shade<false>", workload)
+    prof = profile_entry("k_shade<false>", workload)
+    traffic = prof.get("bytes_per_launch")
+    valu = None
+    if "valu_insts_per_launch" in prof:
+        n = float(prof["valu_insts_per_launch"])
+        valu = {"insts_per_launch": n, "peak_insts_per_s": VALU_PEAK_WAVE_INSTS,
+                "frac": n / (st["ms_shade"] * 1e-3) / VALU_PEAK_WAVE_INSTS}
 
     if args.calib:
         cal = C.CDLL(os.path.join(ROOT, "openclrenderer_amd", "libocr_calib.so"))
@@ -218,7 +228,7 @@ def run_plain(args):
                  "atlas_texels": x_tex, "shadow_texels": x_shadow},
         "roofline": {"bound": "hbm", "kernel": "k_shade", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": B},
+                     "algorithmic_bytes": B, "valu_issue": valu},
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
     })

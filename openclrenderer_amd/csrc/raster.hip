@@ -158,15 +158,17 @@ __device__ __forceinline__ void raster_setup(const f3 p[3], float rc, float ew, 
 // first, then at most one partial one.
 __device__ __forceinline__ uint32_t triangle_samples(const RasterTri& t, float max_y, int op, uint32_t nfrag) {
   if (nfrag == 0 || !(t.w > 0.f)) return 0;
-  const long long q_end = (long long)run_start(max_y, t.w, t.iw, t.min_y);
-  long long m = q_end >= op + 1 ? (q_end - (op + 1)) / op + 1 : 0;
-  if (m > (long long)nfrag) m = nfrag;
-  long long s = (long long)(op + 1) * m;
-  if (m < (long long)nfrag) {
-    const long long r = q_end - (long long)op * m;
-    s += r < 0 ? 0 : (r > op + 1 ? op + 1 : r);
+  // q_end < 2^24 (frames hold at most 16,000,000 pixels): 32-bit arithmetic
+  const uint32_t q_end = (uint32_t)run_start(max_y, t.w, t.iw, t.min_y);
+  const uint32_t uop = (uint32_t)op;
+  uint32_t m = q_end >= uop + 1 ? (q_end - (uop + 1)) / uop + 1 : 0;
+  if (m > nfrag) m = nfrag;
+  uint64_t s = (uint64_t)(uop + 1) * m;
+  if (m < nfrag) {
+    const int64_t r = (int64_t)q_end - (int64_t)uop * m;
+    s += r < 0 ? 0 : (r > (int64_t)uop + 1 ? uop + 1 : (uint64_t)r);
   }
-  return s > (long long)kMaxTriSamples ? kMaxTriSamples : (uint32_t)s;
+  return s > (uint64_t)kMaxTriSamples ? kMaxTriSamples : (uint32_t)s;
 }
 
 // Clip/cull/count of one projected triangle (cl2.cl:4344-4380): returns the
@@ -227,12 +229,17 @@ __device__ __forceinline__ int transform_clip_project(const TriPos& T, const Obj
   }
   f3 tris[2][3];
   int n = clip_near(pr[0], pr[1], pr[2], tris);
-  // both slots are projected unconditionally (static indices keep the
-  // arrays in registers); slot 1 is only read when n == 2
+  // slot 1 exists only for triangles crossing the near plane with one
+  // vertex behind it; static indices keep both slots in registers
 #pragma unroll
-  for (int t = 0; t < 2; t++)
+  for (int i = 0; i < 3; i++) out[0][i] = depth_project(tris[0][i], ew, eh, fovc);
+  if (n == 2) {
 #pragma unroll
-    for (int i = 0; i < 3; i++) out[t][i] = depth_project(tris[t][i], ew, eh, fovc);
+    for (int i = 0; i < 3; i++) out[1][i] = depth_project(tris[1][i], ew, eh, fovc);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; i++) out[1][i] = out[0][i];
+  }
   return n;
 }
 
@@ -402,6 +409,7 @@ struct TriLds {
   float ek[7][kWave];
   uint32_t key[kWave];
   uint32_t begin[kWave + 1];
+  uint32_t start[kWave];   // start[p] == base: a staged triangle begins at sample base + p
   uint32_t pitch[kBatch ? kWave : 1];
   uint32_t* dst[kBatch || kMode == RasterMode::Shadow ? kWave : 1];
   unsigned long long* ids[kBatch && kMode == RasterMode::Id ? kWave : 1];
@@ -419,6 +427,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
   TriLds<kMode, kBatch>& L = lds_all[threadIdx.x / kWave];
   const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  L.start[lane] = 0xFFFFFFFFu;   // never a sample index (total <= 2^30)
   for (uint32_t s0 = wave0 * kChunk; s0 < total; s0 += nwaves * kChunk) {
     const uint32_t s1 = min(s0 + kChunk, total);
     // ---- the chunk's first triangle: last record with sample_begin <= s0
@@ -438,8 +447,10 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
     while (s < s1) {
       // ---- stage triangles c0 .. c0 + 63, one per lane
       const uint32_t ci = c0 + lane;
+      uint32_t my_begin = total;   // this lane's staged triangle's first sample
       if (ci < ncut) {
         const RasterTri t = Ls.rast[ci];
+        my_begin = t.sample_begin;
         L.min_x[lane] = t.min_x;
         L.min_y[lane] = t.min_y;
         L.w[lane] = t.w;
@@ -469,21 +480,40 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const uint32_t stage_end = min(s1, L.begin[kWave]);
       // ---- every sample of [s, stage_end) with all lanes
-      for (uint32_t j = s + lane; j < stage_end; j += kWave) {
-        int k = 0;
-#pragma unroll
-        for (int step = 32; step >= 1; step >>= 1)
-          if (L.begin[k + step] <= j) k += step;
-        // sample -> (fragment a, walk index q); pa = op * a
-        const uint32_t sl = j - L.begin[k];
-        const uint32_t fa = sl / (uint32_t)(kOp + 1);
-        const uint32_t o = sl - fa * (uint32_t)(kOp + 1);
-        const float paf = (float)(fa * (uint32_t)kOp);
-        const float qf = paf + (float)o;
+      for (uint32_t base = s; base < stage_end; base += kWave) {
+        // Lane p takes sample j = base + p.  Its staged triangle k is one less
+        // than the number of staged triangles beginning at or before j: those
+        // beginning at or before `base` (a prefix of the lanes, one ballot)
+        // plus those beginning in (base, j] (flags written by the triangles'
+        // lanes, counted below p with mbcnt).
+        const unsigned long long before = __ballot(my_begin <= base);
+        const uint32_t off = my_begin - base;
+        if (my_begin > base && off < (uint32_t)kWave) L.start[off] = base;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const bool starts_here = L.start[lane] == base;
+        const unsigned long long starts = __ballot(starts_here);
+        const uint32_t j = base + (uint32_t)lane;
+        if (j >= stage_end) continue;
+        const int k = __popcll(before) - 1 +
+                      (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
+                      (starts_here ? 1 : 0);
+        // sample -> (fragment a, walk index q), pa = op * a, in exact binary32
+        // integers (a float quotient estimate corrected by one step each way)
+        const float slf = (float)(j - L.begin[k]);
+        float faf = floorf(slf * (1.0f / (float)(kOp + 1)));
+        float of = fmaf(-faf, (float)(kOp + 1), slf);
+        faf = of < 0.f ? faf - 1.f : (of >= (float)(kOp + 1) ? faf + 1.f : faf);
+        of = fmaf(-faf, (float)(kOp + 1), slf);
+        const float paf = faf * (float)kOp;
+        const float qf = paf + of;
         const float w = L.w[k], iw = L.iw[k], min_y = L.min_y[k];
         const float y = row_of(qf, iw, min_y);
         // cursor column: (r % w) + min_x + (q - r), r = max(pa, row start)
-        const float r = fmaxf(paf, run_start(y, w, iw, min_y));
+        const float rs = run_start(y, w, iw, min_y);
+        const float r = paf > rs ? paf : rs;
         const float m = floorf(r * iw);
         float e = fmaf(-m, w, r);
         e = e < 0.f ? e + w : e;
