@@ -63,6 +63,22 @@ __global__ void __launch_bounds__(256) k_calib_scatter_store(uint32_t* __restric
   for (; i < n; i += stride) buf[mix32((uint32_t)i) % words] = (uint32_t)i;
 }
 
+// As k_calib_atomic_min at workgroup scope: the atomic is performed in the
+// issuing XCD's L2 (throughput probe only; its values are meaningless when
+// several XCDs touch the buffer).
+__global__ void __launch_bounds__(256) k_calib_atomic_min_wg(uint32_t* __restrict__ buf, uint32_t words, size_t n,
+                                                             uint32_t cluster) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    uint32_t g = (uint32_t)(i / cluster);
+    uint32_t a = cluster == 1 ? mix32((uint32_t)i) % words
+                              : (mix32(g) % (words / 64)) * 64 + (uint32_t)(i % 64);
+    __hip_atomic_fetch_min(&buf[a], mix32((uint32_t)i ^ 0x9e3779b9u), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+
 // One 64-bit atomicAdd per wavefront on a single address (n wavefront
 // iterations), the pattern of a global work-list counter.
 __global__ void __launch_bounds__(256) k_calib_atomic_same(unsigned long long* __restrict__ counter, size_t n) {
@@ -74,7 +90,8 @@ __global__ void __launch_bounds__(256) k_calib_atomic_same(unsigned long long* _
 
 // Returns the average kernel time in microseconds of `iters` launches
 // (mode 0: u32 atomicMin, 1: u64 atomicMax, 2: plain scattered store,
-// 3: one same-address u64 atomicAdd per wavefront, n wavefront iterations).
+// 3: one same-address u64 atomicAdd per wavefront, n wavefront iterations,
+// 4: u32 atomicMin at workgroup scope).
 extern "C" double ocr_calib_atomics(int mode, uint32_t words, size_t n, uint32_t cluster, int iters) {
   void* buf = nullptr;
   if (hipMalloc(&buf, (size_t)words * 8) != hipSuccess) return -1.0;
@@ -88,6 +105,7 @@ extern "C" double ocr_calib_atomics(int mode, uint32_t words, size_t n, uint32_t
     if (mode == 0) hipLaunchKernelGGL(k_calib_atomic_min, grid, block, 0, nullptr, (uint32_t*)buf, words, n, cluster);
     else if (mode == 1) hipLaunchKernelGGL(k_calib_atomic_max64, grid, block, 0, nullptr, (unsigned long long*)buf, words, n, cluster);
     else if (mode == 2) hipLaunchKernelGGL(k_calib_scatter_store, grid, block, 0, nullptr, (uint32_t*)buf, words, n);
+    else if (mode == 4) hipLaunchKernelGGL(k_calib_atomic_min_wg, grid, block, 0, nullptr, (uint32_t*)buf, words, n, cluster);
     else hipLaunchKernelGGL(k_calib_atomic_same, grid, block, 0, nullptr, (unsigned long long*)buf, n);
   }
   (void)hipEventRecord(e1, nullptr);

@@ -391,7 +391,11 @@ hipError_t launch_prearrange(const PassArgs& a, hipStream_t s) {
 // (Y - min_y) * width by stepping while y disagrees, so rows whose fma
 // rounding lags or leads by a sample are reproduced exactly.
 
-constexpr uint32_t kChunk = kWave * 8;   // samples per wavefront work item
+constexpr uint32_t kChunk = kWave * 16;   // samples per wavefront work item
+
+// s_waitcnt immediate for vmcnt(0), expcnt(7), lgkmcnt(15) (gfx9 encoding:
+// vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14])
+constexpr int kWaitVmcnt0 = 0x0F70;
 
 enum class RasterMode { Depth, Id, Shadow };
 
@@ -475,6 +479,11 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
         L.begin[lane] = total;
         if (lane == kWave - 1) L.begin[kWave] = total;
       }
+      // Every staging load has landed before the sweep.  vmcnt counts loads
+      // and atomics in one in-order counter: without this unconditional wait
+      // the compiler waits for my_begin inside the loop, i.e. for all the
+      // atomics of the previous iteration, on every iteration.
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
