@@ -216,7 +216,7 @@ int  ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream);
  * triangle id u32 (OCR_NO_TRIANGLE where nothing is visible), RGBA8 and the
  * encoded screen normals (ushort2 per pixel, cl2.cl:5617-5630).  Returns
  * OCR_ERR_OVERFLOW if the last render overflowed a reserved cut list, and
- * OCR_ERR_UNSUPPORTED if it needed more than 2^30 raster samples. */
+ * OCR_ERR_UNSUPPORTED if it needed more than 2^31 - 1 raster samples. */
 int  ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id,
                     uint8_t* rgba8, uint16_t* normals, void* stream);
 /* Device pointers of the frame's buffers (zero-copy consumers). */
@@ -260,7 +260,7 @@ int  ocr_batch_render(ocr_batch* b, void* stream);
 /* The same five launches captured once into a hipGraph and replayed. */
 int  ocr_batch_launch_graph(ocr_batch* b, void* stream);
 /* Synchronises `stream`; counters of the last render (OCR_ERR_UNSUPPORTED if
- * the views needed more than 2^30 raster samples together). */
+ * the views needed more than 2^31 - 1 raster samples together). */
 int  ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out);
 /* Times raster grid sizes x shading tile shapes on the current views and
  * keeps the fastest (reps graph launches each). */

@@ -300,7 +300,7 @@ int ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out) {
     out->ms_total = (float)(sum / b->n_profiled);
   }
   if (c[kOverflow] & kOverflowSamples)
-    return fail(OCR_ERR_UNSUPPORTED, "ocr_batch_get_stats: the batch needs more than 2^30 raster samples");
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_batch_get_stats: the batch needs more than 2^31 - 1 raster samples");
   if (c[kOverflow]) return fail(OCR_ERR_OVERFLOW, "ocr_batch_get_stats: cut list overflowed");
   return OCR_OK;
 }

@@ -170,7 +170,7 @@ constexpr uint32_t kOverflowCuts = 1u, kOverflowSamples = 2u;
 // hold at most 16,000,000 pixels, so a triangle has at most
 // area * (1 + 1/op) + op + 1 < 2^24 samples: sample indices, bounding-box
 // products and row starts are exact in binary32.
-constexpr uint32_t kMaxSamples = 1u << 30;
+constexpr uint32_t kMaxSamples = 0x7FFFFFFFu;
 constexpr uint32_t kMaxTriSamples = (1u << 24) - 1;
 constexpr uint64_t kMaxTargetPixels = 16000000ull;
 

@@ -431,7 +431,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
   TriLds<kMode, kBatch>& L = lds_all[threadIdx.x / kWave];
   const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  L.start[lane] = 0xFFFFFFFFu;   // never a sample index (total <= 2^30)
+  L.start[lane] = 0xFFFFFFFFu;   // never a sample index (total < 2^31)
   for (uint32_t s0 = wave0 * kChunk; s0 < total; s0 += nwaves * kChunk) {
     const uint32_t s1 = min(s0 + kChunk, total);
     // ---- the chunk's first triangle: last record with sample_begin <= s0

@@ -695,7 +695,7 @@ int ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id, uint8_t* rgb
   uint32_t c[kCounterWords];
   OCR_HIP_CHECK(hipMemcpy(c, f->counters, sizeof(c), hipMemcpyDeviceToHost));
   if (c[kOverflow] & kOverflowSamples)
-    return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_read: the render needs more than 2^30 raster samples");
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_read: the render needs more than 2^31 - 1 raster samples");
   if (c[kOverflow]) {
     // A reserved cut list was too small: size the next render's list for
     // this one's count (high word of the packed counter).
