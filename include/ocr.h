@@ -304,8 +304,11 @@ typedef struct ocr_completion {
 typedef struct ocr_server_info {
     uint64_t hbm_budget, hbm_held;
     uint64_t n_batches, n_done, n_refused;
+    uint64_t n_splits;          /* batches re-queued in halves because their
+                                   views needed more raster samples than one
+                                   render holds                            */
     uint32_t n_pending, n_frames;
-} ocr_server_info;              /* 48 B */
+} ocr_server_info;              /* 56 B */
 
 int  ocr_server_create(ocr_ctx* ctx, const ocr_server_config* cfg, ocr_server** out);
 /* Registers a scene (atlas uploaded) and renders its shadow maps once. */
@@ -320,7 +323,8 @@ int  ocr_server_poll(ocr_server* s, ocr_completion* out, uint32_t max, uint32_t*
 int  ocr_server_release(ocr_server* s, uint64_t request_id);
 int  ocr_server_get_info(ocr_server* s, ocr_server_info* info);
 uint64_t ocr_server_now_ns(void);
-/* Waits for in-flight batches, drops requests that never started. */
+/* Waits for in-flight batches, drops requests that never started.  A server
+ * still alive at process exit has its threads stopped by an exit handler. */
 void ocr_server_destroy(ocr_server* s);
 
 #ifdef __cplusplus

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -47,6 +48,7 @@ struct Request {
   uint64_t t_submit = 0, t_start = 0, t_done = 0;
   int32_t status = OCR_OK;
   uint32_t batch_views = 0;
+  uint32_t max_views = 0xFFFFFFFFu;   // lowered when a batch holding it overflowed
   uint64_t hbm = 0;
   ocr_frame* frame = nullptr;
   void* host = nullptr;            // pinned: rgba8 | depth | id key, as read_mask asks
@@ -80,13 +82,44 @@ struct ocr_server {
   std::vector<void*> all_host;
   uint64_t budget = 0, held = 0;
   uint64_t next_id = 1;
-  uint64_t n_batches = 0, n_done = 0, n_refused = 0;
+  uint64_t n_batches = 0, n_done = 0, n_refused = 0, n_splits = 0;
   bool stop = false;
   int error = OCR_OK;
   std::thread dispatcher, completer;
 };
 
 namespace {
+
+// Servers still alive at process exit are stopped (threads joined) before the
+// HIP runtime's own teardown, which runs after this handler.
+std::mutex g_registry_mu;
+std::vector<ocr_server*> g_live_servers;
+bool g_exit_hook = false;
+
+void stop_threads(ocr_server* s);
+
+void stop_live_servers_at_exit() {
+  std::vector<ocr_server*> live;
+  {
+    std::lock_guard<std::mutex> g(g_registry_mu);
+    live.swap(g_live_servers);
+  }
+  for (ocr_server* s : live) stop_threads(s);
+}
+
+void register_server(ocr_server* s) {
+  std::lock_guard<std::mutex> g(g_registry_mu);
+  if (!g_exit_hook) {
+    std::atexit(stop_live_servers_at_exit);
+    g_exit_hook = true;
+  }
+  g_live_servers.push_back(s);
+}
+
+void unregister_server(ocr_server* s) {
+  std::lock_guard<std::mutex> g(g_registry_mu);
+  g_live_servers.erase(std::remove(g_live_servers.begin(), g_live_servers.end(), s), g_live_servers.end());
+}
 
 size_t host_bytes_for(const ocr_server* s, uint32_t w, uint32_t h) {
   const size_t npix = (size_t)w * h;
@@ -165,9 +198,11 @@ int launch_batch(ocr_server* s, Slot& slot, std::vector<Request*>& reqs) {
     Request* r = reqs[i];
     int rc = take_frame(s, r->view.width, r->view.height, &r->frame);
     if (rc) return rc;
-    rc = take_host(s, host_bytes_for(s, r->view.width, r->view.height), &r->host);
-    if (rc) return rc;
-    r->host_bytes = host_bytes_for(s, r->view.width, r->view.height);
+    if (!r->host) {   // a request re-dispatched after a split keeps its buffer
+      rc = take_host(s, host_bytes_for(s, r->view.width, r->view.height), &r->host);
+      if (rc) return rc;
+      r->host_bytes = host_bytes_for(s, r->view.width, r->view.height);
+    }
     sc[i] = static_cast<ocr_scene*>(s->scenes[r->scene]);
     vw[i] = r->view;
     fr[i] = r->frame;
@@ -212,10 +247,12 @@ void dispatcher_main(ocr_server* s) {
       return false;
     });
     if (s->stop) return;
-    // continuous batching: everything pending, up to the batch limits
+    // continuous batching: everything pending, up to the batch limits (and
+    // up to the cap a split left on the oldest request)
     std::vector<Request*> reqs;
     uint64_t tris = 0;
-    while (!s->pending.empty() && reqs.size() < s->cfg.max_batch_views) {
+    const uint32_t limit = std::min(s->cfg.max_batch_views, s->pending.front()->max_views);
+    while (!s->pending.empty() && reqs.size() < limit) {
       Request* r = s->pending.front();
       const uint64_t t = s->scenes[r->scene]->ntri;
       if (!reqs.empty() && tris + t > s->cfg.max_batch_tris) break;
@@ -264,6 +301,24 @@ void completer_main(ocr_server* s) {
     int rc = e == hipSuccess ? ocr_batch_get_stats(slot.batch, slot.stream, &st) : OCR_ERR_HIP;
     const uint64_t t = now_ns();
     lk.lock();
+    if (rc == OCR_ERR_UNSUPPORTED && slot.reqs.size() > 1) {
+      // The views together needed more raster samples than one render
+      // holds: put them back at the head of the queue, in order, to be
+      // rendered in batches of at most half as many views.
+      const uint32_t half = (uint32_t)std::max<size_t>(1, slot.reqs.size() / 2);
+      for (size_t i = slot.reqs.size(); i-- > 0;) {
+        Request* r = slot.reqs[i];
+        s->idle_frames[(uint64_t)r->view.width << 32 | r->view.height].push_back(r->frame);
+        r->frame = nullptr;
+        r->max_views = std::min(r->max_views, half);
+        s->pending.push_front(r);
+      }
+      s->n_splits++;
+      slot.reqs.clear();
+      slot.busy = false;
+      s->cv_dispatch.notify_all();
+      continue;
+    }
     for (Request* r : slot.reqs) {
       r->t_done = t;
       r->status = rc;
@@ -317,6 +372,7 @@ int ocr_server_create(ocr_ctx* ctx, const ocr_server_config* cfg, ocr_server** o
   s->budget = (uint64_t)((double)free_b * cfg->hbm_fraction);
   s->dispatcher = std::thread(dispatcher_main, s);
   s->completer = std::thread(completer_main, s);
+  register_server(s);
   *out = s;
   return OCR_OK;
 }
@@ -411,6 +467,28 @@ int ocr_server_release(ocr_server* s, uint64_t request_id) {
 
 uint64_t ocr_server_now_ns(void) { return now_ns(); }
 
+}  // extern "C"
+
+namespace {
+
+// Lets in-flight batches finish, then stops and joins both threads; requests
+// that never started stay queued (destroy frees them).
+void stop_threads(ocr_server* s) {
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_poll.wait_for(lk, std::chrono::seconds(60), [&] { return s->busy_order.empty(); });
+    s->stop = true;
+    s->cv_dispatch.notify_all();
+    s->cv_complete.notify_all();
+  }
+  if (s->dispatcher.joinable()) s->dispatcher.join();
+  if (s->completer.joinable()) s->completer.join();
+}
+
+}  // namespace
+
+extern "C" {
+
 int ocr_server_get_info(ocr_server* s, ocr_server_info* info) {
   if (!s || !info) return fail(OCR_ERR_INVALID, "ocr_server_get_info: null argument");
   std::lock_guard<std::mutex> g(s->mu);
@@ -420,6 +498,7 @@ int ocr_server_get_info(ocr_server* s, ocr_server_info* info) {
   info->n_batches = s->n_batches;
   info->n_done = s->n_done;
   info->n_refused = s->n_refused;
+  info->n_splits = s->n_splits;
   info->n_pending = (uint32_t)s->pending.size();
   info->n_frames = (uint32_t)s->all_frames.size();
   return OCR_OK;
@@ -427,16 +506,8 @@ int ocr_server_get_info(ocr_server* s, ocr_server_info* info) {
 
 void ocr_server_destroy(ocr_server* s) {
   if (!s) return;
-  {
-    std::unique_lock<std::mutex> lk(s->mu);
-    // let in-flight batches finish; drop what never started
-    s->cv_poll.wait_for(lk, std::chrono::seconds(60), [&] { return s->busy_order.empty(); });
-    s->stop = true;
-    s->cv_dispatch.notify_all();
-    s->cv_complete.notify_all();
-  }
-  if (s->dispatcher.joinable()) s->dispatcher.join();
-  if (s->completer.joinable()) s->completer.join();
+  unregister_server(s);
+  stop_threads(s);
   (void)hipSetDevice(s->ctx->device);
   (void)hipDeviceSynchronize();
   // every admitted request stays in `live` until released

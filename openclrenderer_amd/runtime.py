@@ -66,8 +66,8 @@ class Completion(C.Structure):
 
 class ServerInfo(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("hbm_held", C.c_uint64), ("n_batches", C.c_uint64),
-                ("n_done", C.c_uint64), ("n_refused", C.c_uint64), ("n_pending", C.c_uint32),
-                ("n_frames", C.c_uint32)]
+                ("n_done", C.c_uint64), ("n_refused", C.c_uint64), ("n_splits", C.c_uint64),
+                ("n_pending", C.c_uint32), ("n_frames", C.c_uint32)]
 
 
 READ_RGBA, READ_DEPTH, READ_IDKEY = 1, 2, 4
@@ -431,3 +431,9 @@ class Server:
         if self.h:
             self.lib.ocr_server_destroy(self.h)
             self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
