@@ -96,6 +96,7 @@ struct ViewParams {
 // One scene as the kernels see it.
 struct DevScene {
   const TriPos* tri_pos;
+  const TriPos* world;      // world-space vertices (object transform applied)
   const TriAttr* tri_attr;
   const ObjDesc* objs;
   const uint8_t* pages;
@@ -192,6 +193,7 @@ struct Scene {
   uint32_t ntri = 0, nobj = 0;
   uint32_t max_tid = 0;              // largest descriptor texture id (validated at render)
   TriPos* tri_pos = nullptr;
+  TriPos* world = nullptr;           // world-space copy, refreshed when objects change
   TriAttr* tri_attr = nullptr;
   ObjDesc* objs = nullptr;
   Atlas atlas;
@@ -261,6 +263,7 @@ hipError_t launch_mip(const Atlas& a, uint32_t src_id, uint32_t dst_id, uint32_t
                       hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
 hipError_t launch_light_prep(const ocr_light* lights, uint32_t n, float* lin, hipStream_t s);
+hipError_t launch_world(Scene* sc, hipStream_t s);
 
 constexpr int kWave = 64;
 

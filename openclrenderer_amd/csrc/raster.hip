@@ -215,20 +215,51 @@ __device__ __forceinline__ void store_cut(CutTri* dst, const f3 p[3], float rcon
 
 __device__ __forceinline__ f4 ld_quat(const ObjDesc& g) { return f4{g.q[0], g.q[1], g.q[2], g.q[3]}; }
 
-// full_rotate_quat, cl2.cl:700-729.
-__device__ __forceinline__ int transform_clip_project(const TriPos& T, const ObjDesc& G, f3 cpos,
-                                                      const SinCos& sc, float fovc, float ew, float eh,
-                                                      f3 out[2][3]) {
-  f3 wp = mk3(G.wp[0], G.wp[1], G.wp[2]);
-  f4 q = ld_quat(G);
+// full_rotate_quat's object-to-world step (cl2.cl:700-729): the same for the
+// camera and every light, so it is applied once per object update into the
+// scene's world-space copy of its triangles.
+__global__ void __launch_bounds__(256) k_world(const TriPos* __restrict__ tris, uint32_t ntri,
+                                               const ObjDesc* __restrict__ objs, uint32_t nobj,
+                                               TriPos* __restrict__ world) {
+  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= ntri) return;
+  const TriPos T = tris[id];
+  TriPos W;
+  W.oid = T.oid;
+  if (T.oid < nobj) {
+    const ObjDesc G = objs[T.oid];
+    const f3 wp = mk3(G.wp[0], G.wp[1], G.wp[2]);
+    const f4 q = ld_quat(G);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const f3 r = rot_quat(mk3(T.p[3 * i], T.p[3 * i + 1], T.p[3 * i + 2]) * G.scale, q) + wp;
+      W.p[3 * i] = r.x;
+      W.p[3 * i + 1] = r.y;
+      W.p[3 * i + 2] = r.z;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 9; i++) W.p[i] = 0.f;
+  }
+  world[id] = W;
+}
+
+hipError_t launch_world(Scene* sc, hipStream_t s) {
+  if (sc->ntri)
+    hipLaunchKernelGGL(k_world, dim3((sc->ntri + 255) / 256), dim3(256), 0, s, sc->tri_pos, sc->ntri, sc->objs,
+                       sc->nobj, sc->world);
+  return hipGetLastError();
+}
+
+// full_rotate_quat's rotation into a camera or cube face (cl2.cl:700-729),
+// near clip and projection, from world-space vertices.
+__device__ __forceinline__ int face_clip_project(const f3 w[3], f3 cpos, const SinCos& sc, float fovc, float ew,
+                                                 float eh, f3 out[2][3]) {
   f3 pr[3];
 #pragma unroll
-  for (int i = 0; i < 3; i++) {
-    f3 v = mk3(T.p[3 * i], T.p[3 * i + 1], T.p[3 * i + 2]);
-    pr[i] = rot(rot_quat(v * G.scale, q) + wp, cpos, sc);
-  }
+  for (int i = 0; i < 3; i++) pr[i] = rot(w[i], cpos, sc);
   f3 tris[2][3];
-  int n = clip_near(pr[0], pr[1], pr[2], tris);
+  const int n = clip_near(pr[0], pr[1], pr[2], tris);
   // slot 1 exists only for triangles crossing the near plane with one
   // vertex behind it; static indices keep both slots in registers
 #pragma unroll
@@ -358,13 +389,14 @@ __global__ void __launch_bounds__(kPreThreads) k_prearrange(PassArgs a) {
   uint32_t nf[2] = {0u, 0u}, ns[2] = {0u, 0u};
   uint32_t oid = 0;
   if (id < S.ntri) {
-    const TriPos T = S.tri_pos[id];
+    const TriPos T = S.world[id];
     oid = T.oid;
     if (oid < S.nobj) {
       const ObjDesc G = S.objs[oid];
       f3 cpos = mk3(v.cpos[0], v.cpos[1], v.cpos[2]);
       if (!(fast_length(mk3(G.wp[0], G.wp[1], G.wp[2]) - cpos) > kDepthFar)) {
-        const int num = transform_clip_project(T, G, cpos, v.cam, v.fov, ew, eh, tp);
+        const f3 w[3] = {mk3(T.p[0], T.p[1], T.p[2]), mk3(T.p[3], T.p[4], T.p[5]), mk3(T.p[6], T.p[7], T.p[8])};
+        const int num = face_clip_project(w, cpos, v.cam, v.fov, ew, eh, tp);
         cut_pieces(tp, num, ew, eh, (G.flags & OCR_FEATURE_TWO_SIDED) > 0, kOpSize, rc, rt, nf, ns);
       }
     }
@@ -593,57 +625,86 @@ hipError_t launch_id(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
 // ---------------------------------------------------------------- shadows
 struct FaceTable { SinCos f[6]; };
 
-__global__ void __launch_bounds__(kPreThreads) k_prearrange_shadow(const TriPos* __restrict__ tris, uint32_t ntri,
+// prearrange_realtime_shadowing (cl2.cl:4419-4636) for one light.  Phase 1:
+// one triangle per work item — its world-space vertices, the faces they fall
+// in (ret_cubeface), each (triangle, face) pair appended to a block-local
+// work list in LDS.  Phase 2: the pairs, one per work item per
+// round, clipped, projected, culled and set up for their face, with one
+// record allocation per round.  (Looping over the six faces per triangle
+// would leave most lanes idle: a triangle touches one to three faces.)
+struct ShadowLds {
+  float wpos[kPreThreads][9];
+  uint32_t oid[kPreThreads];
+  uint32_t items[kPreThreads * 3];   // local triangle << 4 | two_sided << 3 | face
+  uint32_t n_items;
+};
+
+__global__ void __launch_bounds__(kPreThreads) k_prearrange_shadow(const TriPos* __restrict__ world, uint32_t ntri,
                                                            const ObjDesc* __restrict__ objs, uint32_t nobj,
                                                            float lx, float ly, float lz, uint32_t L,
                                                            int only_static, FaceTable ft, Lists lists) {
   __shared__ BlockAllocLds sh;
-  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ ShadowLds S;
+  const uint32_t t = threadIdx.x;
+  const uint32_t id = blockIdx.x * blockDim.x + t;
   const float ew = (float)L, eh = (float)L, efov = (float)L / 2.0f;
   const f3 lpos = mk3(lx, ly, lz);
-  int touched = 0;
-  TriPos T;
-  ObjDesc G;
-  bool live = false;
+  if (t == 0) S.n_items = 0;
+  __syncthreads();
+  // ---- phase 1
+  uint32_t touched = 0, two_sided = 0;
   if (id < ntri) {
-    T = tris[id];
+    const TriPos T = world[id];
     if (T.oid < nobj) {
-      G = objs[T.oid];
-      bool is_static = (G.flags & OCR_FEATURE_IS_STATIC) > 0;
-      live = only_static ? is_static : !is_static;
-      f3 wp = mk3(G.wp[0], G.wp[1], G.wp[2]);
+      const ObjDesc G = objs[T.oid];
+      const bool is_static = (G.flags & OCR_FEATURE_IS_STATIC) > 0;
+      bool live = only_static ? is_static : !is_static;
+      const f3 wp = mk3(G.wp[0], G.wp[1], G.wp[2]);
       if (live && fast_length(wp - lpos) > kDepthFar) live = false;
       if (live) {
-        f4 q = ld_quat(G);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-          f3 r = rot_quat(mk3(T.p[3 * k], T.p[3 * k + 1], T.p[3 * k + 2]) * G.scale, q) + wp;
-          touched |= 1 << cubeface(r, lpos);
+          const f3 r = mk3(T.p[3 * k], T.p[3 * k + 1], T.p[3 * k + 2]);
+          S.wpos[t][3 * k] = r.x;
+          S.wpos[t][3 * k + 1] = r.y;
+          S.wpos[t][3 * k + 2] = r.z;
+          touched |= 1u << cubeface(r, lpos);
         }
+        S.oid[t] = T.oid;
+        two_sided = (G.flags & OCR_FEATURE_TWO_SIDED) > 0 ? 1u : 0u;
       }
     }
   }
-  const bool two_sided = live && (G.flags & OCR_FEATURE_TWO_SIDED) > 0;
-  // Faces one at a time; every work item of the block takes part in each
-  // face's allocation (one that does not touch the face appends nothing).
-  __shared__ int faces_touched;
-  if (threadIdx.x == 0) faces_touched = 0;
+  const uint32_t mine = (uint32_t)__popc(touched);
+  const uint32_t incl = wave_incl_scan(mine);
+  const uint32_t wave_total = __shfl(incl, kWave - 1, kWave);
+  uint32_t wbase = 0;
+  if ((t & (kWave - 1)) == 0 && wave_total) wbase = atomicAdd(&S.n_items, wave_total);
+  wbase = __shfl(wbase, 0, kWave);
+  uint32_t slot = wbase + incl - mine;
+  for (uint32_t f = 0; f < 6; f++)
+    if ((touched >> f) & 1u) S.items[slot++] = t << 4 | two_sided << 3 | f;
   __syncthreads();
-  if (touched) atomicOr(&faces_touched, touched);
-  __syncthreads();
-  const int any_touched = faces_touched;
-#pragma unroll
-  for (int kk = 0; kk < 6; kk++) {
-    if (!((any_touched >> kk) & 1)) continue;
+  const uint32_t n_items = S.n_items;
+  // ---- phase 2: same rounds for every work item of the block
+  for (uint32_t r0 = 0; r0 < n_items; r0 += kPreThreads) {
     f3 tp[2][3];
     float rc[2] = {0.f, 0.f};
     RasterTri rt[2];
     uint32_t nf[2] = {0u, 0u}, ns[2] = {0u, 0u};
-    if ((touched >> kk) & 1) {
-      const int num = transform_clip_project(T, G, lpos, ft.f[kk], efov, ew, eh, tp);
-      cut_pieces(tp, num, ew, eh, two_sided, kOpSizeLight, rc, rt, nf, ns);
+    uint32_t face = 0, oid = 0;
+    if (r0 + t < n_items) {
+      const uint32_t it = S.items[r0 + t];
+      const uint32_t lt = it >> 4;
+      face = it & 7u;
+      oid = S.oid[lt];
+      f3 w[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) w[k] = mk3(S.wpos[lt][3 * k], S.wpos[lt][3 * k + 1], S.wpos[lt][3 * k + 2]);
+      const int num = face_clip_project(w, lpos, ft.f[face], efov, ew, eh, tp);
+      cut_pieces(tp, num, ew, eh, ((it >> 3) & 1u) != 0u, kOpSizeLight, rc, rt, nf, ns);
     }
-    append_cuts<false>(lists, sh, tp, rc, rt, nf, ns, (uint32_t)kk, 0u, T.oid, 0u);
+    append_cuts<false>(lists, sh, tp, rc, rt, nf, ns, face, 0u, oid, 0u);
   }
 }
 
@@ -658,7 +719,7 @@ hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, 
   const Lists lists = make_lists(sc->sh_cuts, sc->sh_counters, sc->sh_cap_cut);
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, sc->sh_counters);
   if (sc->ntri)
-    hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + kPreThreads - 1) / kPreThreads), dim3(kPreThreads), 0, s, sc->tri_pos,
+    hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + kPreThreads - 1) / kPreThreads), dim3(kPreThreads), 0, s, sc->world,
                        sc->ntri, sc->objs, sc->nobj, lpos[0], lpos[1], lpos[2], sc->lbuf_dim, only_static,
                        ft, lists);
   PassArgs a;

@@ -87,6 +87,7 @@ DevScene dev_scene(const Scene* sc) {
   DevScene d;
   memset(&d, 0, sizeof(d));
   d.tri_pos = sc->tri_pos;
+  d.world = sc->world;
   d.tri_attr = sc->tri_attr;
   d.objs = sc->objs;
   d.pages = sc->atlas.pages;
@@ -223,7 +224,8 @@ int ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri, cons
     s->max_tid = std::max(s->max_tid, objs[o].tid);
   }
   int rc;
-  if ((rc = dev_alloc(&s->tri_pos, ntri, "triangles")) || (rc = dev_alloc(&s->tri_attr, ntri, "attributes")) ||
+  if ((rc = dev_alloc(&s->tri_pos, ntri, "triangles")) || (rc = dev_alloc(&s->world, ntri, "world-space triangles")) ||
+      (rc = dev_alloc(&s->tri_attr, ntri, "attributes")) ||
       (rc = dev_alloc(&s->objs, nobj, "descriptors")) || (rc = dev_alloc(&s->sh_counters, kCounterWords, "counters"))) {
     ocr_scene_destroy(s);
     return rc;
@@ -232,6 +234,8 @@ int ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri, cons
   if (ntri) e = hipMemcpy(s->tri_pos, pos.data(), sizeof(TriPos) * ntri, hipMemcpyHostToDevice);
   if (e == hipSuccess && ntri) e = hipMemcpy(s->tri_attr, attr.data(), sizeof(TriAttr) * ntri, hipMemcpyHostToDevice);
   if (e == hipSuccess && nobj) e = hipMemcpy(s->objs, od.data(), sizeof(ObjDesc) * nobj, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_world(s, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipEventCreate(&s->ev_sh0);
   if (e == hipSuccess) e = hipEventCreate(&s->ev_sh1);
   if (e != hipSuccess) {
@@ -253,6 +257,9 @@ int ocr_scene_update_objs(ocr_scene* s, uint32_t first, uint32_t n, const ocr_ob
   }
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   OCR_HIP_CHECK(hipMemcpy(s->objs + first, od.data(), sizeof(ObjDesc) * n, hipMemcpyHostToDevice));
+  // the world-space copy follows the objects before any later render
+  OCR_HIP_CHECK(launch_world(s, nullptr));
+  OCR_HIP_CHECK(hipStreamSynchronize(nullptr));
   return OCR_OK;
 }
 
@@ -267,6 +274,7 @@ void ocr_scene_destroy(ocr_scene* s) {
   if (!s) return;
   (void)hipSetDevice(s->ctx->device);
   dev_free(s->tri_pos);
+  dev_free(s->world);
   dev_free(s->tri_attr);
   dev_free(s->objs);
   free_atlas(s->atlas);
