@@ -12,28 +12,67 @@
 
 namespace ocr {
 
+// Pointers in the global address space: loads through them become
+// global_load with a scalar base and a 32-bit vector offset (no per-lane
+// 64-bit address arithmetic, no flat-address counters).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+  return (gptr<T>)(uintptr_t)p;
+}
+
+// Element `idx` with a 32-bit byte offset (every buffer the shading pass
+// reads is far below 4 GiB), which the scalar-base addressing form takes.
+template <typename T>
+__device__ __forceinline__ T ldg(gptr<const T> base, uint32_t idx) {
+  gptr<const char> p = (gptr<const char>)base + idx * (uint32_t)sizeof(T);
+  if constexpr (sizeof(T) <= 8) {
+    return *(gptr<const T>)p;
+  } else if constexpr (sizeof(T) % 16 == 0 && alignof(T) >= 16) {
+    // records: whole 16-byte words, assembled in registers
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 w[sizeof(T) / 16];
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 16; i++) w[i] = ((gptr<const u32x4>)p)[i];
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+  } else {
+    static_assert(sizeof(T) % 8 == 0, "record size");
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    u32x2 w[sizeof(T) / 8];
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 8; i++) w[i] = ((gptr<const u32x2>)p)[i];
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+  }
+}
+
 // What one shading block reads: its view, its scene and the frame's lists.
 struct ShadeArgs {
   ViewParams v;
-  const uint32_t* depth;
-  const unsigned long long* idkey;
-  const CutTri* cuts;
-  const TriPos* tri_pos;
-  const TriAttr* tri_attr;
-  const ObjDesc* objs;
-  const uint8_t* pages;
-  const uint32_t* nums;
-  const uint32_t* sizes;
+  gptr<const uint32_t> depth;
+  gptr<const unsigned long long> idkey;
+  gptr<const CutTri> cuts;
+  gptr<const TriPos> tri_pos;
+  gptr<const TriAttr> tri_attr;
+  gptr<const ObjDesc> objs;
+  gptr<const uint32_t> pages;     // RGBA8 texels as u32
+  gptr<const uint32_t> nums;
+  gptr<const uint32_t> sizes;
   uint32_t mip_start;
-  const ocr_light* lights;
-  const float* light_lin;
+  gptr<const ocr_light> lights;
+  gptr<const float> light_lin;
   uint32_t nlights;
-  const uint32_t* shadow_maps;
-  const uint32_t* static_maps;
+  gptr<const uint32_t> shadow_maps;
+  gptr<const uint32_t> static_maps;
   uint32_t L;
-  const uint32_t* counters;
-  uint32_t* rgba;
-  uint32_t* normals;
+  gptr<const uint32_t> counters;
+  gptr<uint32_t> rgba;
+  gptr<uint32_t> normals;
   const SinCos* face;
   uint32_t* tex_touch;      // count mode: 1 bit per atlas texel
   uint32_t* shadow_touch;   // count mode: 1 bit per dynamic cube-map texel
@@ -66,9 +105,9 @@ __device__ __forceinline__ f4 read_tex_pre(const ShadeArgs& a, float cx, float c
   cy = clampf(cy, 0.001f, width - 0.001f);
   int ix = (int)fmaf(tnumx, width, cx);
   int iy = (int)fmaf(tnumy, width, cy);
-  size_t t = (size_t)slice * 2048 * 2048 + (size_t)iy * 2048 + (size_t)ix;
+  const uint32_t t = (uint32_t)slice * (2048u * 2048u) + (uint32_t)iy * 2048u + (uint32_t)ix;
   touch<kCount>(a.tex_touch, t);
-  uint32_t c = *reinterpret_cast<const uint32_t*>(a.pages + t * 4);
+  uint32_t c = ldg(a.pages, t);
   return f4{(float)(c & 0xFF), (float)((c >> 8) & 0xFF), (float)((c >> 16) & 0xFF), (float)(c >> 24)};
 }
 
@@ -102,8 +141,8 @@ __device__ __forceinline__ float tmod(float v) {
 template <bool kCount>
 __device__ __forceinline__ f4 texture_filter_diff(const ShadeArgs& a, float vtx, float vty, float dx,
                                                   float dy, int tid2) {
-  int nv = (int)a.nums[tid2];
-  int tsize = (int)a.sizes[nv >> 16];
+  int nv = (int)ldg(a.nums, (uint32_t)tid2);
+  int tsize = (int)ldg(a.sizes, (uint32_t)(nv >> 16));
   float vmx = tmod(vtx), vmy = tmod(vty);
   float tx = dx * (float)tsize, ty = dy * (float)tsize;
   float worst = sqrtf(tx * tx + ty * ty);
@@ -112,8 +151,8 @@ __device__ __forceinline__ f4 texture_filter_diff(const ShadeArgs& a, float vtx,
   float fmd = wf - mip_lower;
   int tid_lower = mip_lower == 0 ? tid2 : (int)(mip_lower - 1 + (float)a.mip_start + (float)(tid2 * kMipLevels));
   int tid_higher = (int)(clampf(mip_lower, 0.f, kMipLevels - 1.f) + (float)a.mip_start + (float)(tid2 * kMipLevels));
-  int lnv = (int)a.nums[tid_lower], hnv = (int)a.nums[tid_higher];
-  float sl = (float)a.sizes[lnv >> 16], sh = (float)a.sizes[hnv >> 16];
+  int lnv = (int)ldg(a.nums, (uint32_t)tid_lower), hnv = (int)ldg(a.nums, (uint32_t)tid_higher);
+  float sl = (float)ldg(a.sizes, (uint32_t)(lnv >> 16)), sh = (float)ldg(a.sizes, (uint32_t)(hnv >> 16));
   f4 c1 = bilinear_pre<kCount>(a, vmx * sl, vmy * sl, lnv & 0xFFFF, lnv >> 16, sl);
   f4 c2 = bilinear_pre<kCount>(a, vmx * sh, vmy * sh, hnv & 0xFFFF, hnv >> 16, sh);
   const float i255 = 1.f / 255.f;
@@ -126,7 +165,7 @@ __device__ __forceinline__ float ssao(const ShadeArgs& a, int sx, int sy) {
   const uint32_t W = a.v.width, H = a.v.height;
   uint32_t seed2 = xorshift(wang_hash((uint32_t)sx + W * H * (uint32_t)sy));
   float foffset = (float)seed2 / kMulint;
-  float depth = ((float)a.depth[(size_t)sy * W + sx] / kMulint) * kDepthFar;
+  float depth = ((float)ldg(a.depth, (uint32_t)sy * W + (uint32_t)sx) / kMulint) * kDepthFar;
   float rad = kSsaoRad;
   rad += foffset / 2.f;
   float world_rad = rad * a.v.fov / depth;
@@ -146,7 +185,7 @@ __device__ __forceinline__ float ssao(const ShadeArgs& a, int sx, int sy) {
     for (int x = -2; x <= 2; x++) {
       float ox = roundf((float)x * world_rad);
       float wx = clampf(fsx + ox, 1.f, (float)W - 2.f);
-      float d2 = ((float)a.depth[(size_t)(int)wy * W + (int)wx] / kMulint) * kDepthFar;
+      float d2 = ((float)ldg(a.depth, (uint32_t)(int)wy * W + (uint32_t)(int)wx) / kMulint) * kDepthFar;
 #pragma unroll
       for (int z = 0; z < 5; z++) count += d2 > thr[z] ? 1u : 0u;
     }
@@ -167,7 +206,7 @@ __device__ __forceinline__ float bilerp(float cx, float cy, float v0, float v1, 
 // generate_hard_occlusion, cl2.cl:2536-2701 (SMOOTH_SHADOWS).
 template <bool kCount>
 __device__ __forceinline__ float hard_occlusion(const ShadeArgs& a, f3 lpos, f3 normal, f3 ptl,
-                                                const uint32_t* buf, uint32_t* touch_bits, int face,
+                                                gptr<const uint32_t> buf, uint32_t* touch_bits, int face,
                                                 f3 gpos, int shnum) {
   const uint32_t L = a.L;
   ptl = fast_normalize(ptl);
@@ -175,8 +214,7 @@ __device__ __forceinline__ float hard_occlusion(const ShadeArgs& a, f3 lpos, f3 
   float lf = (float)L;
   f3 post = depth_project(local_pos, lf, lf, lf / 2.0f);
   float dpth = post.z;
-  size_t map_off = ((size_t)face + (size_t)shnum * 6) * L * L;
-  const uint32_t* map = buf + map_off;
+  const uint32_t map_off = ((uint32_t)face + (uint32_t)shnum * 6u) * L * L;
   float px = clampf(post.x, 3.f, lf - 4.f);
   float py = clampf(post.y, 3.f, lf - 4.f);
   int ix = (int)px, iy = (int)py;
@@ -188,9 +226,9 @@ __device__ __forceinline__ float hard_occlusion(const ShadeArgs& a, f3 lpos, f3 
   for (int y = -1; y <= 2; y++)
 #pragma unroll
     for (int x = -1; x <= 2; x++) {
-      size_t t = (size_t)(iy + y) * L + (size_t)(ix + x);
+      const uint32_t t = (uint32_t)(iy + y) * L + (uint32_t)(ix + x);
       touch<kCount>(touch_bits, map_off + t);
-      float ldp = ((float)map[t] / kMulint) * kDepthFar;
+      float ldp = ((float)ldg(buf, map_off + t) / kMulint) * kDepthFar;
       cond |= (dpth > ldp + bias ? 1u : 0u) << ((y + 1) * 4 + (x + 1));
     }
   float shadow = 0.f;
@@ -211,6 +249,57 @@ __device__ __forceinline__ float hard_occlusion(const ShadeArgs& a, f3 lpos, f3 
 __device__ __forceinline__ f4 vcol(uint32_t c) {
   return f4{(float)(c >> 24) / 255.f, (float)((c >> 16) & 0xFF) / 255.f, (float)((c >> 8) & 0xFF) / 255.f,
             (float)(c & 0xFF) / 255.f};
+}
+
+// The view and scene records reach the kernel through a reference that may
+// point into a device table, so the compiler cannot see that their fields
+// are the same for the whole wavefront.  Moving them into scalar registers
+// lets light records come through the scalar cache and every buffer access
+// use a scalar base with a 32-bit vector offset.
+template <typename T>
+__device__ __forceinline__ gptr<T> rfl(gptr<T> p) {
+  const uint64_t v = (uint64_t)(uintptr_t)p;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return (gptr<T>)(uintptr_t)((uint64_t)hi << 32 | lo);
+}
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ float rfl(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ __forceinline__ void uniform(ShadeArgs& a) {
+  a.v.width = rfl(a.v.width);
+  a.v.height = rfl(a.v.height);
+  a.v.fov = rfl(a.v.fov);
+  a.v.linear = rfl(a.v.linear);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    a.v.cpos[i] = rfl(a.v.cpos[i]);
+    a.v.cam.s[i] = rfl(a.v.cam.s[i]);
+    a.v.cam.c[i] = rfl(a.v.cam.c[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) a.v.clear[i] = rfl(a.v.clear[i]);
+  a.depth = rfl(a.depth);
+  a.idkey = rfl(a.idkey);
+  a.cuts = rfl(a.cuts);
+  a.tri_pos = rfl(a.tri_pos);
+  a.tri_attr = rfl(a.tri_attr);
+  a.objs = rfl(a.objs);
+  a.pages = rfl(a.pages);
+  a.nums = rfl(a.nums);
+  a.sizes = rfl(a.sizes);
+  a.mip_start = rfl(a.mip_start);
+  a.lights = rfl(a.lights);
+  a.light_lin = rfl(a.light_lin);
+  a.nlights = rfl(a.nlights);
+  a.shadow_maps = rfl(a.shadow_maps);
+  a.static_maps = rfl(a.static_maps);
+  a.L = rfl(a.L);
+  a.counters = rfl(a.counters);
+  a.rgba = rfl(a.rgba);
+  a.normals = rfl(a.normals);
 }
 
 __device__ __forceinline__ const DevView& shade_view(const PassArgs& p, uint32_t& vi) {
@@ -234,29 +323,30 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
   const DevScene& S = sl.p.scenes ? sl.p.scenes[V.scene] : sl.p.scene0;
   ShadeArgs a;
   a.v = V.v;
-  a.depth = V.depth;
-  a.idkey = V.idkey;
-  a.cuts = sl.p.lists.cuts;
-  a.tri_pos = S.tri_pos;
-  a.tri_attr = S.tri_attr;
-  a.objs = S.objs;
-  a.pages = S.pages;
-  a.nums = S.nums;
-  a.sizes = S.sizes;
+  a.depth = to_global<const uint32_t>(V.depth);
+  a.idkey = to_global<const unsigned long long>(V.idkey);
+  a.cuts = to_global<const CutTri>(sl.p.lists.cuts);
+  a.tri_pos = to_global<const TriPos>(S.tri_pos);
+  a.tri_attr = to_global<const TriAttr>(S.tri_attr);
+  a.objs = to_global<const ObjDesc>(S.objs);
+  a.pages = to_global<const uint32_t>(reinterpret_cast<const uint32_t*>(S.pages));
+  a.nums = to_global<const uint32_t>(S.nums);
+  a.sizes = to_global<const uint32_t>(S.sizes);
   a.mip_start = S.mip_start;
-  a.lights = S.lights;
-  a.light_lin = S.light_lin;
+  a.lights = to_global<const ocr_light>(S.lights);
+  a.light_lin = to_global<const float>(S.light_lin);
   a.nlights = S.nlights;
-  a.shadow_maps = S.shadow_maps;
-  a.static_maps = S.static_maps;
+  a.shadow_maps = to_global<const uint32_t>(S.shadow_maps);
+  a.static_maps = to_global<const uint32_t>(S.static_maps);
   a.L = S.L;
-  a.counters = sl.p.lists.counters;
-  a.rgba = V.rgba;
-  a.normals = V.normals;
+  a.counters = to_global<const uint32_t>(sl.p.lists.counters);
+  a.rgba = to_global<uint32_t>(V.rgba);
+  a.normals = to_global<uint32_t>(V.normals);
   a.face = sl.face;
   a.tex_touch = sl.tex_touch;
   a.shadow_touch = sl.shadow_touch;
   a.static_touch = sl.static_touch;
+  uniform(a);
   const uint32_t W = a.v.width, H = a.v.height;
   const uint32_t lw = sl.p.tile_w_log2, tw = 1u << lw, th = 256u >> lw;
   const uint32_t tiles_x = (W + tw - 1) / tw;
@@ -265,9 +355,9 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
   const int y = (int)((tile / tiles_x) * th + (threadIdx.x >> lw));
   if (x >= (int)W || y >= (int)H) return;
   if (a.counters[kOverflow]) return;
-  const size_t pix = (size_t)y * W + x;
-  const uint32_t ft = a.depth[pix];
-  const unsigned long long key = a.idkey[pix];
+  const uint32_t pix = (uint32_t)y * W + (uint32_t)x;
+  const uint32_t ft = ldg(a.depth, pix);
+  const unsigned long long key = ldg(a.idkey, pix);
   if (ft == 0xFFFFFFFFu || key == 0ull) {
     a.rgba[pix] = quant8(a.v.clear[0]) | (quant8(a.v.clear[1]) << 8) | (quant8(a.v.clear[2]) << 16) |
                   (quant8(a.v.clear[3]) << 24);
@@ -275,12 +365,12 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
     return;
   }
   const float fov = a.v.fov;
-  const CutTri ct = a.cuts[(uint32_t)key];
+  const CutTri ct = ldg(a.cuts, (uint32_t)key);
   const uint32_t tri = (uint32_t)(key >> 32) - 1u;
   const uint32_t tri_global = tri >> 1;
-  const TriPos T = a.tri_pos[tri_global];
-  const TriAttr TA = a.tri_attr[tri_global];
-  const ObjDesc G = a.objs[ct.oid];
+  const TriPos T = ldg(a.tri_pos, tri_global);
+  const TriAttr TA = ldg(a.tri_attr, tri_global);
+  const ObjDesc G = ldg(a.objs, ct.oid);
   const f3 camera_pos = mk3(a.v.cpos[0], a.v.cpos[1], a.v.cpos[2]);
   const f3 p1 = mk3(T.p[0], T.p[1], T.p[2]) * G.scale;
   const f3 p2 = mk3(T.p[3], T.p[4], T.p[5]) * G.scale;
@@ -358,7 +448,7 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
   int shnum = 0, static_num = 0;
   f3 diffuse_sum = mk3(0, 0, 0), specular_sum = mk3(0, 0, 0);
   for (uint32_t i = 0; i < a.nlights; i++) {
-    const ocr_light l = a.lights[i];
+    const ocr_light l = ldg(a.lights, i);
     const f3 lpos = mk3(l.pos[0], l.pos[1], l.pos[2]);
     f3 ptl = lpos - global_position;
     float occlusion = 1;
@@ -376,7 +466,7 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
     illumination *= 1.f / (1.f - cutoff);
     if (illumination <= 0) continue;
     f3 light_col = mk3(l.col[0], l.col[1], l.col[2]);
-    if (linear) light_col = mk3(a.light_lin[4 * i], a.light_lin[4 * i + 1], a.light_lin[4 * i + 2]);
+    if (linear) light_col = mk3(ldg(a.light_lin, 4 * i), ldg(a.light_lin, 4 * i + 1), ldg(a.light_lin, 4 * i + 2));
     if (l.shadow && recv_dyn) {
       int face = cubeface(global_position, lpos);
       float dyn = 1.f - hard_occlusion<kCount>(a, lpos, normal, ptl, a.shadow_maps, a.shadow_touch, face,
