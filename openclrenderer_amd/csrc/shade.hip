@@ -186,8 +186,11 @@ __device__ __forceinline__ float ssao(const ShadeArgs& a, int sx, int sy) {
       float ox = roundf((float)x * world_rad);
       float wx = clampf(fsx + ox, 1.f, (float)W - 2.f);
       float d2 = ((float)ldg(a.depth, (uint32_t)(int)wy * W + (uint32_t)(int)wx) / kMulint) * kDepthFar;
-#pragma unroll
-      for (int z = 0; z < 5; z++) count += d2 > thr[z] ? 1u : 0u;
+      // thr[] is non-decreasing (rounding is monotone), so the number of
+      // thresholds below d2 comes from the middle one and the pair on its side
+      const bool hi = d2 > thr[2];
+      const float ta = hi ? thr[3] : thr[0], tb = hi ? thr[4] : thr[1];
+      count += (hi ? 3u : 0u) + (d2 > ta ? 1u : 0u) + (d2 > tb ? 1u : 0u);
     }
   }
   float acc = (float)count;
