@@ -219,12 +219,22 @@ struct Frame {
   unsigned long long* idkey = nullptr;
   uint32_t* rgba = nullptr;
   uint32_t* normals = nullptr;
-  CutTri* cuts = nullptr;
-  uint32_t* counters = nullptr;
-  uint32_t cap_cut = 0;
+  // Cut lists, double-buffered: render N+1's prearrange runs on the frame's
+  // side stream while render N's shading still reads the other list.
+  struct ListBuf {
+    CutTri* cuts = nullptr;
+    uint32_t* counters = nullptr;
+    uint32_t cap_cut = 0;
+    hipEvent_t ready = nullptr;      // prearrange into this list done (side stream)
+    hipEvent_t released = nullptr;   // the last shading pass reading it done
+    bool used = false;
+  };
+  ListBuf lb[2];
+  int cur = 0;                       // list of the last render
+  hipStream_t side = nullptr;
   uint32_t want_cut = 0;             // capacity set by ocr_frame_reserve (0 = automatic)
   bool profiling = false;
-  static constexpr int kEvents = 6;       // clear, prearrange, depth, id, shade, end
+  static constexpr int kEvents = 8;       // clear 0-1, depth 2-3, id 3-4, shade 4-5, prearrange 6-7 (side)
   static constexpr int kRing = 1024;      // renders averaged per profiling window
   std::vector<hipEvent_t> ev;             // kRing x kEvents, created on first use
   int n_profiled = 0;
@@ -262,6 +272,7 @@ hipError_t launch_atlas_upload(const Atlas& a, uint32_t tid, const uint8_t* src,
 hipError_t launch_mip(const Atlas& a, uint32_t src_id, uint32_t dst_id, uint32_t gw, uint32_t gh,
                       hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
+hipError_t launch_zero_counters(uint32_t* counters, hipStream_t s);
 hipError_t launch_light_prep(const ocr_light* lights, uint32_t n, float* lin, hipStream_t s);
 hipError_t launch_world(Scene* sc, hipStream_t s);
 

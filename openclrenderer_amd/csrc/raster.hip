@@ -341,7 +341,8 @@ __device__ __forceinline__ uint32_t view_of(const PassArgs& a, uint32_t slot) {
 // ------------------------------------------------------------------ clear
 __global__ void __launch_bounds__(256) k_clear(PassArgs a) {
   const uint32_t slot = blockIdx.x * 256u;
-  if (blockIdx.x == 0 && threadIdx.x < kCounterWords) a.lists.counters[threadIdx.x] = 0u;
+  // (a plain render zeroes its counters on its side stream instead)
+  if (a.lists.counters && blockIdx.x == 0 && threadIdx.x < kCounterWords) a.lists.counters[threadIdx.x] = 0u;
   const DevView& V = view_at(a, view_of<Range::Pix>(a, slot));
   const uint32_t i = slot - V.pix_begin + threadIdx.x;
   if (i >= V.v.width * V.v.height) return;
@@ -710,6 +711,11 @@ __global__ void __launch_bounds__(kPreThreads) k_prearrange_shadow(const TriPos*
 
 __global__ void k_zero_counters(uint32_t* c) {
   if (threadIdx.x < kCounterWords) c[threadIdx.x] = 0u;
+}
+
+hipError_t launch_zero_counters(uint32_t* counters, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, counters);
+  return hipGetLastError();
 }
 
 hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, uint32_t* map,
