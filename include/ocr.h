@@ -206,8 +206,15 @@ int  ocr_frame_reserve(ocr_frame* f, uint32_t max_cut_triangles);
 int  ocr_frame_set_profiling(ocr_frame* f, int enable);
 
 /* engine::draw_bulk_objs_n -> render_tris (engine.cpp:1794-2161):
- * prearrange -> depth (kernel1) -> id (kernel2) -> shade (kernel3), all on
- * `stream`, with no host round trip.  The view size must equal the frame's. */
+ * prearrange -> depth (kernel1) -> id (kernel2) -> shade (kernel3), with no
+ * host round trip.  The view size must equal the frame's.  A frame holds two
+ * output sets and each render writes the set the previous one did not, on a
+ * stream of the frame's own, so render N+1 can start while render N shades;
+ * `stream` waits for the render's end.  Render N+1 waits for the work on
+ * `stream` enqueued before render N was issued: enqueue what reads a
+ * render's outputs (ocr_frame_device_buffers changes with every render) on
+ * `stream` before the next ocr_render on the frame, and use one stream for
+ * the renders of one frame. */
 int  ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream);
 
 /* Replaces the CL/GL blit and its host read-back path

@@ -215,31 +215,46 @@ struct Scene {
 struct Frame {
   Ctx* ctx = nullptr;
   uint32_t width = 0, height = 0;
+  // Output and cut-list sets, double-buffered: render N uses set N mod 2 on
+  // its own stream, so it can overlap render N-1.  The pointers below alias
+  // the set of the last render.
+  struct OutBuf {
+    uint32_t* depth = nullptr;
+    unsigned long long* idkey = nullptr;
+    uint32_t* rgba = nullptr;
+    uint32_t* normals = nullptr;
+    CutTri* cuts = nullptr;
+    uint32_t* counters = nullptr;
+    uint32_t cap_cut = 0;
+  };
+  OutBuf out[2];
+  int cur = 0;
   uint32_t* depth = nullptr;
   unsigned long long* idkey = nullptr;
   uint32_t* rgba = nullptr;
   uint32_t* normals = nullptr;
-  // Cut lists, double-buffered: render N+1's prearrange runs on the frame's
-  // side stream while render N's shading still reads the other list.
-  struct ListBuf {
-    CutTri* cuts = nullptr;
-    uint32_t* counters = nullptr;
-    uint32_t cap_cut = 0;
-    hipEvent_t ready = nullptr;      // prearrange into this list done (side stream)
-    hipEvent_t released = nullptr;   // the last shading pass reading it done
-    bool used = false;
-  };
-  ListBuf lb[2];
-  int cur = 0;                       // list of the last render
-  hipStream_t side = nullptr;
+  hipStream_t ss[2] = {nullptr, nullptr};   // the streams renders of each set run on
+  hipEvent_t entry[2] = {nullptr, nullptr}; // caller's stream when a render of the set was issued
+  hipEvent_t done[2] = {nullptr, nullptr};  // a render of the set finished
+  bool entry_valid[2] = {false, false};
   uint32_t want_cut = 0;             // capacity set by ocr_frame_reserve (0 = automatic)
   bool profiling = false;
-  static constexpr int kEvents = 8;       // clear 0-1, depth 2-3, id 3-4, shade 4-5, prearrange 6-7 (side)
+  static constexpr int kEvents = 6;       // prearrange, clear, depth, id, shade boundaries
   static constexpr int kRing = 1024;      // renders averaged per profiling window
   std::vector<hipEvent_t> ev;             // kRing x kEvents, created on first use
   int n_profiled = 0;
   bool rendered = false;
+  void select(int b) {
+    cur = b;
+    depth = out[b].depth;
+    idkey = out[b].idkey;
+    rgba = out[b].rgba;
+    normals = out[b].normals;
+  }
 };
+
+// Allocates output set b of a frame (idempotent); counters zeroed.
+int frame_alloc_set(Frame* f, int b);
 
 // ---- error plumbing -------------------------------------------------------
 void set_error(const std::string& msg);

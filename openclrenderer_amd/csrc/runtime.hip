@@ -130,9 +130,22 @@ static PassArgs single_pass(const Scene* s, const ocr_view* view, Frame* f) {
   a.pix_slots = round256((uint64_t)view->width * view->height);
   a.tile_w_log2 = 4;
   a.tiles = view_tiles(view->width, view->height, a.tile_w_log2);
-  const Frame::ListBuf& lb = f->lb[f->cur];
-  a.lists = make_lists(lb.cuts, lb.counters, lb.cap_cut);
+  const Frame::OutBuf& ob = f->out[f->cur];
+  a.lists = make_lists(ob.cuts, ob.counters, ob.cap_cut);
   return a;
+}
+
+int frame_alloc_set(Frame* f, int b) {
+  Frame::OutBuf& o = f->out[b];
+  if (o.depth) return OCR_OK;
+  const size_t npix = (size_t)f->width * f->height;
+  int rc;
+  if ((rc = dev_alloc(&o.depth, npix, "depth")) || (rc = dev_alloc(&o.idkey, npix, "id buffer")) ||
+      (rc = dev_alloc(&o.rgba, npix, "rgba")) || (rc = dev_alloc(&o.normals, npix, "normals")) ||
+      (rc = dev_alloc(&o.counters, kCounterWords, "counters")))
+    return rc;
+  OCR_HIP_CHECK(hipMemset(o.counters, 0, 4 * kCounterWords));
+  return OCR_OK;
 }
 
 }  // namespace ocr
@@ -508,21 +521,17 @@ int ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** 
   f->ctx = ctx;
   f->width = width;
   f->height = height;
-  size_t npix = (size_t)width * height;
-  int rc;
-  if ((rc = dev_alloc(&f->depth, npix, "depth")) || (rc = dev_alloc(&f->idkey, npix, "id buffer")) ||
-      (rc = dev_alloc(&f->rgba, npix, "rgba")) || (rc = dev_alloc(&f->normals, npix, "normals")) ||
-      (rc = dev_alloc(&f->lb[0].counters, kCounterWords, "counters")) ||
-      (rc = dev_alloc(&f->lb[1].counters, kCounterWords, "counters"))) {
+  int rc = frame_alloc_set(f, 0);
+  if (rc) {
     ocr_frame_destroy(f);
     return rc;
   }
-  hipError_t e = hipMemset(f->lb[0].counters, 0, 4 * kCounterWords);
-  if (e == hipSuccess) e = hipMemset(f->lb[1].counters, 0, 4 * kCounterWords);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&f->side, hipStreamNonBlocking);
+  f->select(0);
+  hipError_t e = hipSuccess;
   for (int b = 0; b < 2 && e == hipSuccess; b++) {
-    e = hipEventCreateWithFlags(&f->lb[b].ready, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&f->lb[b].released, hipEventDisableTiming);
+    e = hipStreamCreateWithFlags(&f->ss[b], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&f->entry[b], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&f->done[b], hipEventDisableTiming);
   }
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
@@ -536,17 +545,19 @@ int ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** 
 void ocr_frame_destroy(ocr_frame* f) {
   if (!f) return;
   (void)hipSetDevice(f->ctx->device);
-  dev_free(f->depth);
-  dev_free(f->idkey);
-  dev_free(f->rgba);
-  dev_free(f->normals);
-  for (Frame::ListBuf& lb : f->lb) {
-    dev_free(lb.cuts);
-    dev_free(lb.counters);
-    if (lb.ready) (void)hipEventDestroy(lb.ready);
-    if (lb.released) (void)hipEventDestroy(lb.released);
+  for (Frame::OutBuf& o : f->out) {
+    dev_free(o.depth);
+    dev_free(o.idkey);
+    dev_free(o.rgba);
+    dev_free(o.normals);
+    dev_free(o.cuts);
+    dev_free(o.counters);
   }
-  if (f->side) (void)hipStreamDestroy(f->side);
+  for (int b = 0; b < 2; b++) {
+    if (f->ss[b]) (void)hipStreamDestroy(f->ss[b]);
+    if (f->entry[b]) (void)hipEventDestroy(f->entry[b]);
+    if (f->done[b]) (void)hipEventDestroy(f->done[b]);
+  }
   for (hipEvent_t e : f->ev)
     if (e) (void)hipEventDestroy(e);
   delete f;
@@ -568,9 +579,9 @@ int ocr_frame_reserve(ocr_frame* f, uint32_t max_cut_triangles) {
   if (!f || max_cut_triangles == 0) return fail(OCR_ERR_INVALID, "ocr_frame_reserve: bad argument");
   OCR_HIP_CHECK(hipSetDevice(f->ctx->device));
   OCR_HIP_CHECK(hipDeviceSynchronize());
-  for (Frame::ListBuf& lb : f->lb) {
-    dev_free(lb.cuts);
-    lb.cap_cut = 0;
+  for (Frame::OutBuf& o : f->out) {
+    dev_free(o.cuts);
+    o.cap_cut = 0;
   }
   f->want_cut = max_cut_triangles;
   return OCR_OK;
@@ -579,7 +590,7 @@ int ocr_frame_reserve(ocr_frame* f, uint32_t max_cut_triangles) {
 // A cut list holds at most 2 pieces per scene triangle; unless the caller
 // reserved a capacity it is sized for that, so it never overflows.
 static int ensure_frame_list(ocr_frame* f, int b, uint32_t ntri) {
-  Frame::ListBuf& lb = f->lb[b];
+  Frame::OutBuf& lb = f->out[b];
   uint64_t want = f->want_cut ? (uint64_t)f->want_cut : (uint64_t)ntri * 2 + 64;
   want = std::min<uint64_t>(want, 0xFFFFFFF0ull);
   if (lb.cuts && lb.cap_cut == want) return OCR_OK;
@@ -603,9 +614,10 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
     return fail(OCR_ERR_INVALID, "ocr_render: an object's texture id is outside the atlas");
   if (!s->lights && s->nlights) return fail(OCR_ERR_INVALID, "ocr_render: lights missing");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
-  // this render's cut list: the one the previous render did not use
+  // this render's buffer set: the one the previous render did not use
   const int b = f->cur ^ 1;
-  int rc = ensure_frame_list(f, b, s->ntri);
+  int rc = frame_alloc_set(f, b);
+  if (!rc) rc = ensure_frame_list(f, b, s->ntri);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (!s->lights) {
@@ -615,36 +627,34 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   }
   const bool prof = f->profiling && f->n_profiled < Frame::kRing;
   hipEvent_t* ev = prof ? &f->ev[(size_t)f->n_profiled * Frame::kEvents] : nullptr;
-  f->cur = b;
-  Frame::ListBuf& lb = f->lb[b];
+  hipStream_t sb = f->ss[b];
+  // Render N runs on the stream of its set once render N-2 (same buffers)
+  // and everything enqueued on the caller's stream before render N-1 was
+  // issued have completed (the event recorded at that point), so it can
+  // overlap render N-1; the caller's stream then waits for it.  The first
+  // render follows the caller's stream.
+  OCR_HIP_CHECK(hipEventRecord(f->entry[b], st));
+  OCR_HIP_CHECK(hipStreamWaitEvent(sb, f->entry_valid[b ^ 1] ? f->entry[b ^ 1] : f->entry[b], 0));
+  f->entry_valid[b] = true;
+  f->select(b);
+  const Frame::OutBuf& o = f->out[b];
   const PassArgs pa = single_pass(s, view, f);
-  // Prearrange reads only the scene (updated synchronously) and writes list
-  // b, which the shading pass of the render before last may still read.  On
-  // the side stream it waits for that pass alone, so it runs beside the
-  // previous render's passes — mostly its depth pass, whose VALUs wait on
-  // atomics — instead of after them.  (Held back to run beside the VALU-bound
-  // shading pass instead, it only stretches.)
-  if (lb.used) OCR_HIP_CHECK(hipStreamWaitEvent(f->side, lb.released, 0));
-  OCR_HIP_CHECK(launch_zero_counters(lb.counters, f->side));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[6], f->side));
-  OCR_HIP_CHECK(launch_prearrange(pa, f->side));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[7], f->side));
-  OCR_HIP_CHECK(hipEventRecord(lb.ready, f->side));
   PassArgs pc = pa;
-  pc.lists.counters = nullptr;   // zeroed on the side stream above
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[0], st));
-  OCR_HIP_CHECK(launch_clear(pc, st));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[1], st));
-  OCR_HIP_CHECK(hipStreamWaitEvent(st, lb.ready, 0));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[2], st));
-  OCR_HIP_CHECK(launch_depth(pa, kRasterBlocksPerCu, st));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], st));
-  OCR_HIP_CHECK(launch_id(pa, kRasterBlocksPerCu, st));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], st));
-  OCR_HIP_CHECK(launch_shade(pa, s->ctx->face_sc, st));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[5], st));
-  OCR_HIP_CHECK(hipEventRecord(lb.released, st));
-  lb.used = true;
+  pc.lists.counters = nullptr;   // zeroed before prearrange below
+  OCR_HIP_CHECK(launch_zero_counters(o.counters, sb));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[0], sb));
+  OCR_HIP_CHECK(launch_prearrange(pa, sb));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[1], sb));
+  OCR_HIP_CHECK(launch_clear(pc, sb));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[2], sb));
+  OCR_HIP_CHECK(launch_depth(pa, kRasterBlocksPerCu, sb));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], sb));
+  OCR_HIP_CHECK(launch_id(pa, kRasterBlocksPerCu, sb));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
+  OCR_HIP_CHECK(launch_shade(pa, s->ctx->face_sc, sb));
+  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[5], sb));
+  OCR_HIP_CHECK(hipEventRecord(f->done[b], sb));
+  OCR_HIP_CHECK(hipStreamWaitEvent(st, f->done[b], 0));
   if (prof) f->n_profiled++;
   f->rendered = true;
   return OCR_OK;
@@ -655,17 +665,16 @@ int ocr_frame_get_stats(ocr_frame* f, void* stream, ocr_frame_stats* out) {
   OCR_HIP_CHECK(hipSetDevice(f->ctx->device));
   OCR_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
   uint32_t c[kCounterWords];
-  OCR_HIP_CHECK(hipMemcpy(c, f->lb[f->cur].counters, sizeof(c), hipMemcpyDeviceToHost));
+  OCR_HIP_CHECK(hipMemcpy(c, f->out[f->cur].counters, sizeof(c), hipMemcpyDeviceToHost));
   memset(out, 0, sizeof(*out));
   out->n_cut = c[kPacked + 1];
   out->n_frag = c[kFragCount];
   out->overflow = c[kOverflow] ? 1u : 0u;
   out->n_profiled = (uint32_t)f->n_profiled;
   if (f->n_profiled > 0) {
-    // clear, prearrange (side stream), depth, id, shade, and the render
-    // stream's span from the clear to the end of shading
+    // clear, prearrange, depth, id, shade, and the render's whole span
     double sum[6] = {0, 0, 0, 0, 0, 0};
-    const int from[6] = {0, 6, 2, 3, 4, 0}, to[6] = {1, 7, 3, 4, 5, 5};
+    const int from[6] = {1, 0, 2, 3, 4, 0}, to[6] = {2, 1, 3, 4, 5, 5};
     for (int r = 0; r < f->n_profiled; r++) {
       hipEvent_t* ev = &f->ev[(size_t)r * Frame::kEvents];
       float ms = 0.f;
@@ -736,7 +745,7 @@ int ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id, uint8_t* rgb
   hipStream_t st = (hipStream_t)stream;
   OCR_HIP_CHECK(hipStreamSynchronize(st));
   uint32_t c[kCounterWords];
-  OCR_HIP_CHECK(hipMemcpy(c, f->lb[f->cur].counters, sizeof(c), hipMemcpyDeviceToHost));
+  OCR_HIP_CHECK(hipMemcpy(c, f->out[f->cur].counters, sizeof(c), hipMemcpyDeviceToHost));
   if (c[kOverflow] & kOverflowSamples)
     return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_read: the render needs more than 2^31 - 1 raster samples");
   if (c[kOverflow]) {

@@ -148,16 +148,13 @@ int take_frame(ocr_server* s, uint32_t w, uint32_t h, ocr_frame** out) {
   f->ctx = s->ctx;
   f->width = w;
   f->height = h;
-  const size_t npix = (size_t)w * h;
-  hipError_t e = hipMalloc((void**)&f->depth, 4 * npix);
-  if (e == hipSuccess) e = hipMalloc((void**)&f->idkey, 8 * npix);
-  if (e == hipSuccess) e = hipMalloc((void**)&f->rgba, 4 * npix);
-  if (e == hipSuccess) e = hipMalloc((void**)&f->normals, 4 * npix);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
+  // one output set: server frames are only rendered in batches
+  const int rc = frame_alloc_set(f, 0);
+  if (rc) {
     ocr_frame_destroy(f);
-    return fail(OCR_ERR_OOM, std::string("server frame: ") + hipGetErrorString(e));
+    return rc;
   }
+  f->select(0);
   std::lock_guard<std::mutex> g(s->mu);
   s->all_frames.push_back(f);
   *out = f;
