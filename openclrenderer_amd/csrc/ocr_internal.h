@@ -208,8 +208,10 @@ struct Scene {
   CutTri* sh_cuts = nullptr;
   uint32_t* sh_counters = nullptr;
   uint32_t sh_cap_cut = 0;
-  hipEvent_t ev_sh0 = nullptr, ev_sh1 = nullptr;
-  float ms_shadow = 0.f;
+  // recorded on the stream of the last shadow pass after its last launch;
+  // renders wait for it (they run on streams of their frame)
+  hipEvent_t written = nullptr;
+  bool written_valid = false;
 };
 
 struct Frame {

@@ -250,8 +250,7 @@ int ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri, cons
   if (e == hipSuccess && nobj) e = hipMemcpy(s->objs, od.data(), sizeof(ObjDesc) * nobj, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = launch_world(s, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e == hipSuccess) e = hipEventCreate(&s->ev_sh0);
-  if (e == hipSuccess) e = hipEventCreate(&s->ev_sh1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&s->written, hipEventDisableTiming);
   if (e != hipSuccess) {
     ocr_scene_destroy(s);
     return fail(OCR_ERR_HIP, std::string("ocr_scene_create: ") + hipGetErrorString(e));
@@ -270,6 +269,8 @@ int ocr_scene_update_objs(ocr_scene* s, uint32_t first, uint32_t n, const ocr_ob
     s->max_tid = std::max(s->max_tid, objs[o].tid);
   }
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  // renders run on streams of their frames: let those reading the scene end
+  OCR_HIP_CHECK(hipDeviceSynchronize());
   OCR_HIP_CHECK(hipMemcpy(s->objs + first, od.data(), sizeof(ObjDesc) * n, hipMemcpyHostToDevice));
   // the world-space copy follows the objects before any later render
   OCR_HIP_CHECK(launch_world(s, nullptr));
@@ -298,8 +299,7 @@ void ocr_scene_destroy(ocr_scene* s) {
   dev_free(s->static_maps);
   dev_free(s->sh_cuts);
   dev_free(s->sh_counters);
-  if (s->ev_sh0) (void)hipEventDestroy(s->ev_sh0);
-  if (s->ev_sh1) (void)hipEventDestroy(s->ev_sh1);
+  if (s->written) (void)hipEventDestroy(s->written);
   delete s;
 }
 
@@ -313,6 +313,7 @@ static uint32_t pow2_ceil(uint32_t v) {
 int ocr_atlas_upload(ocr_scene* s, const ocr_tex_src* texs, uint32_t ntex) {
   if (!s || (ntex && !texs)) return fail(OCR_ERR_INVALID, "ocr_atlas_upload: bad argument");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  OCR_HIP_CHECK(hipDeviceSynchronize());   // renders still reading the atlas end first
   // texture_context.cpp:94-165: count slots per power-of-two size class
   // (base + 4 mips per texture) and cut them into 2048^2 pages, smallest
   // class first.
@@ -420,6 +421,7 @@ int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_
   if (light_buffer_dim < 8 || (uint64_t)light_buffer_dim * light_buffer_dim > kMaxTargetPixels)
     return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light buffer dimension must be 8..4000");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  OCR_HIP_CHECK(hipDeviceSynchronize());   // renders still reading the lights end first
   dev_free(s->lights);
   dev_free(s->light_lin);
   dev_free(s->shadow_maps);
@@ -491,6 +493,8 @@ int ocr_shadow_pass(ocr_scene* s, void* stream) {
     if (l.shadow && l.is_static) kk++;
   }
   s->static_dirty = false;
+  OCR_HIP_CHECK(hipEventRecord(s->written, st));
+  s->written_valid = true;
   return OCR_OK;
 }
 
@@ -651,6 +655,9 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], sb));
   OCR_HIP_CHECK(launch_id(pa, kRasterBlocksPerCu, sb));
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
+  // Shading reads the shadow maps: it waits for the scene's last shadow
+  // pass, wherever that was enqueued (the raster passes do not read them).
+  if (s->written_valid) OCR_HIP_CHECK(hipStreamWaitEvent(sb, s->written, 0));
   OCR_HIP_CHECK(launch_shade(pa, s->ctx->face_sc, sb));
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[5], sb));
   OCR_HIP_CHECK(hipEventRecord(f->done[b], sb));

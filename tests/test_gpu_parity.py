@@ -156,6 +156,30 @@ def test_repeat_renders_are_identical_and_objects_update(gpu_ctx, oracle):
     assert not np.array_equal(a["depth"], c["depth"])
 
 
+def test_back_to_back_renders_follow_scene_writes(gpu_ctx, oracle):
+    """Renders of one frame overlap each other; each still sees the object
+    update and the shadow pass issued before it, and a render of another view
+    queued behind it leaves the frame holding the last view."""
+    sc, view = scenes.config1(shadow=1)
+    g = runtime.GpuScene(gpu_ctx, sc)
+    fr = runtime.Frame(gpu_ctx, 800, 600)
+    other = scenes.make_view(800, 600, (40.0, 10.0, -900.0), c_rot=(0.05, 0.1, 0.0), hfov=70.0)
+    objs = sc.objs.copy()
+    for x in (0.0, 140.0, -90.0):
+        objs["world_pos"][0, :3] = [x, -40.0, 60.0]
+        g.update_objs(0, objs[:1])
+        g.shadow_pass()
+        fr.render(g, other)
+        fr.render(g, view)
+    out = fr.read()
+    sc2 = scenes.Scene(sc.tris, objs, sc.lights, sc.textures, sc.light_buffer_dim)
+    maps = oracle.shadow_maps(sc2)
+    assert np.array_equal(g.read_shadow_maps()[0], maps[0])
+    assert_frame_parity(out, oracle.render(sc2, view, maps=maps), "last view")
+    fr.render(g, other)
+    assert_frame_parity(fr.read(), oracle.render(sc2, other, maps=maps), "other view")
+
+
 def test_invalid_inputs_are_rejected(gpu_ctx):
     sc, view = scenes.config1()
     g = runtime.GpuScene(gpu_ctx, sc)
