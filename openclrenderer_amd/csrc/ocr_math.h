@@ -49,6 +49,25 @@ __device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y 
 __device__ __forceinline__ f3 cross(f3 a, f3 b) {
   return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
+// a / b, correctly rounded, from y = RN(1/b) (Markstein's correction: the
+// residual a - b*q is exact under fma, and one fma step from a faithful
+// quotient lands on the rounded one).  Valid when a is +0 or a, b and a/b
+// are normal binary32 numbers; for the constant divisors below it is checked
+// against IEEE division for every finite a >= 0.  Three instructions in
+// place of the eleven of a full correctly rounded division.
+__device__ __forceinline__ float div_rcp(float a, float b, float y) {
+  const float q = a * y;
+  const float r = fmaf(-q, b, a);
+  return fmaf(r, y, q);
+}
+constexpr float kInv9 = 1.0f / 9.0f;       // RN(1/9), folded by the compiler
+constexpr float kInv125 = 1.0f / 125.0f;
+constexpr float kInvSsaoDiv = 1.0f / kSsaoDiv;
+constexpr float kInv255 = 1.0f / 255.0f;
+// A uniform divisor d takes the fast path when it and 1/d are comfortably
+// normal, which keeps every quotient the path forms in the normal range.
+__device__ __forceinline__ bool rcp_safe(float d) { return fabsf(d) >= 0x1p-60f && fabsf(d) <= 0x1p60f; }
+
 __device__ __forceinline__ f3 fast_normalize(f3 v) { return v * (1.0f / sqrtf(dot(v, v))); }
 __device__ __forceinline__ float fast_length(f3 v) { return sqrtf(dot(v, v)); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }

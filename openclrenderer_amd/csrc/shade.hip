@@ -194,8 +194,8 @@ __device__ __forceinline__ float ssao(const ShadeArgs& a, int sx, int sy) {
     }
   }
   float acc = (float)count;
-  acc /= 125.f;
-  return 1.f - (1.f - acc) / kSsaoDiv;
+  acc = div_rcp(acc, 125.f, kInv125);
+  return 1.f - div_rcp(1.f - acc, kSsaoDiv, kInvSsaoDiv);
 }
 
 // bilinear_interpolate, cl2.cl:2286-2302.
@@ -206,52 +206,87 @@ __device__ __forceinline__ float bilerp(float cx, float cy, float v0, float v1, 
   return (v0 * bx + v1 * ux) * by + (v2 * bx + v3 * ux) * uy;
 }
 
-// generate_hard_occlusion, cl2.cl:2536-2701 (SMOOTH_SHADOWS).
-template <bool kCount>
-__device__ __forceinline__ float hard_occlusion(const ShadeArgs& a, f3 lpos, f3 normal, f3 ptl,
-                                                gptr<const uint32_t> buf, uint32_t* touch_bits, int face,
-                                                f3 gpos, int shnum) {
+// generate_hard_occlusion, cl2.cl:2536-2701 (SMOOTH_SHADOWS), in two
+// parts: where the pixel falls in the light's cube face and the depth bias
+// (the same for the static and the dynamic map of a light), then the taps.
+struct ShadowSetup {
+  float px, py;      // clamped position on the face
+  float dpth;        // the pixel's depth from the light
+  float bias;
+  uint32_t face_off; // (iy - 1) * L + (ix - 1) within the face
+  int face;
+};
+
+// float(pi/4) rounded up: for every binary32 a in [this, rational_acos(0.05)]
+// tan_f(a) >= 1, so the bias saturates at SHADOWBIAS without evaluating tan.
+constexpr float kTanOneFrom = 0x1.921fb6p-1f;
+
+__device__ __forceinline__ ShadowSetup shadow_setup(const ShadeArgs& a, f3 lpos, f3 normal, f3 ptl_n, f3 gpos) {
+  ShadowSetup s;
+  s.face = cubeface(gpos, lpos);
   const uint32_t L = a.L;
-  ptl = fast_normalize(ptl);
-  f3 local_pos = rot(gpos, lpos, a.face[face]);
+  f3 local_pos = rot(gpos, lpos, a.face[s.face]);
   float lf = (float)L;
   f3 post = depth_project(local_pos, lf, lf, lf / 2.0f);
-  float dpth = post.z;
-  const uint32_t map_off = ((uint32_t)face + (uint32_t)shnum * 6u) * L * L;
-  float px = clampf(post.x, 3.f, lf - 4.f);
-  float py = clampf(post.y, 3.f, lf - 4.f);
-  int ix = (int)px, iy = (int)py;
-  float acos_res = rational_acos(clampf(dot(normal, ptl), 0.05f, 0.95f));
-  float bias = kShadowBias * tan_f(acos_res);
-  bias = clampf(bias, 0.1f * kShadowBias, (float)kShadowBias);
+  s.dpth = post.z;
+  s.px = clampf(post.x, 3.f, lf - 4.f);
+  s.py = clampf(post.y, 3.f, lf - 4.f);
+  const int ix = (int)s.px, iy = (int)s.py;
+  s.face_off = (uint32_t)(iy - 1) * L + (uint32_t)(ix - 1);
+  float acos_res = rational_acos(clampf(dot(normal, ptl_n), 0.05f, 0.95f));
+  if (acos_res >= kTanOneFrom) {
+    s.bias = (float)kShadowBias;
+  } else {
+    float bias = kShadowBias * tan_f(acos_res);
+    s.bias = clampf(bias, 0.1f * kShadowBias, (float)kShadowBias);
+  }
+  return s;
+}
+
+// Returns the shadow term (0 = lit).  A 4x4 block of depth compares feeds 9
+// bilinear filters of 0/1 values; when all 16 compares agree the filters are
+// evaluated with those constants (all zero: the sum is exactly 0).
+template <bool kCount>
+__device__ __forceinline__ float shadow_taps(const ShadeArgs& a, const ShadowSetup& s,
+                                             gptr<const uint32_t> buf, uint32_t* touch_bits, int shnum) {
+  const uint32_t L = a.L;
+  const uint32_t map_off = ((uint32_t)s.face + (uint32_t)shnum * 6u) * L * L + s.face_off;
   uint32_t cond = 0;   // 4x4 taps, bit (y+1)*4 + (x+1)
 #pragma unroll
-  for (int y = -1; y <= 2; y++)
+  for (int y = 0; y < 4; y++)
 #pragma unroll
-    for (int x = -1; x <= 2; x++) {
-      const uint32_t t = (uint32_t)(iy + y) * L + (uint32_t)(ix + x);
-      touch<kCount>(touch_bits, map_off + t);
-      float ldp = ((float)ldg(buf, map_off + t) / kMulint) * kDepthFar;
-      cond |= (dpth > ldp + bias ? 1u : 0u) << ((y + 1) * 4 + (x + 1));
+    for (int x = 0; x < 4; x++) {
+      const uint32_t t = map_off + (uint32_t)y * L + (uint32_t)x;
+      touch<kCount>(touch_bits, t);
+      float ldp = ((float)ldg(buf, t) / kMulint) * kDepthFar;
+      cond |= (s.dpth > ldp + s.bias ? 1u : 0u) << (y * 4 + x);
     }
+  if (cond == 0u) return 0.f;
   float shadow = 0.f;
+  if (cond == 0xFFFFu) {
 #pragma unroll
-  for (int y = -1; y <= 1; y++)
+    for (int y = -1; y <= 1; y++)
 #pragma unroll
-    for (int x = -1; x <= 1; x++) {
-      float v0 = (float)((cond >> ((y + 1) * 4 + x + 1)) & 1u);
-      float v1 = (float)((cond >> ((y + 1) * 4 + x + 2)) & 1u);
-      float v2 = (float)((cond >> ((y + 2) * 4 + x + 1)) & 1u);
-      float v3 = (float)((cond >> ((y + 2) * 4 + x + 2)) & 1u);
-      shadow += bilerp(px + 0.5f + (float)x, py + 0.5f + (float)y, v0, v1, v2, v3);
-    }
-  shadow /= 9.f;
-  return shadow;
+      for (int x = -1; x <= 1; x++)
+        shadow += bilerp(s.px + 0.5f + (float)x, s.py + 0.5f + (float)y, 1.f, 1.f, 1.f, 1.f);
+  } else {
+#pragma unroll
+    for (int y = -1; y <= 1; y++)
+#pragma unroll
+      for (int x = -1; x <= 1; x++) {
+        float v0 = (float)((cond >> ((y + 1) * 4 + x + 1)) & 1u);
+        float v1 = (float)((cond >> ((y + 1) * 4 + x + 2)) & 1u);
+        float v2 = (float)((cond >> ((y + 2) * 4 + x + 1)) & 1u);
+        float v3 = (float)((cond >> ((y + 2) * 4 + x + 2)) & 1u);
+        shadow += bilerp(s.px + 0.5f + (float)x, s.py + 0.5f + (float)y, v0, v1, v2, v3);
+      }
+  }
+  return div_rcp(shadow, 9.f, kInv9);
 }
 
 __device__ __forceinline__ f4 vcol(uint32_t c) {
-  return f4{(float)(c >> 24) / 255.f, (float)((c >> 16) & 0xFF) / 255.f, (float)((c >> 8) & 0xFF) / 255.f,
-            (float)(c & 0xFF) / 255.f};
+  return f4{div_rcp((float)(c >> 24), 255.f, kInv255), div_rcp((float)((c >> 16) & 0xFF), 255.f, kInv255),
+            div_rcp((float)((c >> 8) & 0xFF), 255.f, kInv255), div_rcp((float)(c & 0xFF), 255.f, kInv255)};
 }
 
 // The view and scene records reach the kernel through a reference that may
@@ -382,8 +417,12 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
   const f4 q = f4{G.q[0], G.q[1], G.q[2], G.q[3]};
 
   const float actual_depth = ((float)ft / kMulint) * kDepthFar;
-  f3 local_position = mk3(((float)x - (float)W / 2.0f) * actual_depth / fov,
-                          ((float)y - (float)H / 2.0f) * actual_depth / fov, actual_depth);
+  // quotients by the view's fov from its correctly rounded reciprocal
+  const bool fov_fast = rcp_safe(fov);
+  const float ifov = fov_fast ? 1.f / fov : 0.f;
+  auto by_fov = [&](float n) { return fov_fast ? div_rcp(n, fov, ifov) : n / fov; };
+  f3 local_position = mk3(by_fov(((float)x - (float)W / 2.0f) * actual_depth),
+                          by_fov(((float)y - (float)H / 2.0f) * actual_depth), actual_depth);
   f3 global_position = back_rot(local_position, a.v.cam) + camera_pos;
   f3 object_local = back_rot_quat(global_position - wpos, q);
   float l1, l2, l3;
@@ -403,8 +442,14 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
     const float DA = ct.da, DB = ct.db, DC = ct.dc;
     float dmx = fmaf(DA, fx + 1, fmaf(DB, fy, DC));
     float dmy = fmaf(DA, fx, fmaf(DB, fy + 1, DC));
-    f3 lmx = mk3((fx + 1 - (float)W / 2.f) / fov, (fy - (float)H / 2.f) / fov, 1) / dmx;
-    f3 lmy = mk3((fx - (float)W / 2.f) / fov, (fy + 1 - (float)H / 2.f) / fov, 1) / dmy;
+    // (n / fov) / d for both components, and 1 / d, which is the correctly
+    // rounded reciprocal the other two quotients are formed from
+    auto over = [](float n0, float n1, float d) {
+      const float y = 1.f / d;
+      return rcp_safe(d) ? mk3(div_rcp(n0, d, y), div_rcp(n1, d, y), y) : mk3(n0 / d, n1 / d, y);
+    };
+    f3 lmx = over(by_fov(fx + 1 - (float)W / 2.f), by_fov(fy - (float)H / 2.f), dmx);
+    f3 lmy = over(by_fov(fx - (float)W / 2.f), by_fov(fy + 1 - (float)H / 2.f), dmy);
     f3 gmx = back_rot_quat((back_rot(lmx, a.v.cam) + camera_pos) - wpos, q);
     f3 gmy = back_rot_quat((back_rot(lmy, a.v.cam) + camera_pos) - wpos, q);
     float lx1, lx2, lx3, ly1, ly2, ly3;
@@ -455,14 +500,21 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
     const f3 lpos = mk3(l.pos[0], l.pos[1], l.pos[2]);
     f3 ptl = lpos - global_position;
     float occlusion = 1;
-    if (l.shadow && l.is_static) {
-      int face = cubeface(global_position, lpos);
-      occlusion = 1.f - hard_occlusion<kCount>(a, lpos, normal, ptl, a.static_maps, a.static_touch, face,
-                                              global_position, static_num);
+    // the static and the dynamic map of a light share the face, position
+    // and bias; both use the normalised point-to-light vector shading uses
+    const bool stat = l.shadow && l.is_static;
+    f3 ptl_n;
+    ShadowSetup ss;
+    if (stat) {
+      ptl_n = fast_normalize(ptl);
+      ss = shadow_setup(a, lpos, normal, ptl_n, global_position);
+      occlusion = 1.f - shadow_taps<kCount>(a, ss, a.static_maps, a.static_touch, static_num);
       static_num++;
     }
     float distance = fast_length(ptl);
-    float t = (distance / l.radius) + 1;
+    // 1/radius from the light upload (0 when the radius is out of range)
+    const float inv_r = rfl(ldg(a.light_lin, 4 * i + 3));
+    float t = (inv_r != 0.f ? div_rcp(distance, l.radius, inv_r) : distance / l.radius) + 1;
     float illumination = l.brightness / (t * t);
     const float cutoff = 0.1f;
     illumination -= cutoff;
@@ -470,14 +522,14 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
     if (illumination <= 0) continue;
     f3 light_col = mk3(l.col[0], l.col[1], l.col[2]);
     if (linear) light_col = mk3(ldg(a.light_lin, 4 * i), ldg(a.light_lin, 4 * i + 1), ldg(a.light_lin, 4 * i + 2));
+    if (!stat) ptl_n = fast_normalize(ptl);
     if (l.shadow && recv_dyn) {
-      int face = cubeface(global_position, lpos);
-      float dyn = 1.f - hard_occlusion<kCount>(a, lpos, normal, ptl, a.shadow_maps, a.shadow_touch, face,
-                                              global_position, shnum);
+      if (!stat) ss = shadow_setup(a, lpos, normal, ptl_n, global_position);
+      float dyn = 1.f - shadow_taps<kCount>(a, ss, a.shadow_maps, a.shadow_touch, shnum);
       occlusion = fminf(occlusion, dyn);
       shnum++;
     }
-    ptl = fast_normalize(ptl);
+    ptl = ptl_n;
     float light = dot(ptl, lighting_normal);
     light *= occlusion;
     light = fmaxf(light, 0.f);
@@ -525,7 +577,9 @@ __global__ void k_light_prep(const ocr_light* __restrict__ lights, uint32_t n, f
   lin[4 * i] = gamma_fwd(lights[i].col[0]);
   lin[4 * i + 1] = gamma_fwd(lights[i].col[1]);
   lin[4 * i + 2] = gamma_fwd(lights[i].col[2]);
-  lin[4 * i + 3] = 0.f;
+  // the correctly rounded 1/radius the shading pass divides by
+  const float r = lights[i].radius;
+  lin[4 * i + 3] = rcp_safe(r) ? 1.f / r : 0.f;
 }
 
 hipError_t launch_light_prep(const ocr_light* lights, uint32_t n, float* lin, hipStream_t s) {

@@ -1,0 +1,48 @@
+"""The shading pass replaces some correctly rounded divisions with a
+three-instruction form (div_rcp: q = a*y, r = fma(-q, b, a), fma(r, y, q)
+with y = RN(1/b)) and skips tan where the bias saturates.  These checks fix
+that both changes leave every result bit-identical:
+
+* constant divisors 9, 125, 2.5, 255: every finite a >= 0, exhaustively;
+* divisors known only at run time (fov, light radius, the 1/z plane value):
+  Markstein's correction theorem for normal operands, exercised on random
+  pairs over exponents -60..60;
+* tan(a) >= 1 for every binary32 a from float(pi/4) rounded up to
+  rational_acos(0.05), so the bias clamp gives SHADOWBIAS there.
+"""
+import os
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("div") / "exact_division_check")
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, os.path.join(HERE, "exact_division_check.c"),
+                    "-lm", "-ldl"], check=True)
+    return exe
+
+
+def test_constant_divisors_exhaustive(checker):
+    procs = [subprocess.Popen([checker, "const", b], stdout=subprocess.PIPE, text=True)
+             for b in ("9", "125", "2.5", "255")]
+    outs = [(p.communicate(timeout=600)[0], p.returncode) for p in procs]
+    for out, rc in outs:
+        sys.stdout.write(out)
+        assert rc == 0 and " 0 mismatches" in out, out
+
+
+def test_runtime_divisors_random_pairs(checker):
+    r = subprocess.run([checker, "random", "200000000"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and " 0 mismatches" in r.stdout, r.stdout
+
+
+def test_tan_saturates_from_quarter_pi(checker, oracle):
+    from oracle.binding import LIB_PATH
+    r = subprocess.run([checker, "tan", LIB_PATH], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout
