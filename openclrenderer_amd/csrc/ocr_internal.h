@@ -132,9 +132,10 @@ struct Lists {
   RasterTri* rast;
   uint32_t* counters;
   uint32_t cap_cut;
+  uint32_t count_frags;   // keep the reference's fragment count (statistics)
 };
-inline Lists make_lists(CutTri* cuts, uint32_t* counters, uint32_t cap_cut) {
-  return Lists{cuts, reinterpret_cast<RasterTri*>(cuts + cap_cut), counters, cap_cut};
+inline Lists make_lists(CutTri* cuts, uint32_t* counters, uint32_t cap_cut, bool count_frags = true) {
+  return Lists{cuts, reinterpret_cast<RasterTri*>(cuts + cap_cut), counters, cap_cut, count_frags ? 1u : 0u};
 }
 constexpr uint32_t kMaxViews = 4096;
 
@@ -158,12 +159,14 @@ __host__ __device__ inline uint32_t view_tiles(uint32_t w, uint32_t h, uint32_t 
 }
 
 // Device counters of one render: words 0-1 hold a u64 whose high half is the
-// cut-triangle count and low half the raster sample count.
+// cut-triangle count and low half the raster sample count.  Every block of
+// prearrange adds to it; atomics on one cache line serialise (about 12 ns
+// each), so the fragment statistic lives on a line of its own.
 enum Counter : int {
   kPacked = 0,
   kOverflow = 2,
-  kFragCount = 3,          // reference fragments (statistics)
-  kCounterWords = 8,
+  kFragCount = 32,         // reference fragments (statistics), byte 128
+  kCounterWords = 64,
 };
 // Bits of the overflow word: the cut list is full / the sample space is full.
 constexpr uint32_t kOverflowCuts = 1u, kOverflowSamples = 2u;

@@ -85,7 +85,7 @@ __device__ __forceinline__ bool block_alloc(const Lists& L, BlockAllocLds& sh, u
       if ((end >> 32) > L.cap_cut) bad |= kOverflowCuts;
       if ((uint32_t)end > kMaxSamples) bad |= kOverflowSamples;
     }
-    if (tf) atomicAdd(&L.counters[kFragCount], tf);
+    if (tf && L.count_frags) atomicAdd(&L.counters[kFragCount], tf);
     if (bad) atomicOr(&L.counters[kOverflow], bad);
     sh.base = base;
     sh.ok = bad == 0;
@@ -722,7 +722,7 @@ hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, 
                                hipStream_t s) {
   FaceTable ft;
   for (int i = 0; i < 6; i++) ft.f[i] = sc->ctx->face_sc[i];
-  const Lists lists = make_lists(sc->sh_cuts, sc->sh_counters, sc->sh_cap_cut);
+  const Lists lists = make_lists(sc->sh_cuts, sc->sh_counters, sc->sh_cap_cut, false);
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, sc->sh_counters);
   if (sc->ntri)
     hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + kPreThreads - 1) / kPreThreads), dim3(kPreThreads), 0, s, sc->world,
