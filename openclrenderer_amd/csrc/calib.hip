@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ocr_math.h"
+
 __global__ void __launch_bounds__(256) k_calib_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                     size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -116,4 +118,53 @@ extern "C" double ocr_calib_atomics(int mode, uint32_t words, size_t n, uint32_t
   (void)hipEventDestroy(e1);
   (void)hipFree(buf);
   return e == hipSuccess ? (double)ms * 1e3 / iters : -3.0;
+}
+
+// Exhaustive checks of the short reciprocal sequences against IEEE
+// arithmetic, over every binary32 bit pattern in [lo, hi] (NaN inputs must
+// give NaN): mode 0 ocr::rcp_rn(b) vs 1.0f / b, mode 1 ocr::inv_sqrt_cr(d) vs
+// 1.0f / sqrtf(d).  One count per work item, added once, so a bad helper
+// cannot turn the check into billions of same-address atomics.
+__global__ void __launch_bounds__(256) k_calib_rcp_check(int mode, uint32_t lo, uint32_t hi,
+                                                         unsigned long long* __restrict__ bad,
+                                                         uint32_t* __restrict__ first_bad) {
+#pragma clang fp contract(off)
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long n = 0;
+  uint32_t first = 0xFFFFFFFFu;
+  for (uint64_t u = (uint64_t)lo + i; u <= (uint64_t)hi; u += stride) {
+    const float b = __uint_as_float((uint32_t)u);
+    const float fast = mode == 0 ? ocr::rcp_rn(b) : ocr::inv_sqrt_cr(b);
+    const float ref = mode == 0 ? 1.0f / b : 1.0f / sqrtf(b);
+    const bool same = __float_as_uint(fast) == __float_as_uint(ref) || (ref != ref && fast != fast);
+    if (!same) {
+      n++;
+      first = min(first, (uint32_t)u);
+    }
+  }
+  if (n) {
+    atomicAdd(bad, n);
+    atomicMin(first_bad, first);
+  }
+}
+
+extern "C" int ocr_calib_rcp_check(int mode, uint32_t lo, uint32_t hi, unsigned long long* mismatches,
+                                   uint32_t* first_bad) {
+  unsigned long long* d_bad = nullptr;
+  uint32_t* d_first = nullptr;
+  if (hipMalloc(&d_bad, sizeof(*d_bad)) != hipSuccess) return -1;
+  if (hipMalloc(&d_first, sizeof(*d_first)) != hipSuccess) { (void)hipFree(d_bad); return -1; }
+  hipError_t e = hipMemset(d_bad, 0, sizeof(*d_bad));
+  if (e == hipSuccess) e = hipMemset(d_first, 0xFF, sizeof(*d_first));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_calib_rcp_check, dim3(8192), dim3(256), 0, nullptr, mode, lo, hi, d_bad, d_first);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(mismatches, d_bad, sizeof(*d_bad), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(first_bad, d_first, sizeof(*d_first), hipMemcpyDeviceToHost);
+  (void)hipFree(d_bad);
+  (void)hipFree(d_first);
+  return e == hipSuccess ? 0 : -3;
 }

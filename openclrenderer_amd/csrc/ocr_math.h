@@ -60,6 +60,15 @@ __device__ __forceinline__ float div_rcp(float a, float b, float y) {
   const float r = fmaf(-q, b, a);
   return fmaf(r, y, q);
 }
+// RN(1/b) from the hardware reciprocal estimate and one Newton step (3
+// instructions).  Equal to the IEEE quotient 1/b for every binary32 b with
+// |b| in [2^-126, 2^126]: checked on the device for each such b
+// (ocr_calib_rcp_check in calib.hip).  Callers keep b inside that range.
+__device__ __forceinline__ float rcp_rn(float b) {
+  const float r0 = __builtin_amdgcn_rcpf(b);
+  const float e = fmaf(-b, r0, 1.0f);
+  return fmaf(e, r0, r0);
+}
 constexpr float kInv9 = 1.0f / 9.0f;       // RN(1/9), folded by the compiler
 constexpr float kInv125 = 1.0f / 125.0f;
 constexpr float kInvSsaoDiv = 1.0f / kSsaoDiv;
@@ -68,7 +77,17 @@ constexpr float kInv255 = 1.0f / 255.0f;
 // normal, which keeps every quotient the path forms in the normal range.
 __device__ __forceinline__ bool rcp_safe(float d) { return fabsf(d) >= 0x1p-60f && fabsf(d) <= 0x1p60f; }
 
-__device__ __forceinline__ f3 fast_normalize(f3 v) { return v * (1.0f / sqrtf(dot(v, v))); }
+// 1 / sqrt(d), both correctly rounded, for d >= 0: a nonzero d has a normal
+// square root (>= 2^-75), where rcp_rn applies; at +0 and +inf the hardware
+// estimate is the exact +inf / +0.  Equal to 1.0f / sqrtf(d) for every
+// binary32 d >= 0 and NaN: checked on the device for each (calib.hip).
+__device__ __forceinline__ float inv_sqrt_cr(float d) {
+  const float s = sqrtf(d);
+  const float r0 = __builtin_amdgcn_rcpf(s);
+  const float r = fmaf(fmaf(-s, r0, 1.0f), r0, r0);
+  return (s == 0.f || __builtin_isinf(s)) ? r0 : r;
+}
+__device__ __forceinline__ f3 fast_normalize(f3 v) { return v * inv_sqrt_cr(dot(v, v)); }
 __device__ __forceinline__ float fast_length(f3 v) { return sqrtf(dot(v, v)); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(fminf(a, b), c); }
@@ -79,6 +98,16 @@ __device__ __forceinline__ float max3f(float a, float b, float c) { return fmaxf
 __device__ __forceinline__ uint32_t sat_u32(float f) {
   const uint32_t v = (uint32_t)fminf(fmaxf(f, 0.0f), 4294967040.0f);
   return f >= 4294967296.0f ? 0xFFFFFFFFu : v;
+}
+
+// convert_uint_sat(UINT_MAX / fd), the quantised depth of kernel1/2
+// (cl2.cl:5115): 2^32 * RN(1/fd) is RN(2^32/fd) wherever the saturated value
+// depends on it (quotients in [1, 2^32)), and outside rcp_rn's range both
+// saturate alike except for +0 and positive subnormal fd, where the
+// quotient is +inf.
+__device__ __forceinline__ uint32_t depth_quant(float fd) {
+  const uint32_t d = sat_u32(kMulint * rcp_rn(fd));
+  return (fd >= 0.f && fd < 0x1p-126f && !signbit(fd)) ? 0xFFFFFFFFu : d;
 }
 
 // exp in binary64: Cody-Waite reduction by ln2 + degree-13 Taylor polynomial.
@@ -179,7 +208,7 @@ __device__ __forceinline__ f3 back_rot(f3 rel, const SinCos& sc) {
 // rot_quat, cl2.cl:350-357.
 __device__ __forceinline__ f3 rot_quat(f3 p, f4 q) {
   float d = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
-  float inv = 1.0f / sqrtf(d);
+  float inv = inv_sqrt_cr(d);
   f3 qv = mk3(q.x * inv, q.y * inv, q.z * inv);
   float qw = q.w * inv;
   f3 t = cross(qv, p) * 2.f;
@@ -357,7 +386,7 @@ __device__ __forceinline__ uint32_t to_short(float r) {
 }
 __device__ __forceinline__ uint32_t encode_normal(f3 v) {
   if (v.x * v.x + v.y * v.y < 0.0001f) v.x = 0.01f;
-  float inv = 1.0f / sqrtf(v.x * v.x + v.y * v.y);
+  float inv = inv_sqrt_cr(v.x * v.x + v.y * v.y);
   float sq = sqrtf(fmaxf(v.z * 0.5f + 0.5f, 0.f));
   return to_short((v.x * inv) * sq) | (to_short((v.y * inv) * sq) << 16);
 }

@@ -567,7 +567,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
         const float t_ = L.ek[3][k] + L.ek[4][k] * x + L.ek[5][k] * y;
         if (!(s_ > -0.0001f && t_ > -0.0001f && (s_ + t_) < L.ek[6][k])) continue;
         const float fd = fmaf(L.pa[k], x, fmaf(L.pb[k], y, L.pc[k]));
-        const uint32_t d = sat_u32(kMulint / fd);
+        const uint32_t d = depth_quant(fd);
         uint32_t pitch;
         uint32_t* dst;
         if constexpr (kMode == RasterMode::Shadow) {
