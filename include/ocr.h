@@ -111,7 +111,7 @@ typedef struct ocr_light {
 typedef struct ocr_view {
     uint32_t width;
     uint32_t height;
-    float    fov_const;         /* (width/2)/tan(hfov/2), engine.cpp:119-133 */
+    float    fov_const;         /* (width/2)/tan(hfov/2), engine.cpp:119-133; in [2^-60, 2^60] */
     uint32_t use_linear_rendering;  /* object_context.hpp:87, TEST_LINEAR path */
     float    c_pos[4];
     float    c_rot[4];

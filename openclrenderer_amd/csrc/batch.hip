@@ -178,7 +178,8 @@ int ocr_batch_set_views(ocr_batch* b, ocr_scene* const* scenes, const ocr_view* 
     if (!s || !f) return fail(OCR_ERR_INVALID, "ocr_batch_set_views: null scene or frame");
     if (views[i].width != f->width || views[i].height != f->height)
       return fail(OCR_ERR_INVALID, "ocr_batch_set_views: view size differs from its frame");
-    if (!(views[i].fov_const > 0.f)) return fail(OCR_ERR_INVALID, "ocr_batch_set_views: fov_const must be > 0");
+    if (!(views[i].fov_const >= 0x1p-60f && views[i].fov_const <= 0x1p60f))
+      return fail(OCR_ERR_INVALID, "ocr_batch_set_views: fov_const must be in [2^-60, 2^60]");
     if (!s->atlas.pages || (s->nlights && !s->lights))
       return fail(OCR_ERR_INVALID, "ocr_batch_set_views: scene needs its atlas and lights uploaded");
     if (s->nobj && s->max_tid >= std::max<uint32_t>(s->atlas.mip_start, 1))

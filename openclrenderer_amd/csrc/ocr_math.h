@@ -53,8 +53,10 @@ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
 // residual a - b*q is exact under fma, and one fma step from a faithful
 // quotient lands on the rounded one).  Valid when a is +0 or a, b and a/b
 // are normal binary32 numbers; for the constant divisors below it is checked
-// against IEEE division for every finite a >= 0.  Three instructions in
-// place of the eleven of a full correctly rounded division.
+// against IEEE division for a = 0 and every finite a >= 2^-100 (the values
+// divided are integers, depths of at least 20, or sums of weights in
+// [0, 9]).  Three instructions in place of the eleven of a full correctly
+// rounded division.
 __device__ __forceinline__ float div_rcp(float a, float b, float y) {
   const float q = a * y;
   const float r = fmaf(-q, b, a);
@@ -69,10 +71,19 @@ __device__ __forceinline__ float rcp_rn(float b) {
   const float e = fmaf(-b, r0, 1.0f);
   return fmaf(e, r0, r0);
 }
+// RN(1/b) for b that is +-0, +-inf, NaN or has |b| in [2^-126, 2^126] (no
+// subnormals): rcp_rn, with the hardware estimate's exact +-inf / +-0 at
+// the ends.  Integer-valued b and sqrt results qualify.
+__device__ __forceinline__ float rcp_cr(float b) {
+  const float r0 = __builtin_amdgcn_rcpf(b);
+  const float r = fmaf(fmaf(-b, r0, 1.0f), r0, r0);
+  return (b == 0.f || __builtin_isinf(b)) ? r0 : r;
+}
 constexpr float kInv9 = 1.0f / 9.0f;       // RN(1/9), folded by the compiler
 constexpr float kInv125 = 1.0f / 125.0f;
 constexpr float kInvSsaoDiv = 1.0f / kSsaoDiv;
 constexpr float kInv255 = 1.0f / 255.0f;
+constexpr float kInvDepthFar = 1.0f / kDepthFar;
 // A uniform divisor d takes the fast path when it and 1/d are comfortably
 // normal, which keeps every quotient the path forms in the normal range.
 __device__ __forceinline__ bool rcp_safe(float d) { return fabsf(d) >= 0x1p-60f && fabsf(d) <= 0x1p60f; }
@@ -81,12 +92,7 @@ __device__ __forceinline__ bool rcp_safe(float d) { return fabsf(d) >= 0x1p-60f 
 // square root (>= 2^-75), where rcp_rn applies; at +0 and +inf the hardware
 // estimate is the exact +inf / +0.  Equal to 1.0f / sqrtf(d) for every
 // binary32 d >= 0 and NaN: checked on the device for each (calib.hip).
-__device__ __forceinline__ float inv_sqrt_cr(float d) {
-  const float s = sqrtf(d);
-  const float r0 = __builtin_amdgcn_rcpf(s);
-  const float r = fmaf(fmaf(-s, r0, 1.0f), r0, r0);
-  return (s == 0.f || __builtin_isinf(s)) ? r0 : r;
-}
+__device__ __forceinline__ float inv_sqrt_cr(float d) { return rcp_cr(sqrtf(d)); }
 __device__ __forceinline__ f3 fast_normalize(f3 v) { return v * inv_sqrt_cr(dot(v, v)); }
 __device__ __forceinline__ float fast_length(f3 v) { return sqrtf(dot(v, v)); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
@@ -227,6 +233,13 @@ __device__ __forceinline__ f3 depth_project(f3 r, float width, float height, flo
   float f = fovc / r.z;
   return mk3(fmaf(r.x, f, width / 2.f), fmaf(r.y, f, height / 2.f), r.z);
 }
+// The same for vertices that passed clip_near, whose z lies in [20, 350000]
+// (or is the rounding of a point on z = 20), with fovc normal (views are
+// checked on the host, cube faces use LIGHTBUFFERDIM / 2).
+__device__ __forceinline__ f3 depth_project_clipped(f3 r, float width, float height, float fovc) {
+  float f = div_rcp(fovc, r.z, rcp_rn(r.z));
+  return mk3(fmaf(r.x, f, width / 2.f), fmaf(r.y, f, height / 2.f), r.z);
+}
 
 __device__ __forceinline__ f3 sel3(int k, f3 a, f3 b, f3 c) {
   return k == 0 ? a : (k == 1 ? b : c);
@@ -288,8 +301,10 @@ __device__ __forceinline__ void calc_min_max(const float xs[3], const float ys[3
 }
 
 // calc_rconstant_v, cl2.cl:403-406, on rounded coordinates.
+// The determinant of rounded (integer-valued) coordinates is integer-valued,
+// so its reciprocal is rcp_cr's.
 __device__ __forceinline__ float rconstant(f3 x, f3 y) {
-  return 1.0f / (x.y * y.z + x.x * (y.y - y.z) - x.z * y.y + (x.z - x.y) * y.x);
+  return rcp_cr(x.y * y.z + x.x * (y.y - y.z) - x.z * y.y + (x.z - x.y) * y.x);
 }
 
 // interpolate_get_const, cl2.cl:408-418.

@@ -134,10 +134,13 @@ __device__ __forceinline__ void raster_setup(const f3 p[3], float rc, float ew, 
   t.min_y = mm[2];
   max_y = mm[3];
   t.w = mm[1] - mm[0];
-  t.iw = 1.f / (float)(int)t.w;
+  t.iw = rcp_cr((float)(int)t.w);   // integer-valued
   const float xp[3] = {roundf(p[0].x), roundf(p[1].x), roundf(p[2].x)};
   const float yp[3] = {roundf(p[0].y), roundf(p[1].y), roundf(p[2].y)};
-  const f3 depths = mk3(1.0f / (p[0].z / kDepthFar), 1.0f / (p[1].z / kDepthFar), 1.0f / (p[2].z / kDepthFar));
+  // 1 / (z / depth_far) for z in [20, 350000]: the quotient by the constant
+  // is exact for every finite z >= 0, its reciprocal lies in rcp_rn's range
+  auto inv_rel = [](float z) { return rcp_rn(div_rcp(z, kDepthFar, kInvDepthFar)); };
+  const f3 depths = mk3(inv_rel(p[0].z), inv_rel(p[1].z), inv_rel(p[2].z));
   plane_const(depths, mk3(xp[0], xp[1], xp[2]), mk3(yp[0], yp[1], yp[2]), rc, t.A, t.B, t.C);
   // point_in_tri (cl2.cl:4798-4807), loop-invariant terms in the same order
   const float p0x = xp[0], p0y = yp[0], p1x = xp[1], p1y = yp[1], p2x = xp[2], p2y = yp[2];
@@ -188,7 +191,7 @@ __device__ __forceinline__ uint32_t cull_count(const f3 p[3], float ew, float eh
   float mm[4];
   calc_min_max(xs, ys, ew, eh, mm);
   float area = (mm[1] - mm[0]) * (mm[3] - mm[2]);
-  float n = ceilf(area / (float)op_size);
+  float n = ceilf(div_rcp(area, (float)op_size, 1.0f / (float)op_size));   // area >= 0, op 300 or 500
   return n > 0.f ? (uint32_t)n : 0u;
 }
 
@@ -207,7 +210,7 @@ __device__ __forceinline__ void store_cut(CutTri* dst, const f3 p[3], float rcon
     // the plane kernel3's get_vtdiff builds per pixel (cl2.cl:5691-5760)
     const f3 xpv = mk3(roundf(p[0].x), roundf(p[1].x), roundf(p[2].x));
     const f3 ypv = mk3(roundf(p[0].y), roundf(p[1].y), roundf(p[2].y));
-    const f3 depths = mk3(1.0f / p[0].z, 1.0f / p[1].z, 1.0f / p[2].z);
+    const f3 depths = mk3(rcp_rn(p[0].z), rcp_rn(p[1].z), rcp_rn(p[2].z));   // z in [20, 350000]
     plane_const(depths, xpv, ypv, rconst, c.da, c.db, c.dc);
   }
   *dst = c;
@@ -263,10 +266,10 @@ __device__ __forceinline__ int face_clip_project(const f3 w[3], f3 cpos, const S
   // slot 1 exists only for triangles crossing the near plane with one
   // vertex behind it; static indices keep both slots in registers
 #pragma unroll
-  for (int i = 0; i < 3; i++) out[0][i] = depth_project(tris[0][i], ew, eh, fovc);
+  for (int i = 0; i < 3; i++) out[0][i] = depth_project_clipped(tris[0][i], ew, eh, fovc);
   if (n == 2) {
 #pragma unroll
-    for (int i = 0; i < 3; i++) out[1][i] = depth_project(tris[1][i], ew, eh, fovc);
+    for (int i = 0; i < 3; i++) out[1][i] = depth_project_clipped(tris[1][i], ew, eh, fovc);
   } else {
 #pragma unroll
     for (int i = 0; i < 3; i++) out[1][i] = out[0][i];

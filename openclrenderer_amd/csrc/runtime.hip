@@ -612,7 +612,8 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   if (!s || !view || !f) return fail(OCR_ERR_INVALID, "ocr_render: null argument");
   if (view->width != f->width || view->height != f->height)
     return fail(OCR_ERR_INVALID, "ocr_render: view size differs from the frame");
-  if (!(view->fov_const > 0.f)) return fail(OCR_ERR_INVALID, "ocr_render: fov_const must be > 0");
+  if (!(view->fov_const >= 0x1p-60f && view->fov_const <= 0x1p60f))
+    return fail(OCR_ERR_INVALID, "ocr_render: fov_const must be in [2^-60, 2^60]");
   if (!s->atlas.pages) return fail(OCR_ERR_INVALID, "ocr_render: scene has no atlas (ocr_atlas_upload)");
   if (s->nobj && s->max_tid >= std::max<uint32_t>(s->atlas.mip_start, 1))
     return fail(OCR_ERR_INVALID, "ocr_render: an object's texture id is outside the atlas");

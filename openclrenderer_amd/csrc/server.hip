@@ -395,7 +395,7 @@ int ocr_server_add_scene(ocr_server* s, ocr_scene* scene, uint32_t* slot) {
 int ocr_server_submit(ocr_server* s, uint32_t scene_slot, const ocr_view* view, uint64_t* request_id) {
   if (!s || !view || !request_id) return fail(OCR_ERR_INVALID, "ocr_server_submit: null argument");
   if (view->width < 4 || view->height < 4 || (uint64_t)view->width * view->height > kMaxTargetPixels ||
-      !(view->fov_const > 0.f))
+      !(view->fov_const >= 0x1p-60f && view->fov_const <= 0x1p60f))
     return fail(OCR_ERR_INVALID, "ocr_server_submit: bad view");
   Request* r = new (std::nothrow) Request();
   if (!r) return fail(OCR_ERR_OOM, "ocr_server_submit: host allocation");

@@ -1,8 +1,11 @@
 /* Checks of the arithmetic identities the shading pass uses in place of
  * full correctly rounded divisions (div_rcp in csrc/ocr_math.h):
  *
- *   const  B      every finite binary32 a >= 0: fma(fma(-q, B, a), y, q) with
- *                 y = RN(1/B), q = RN(a*y) equals the IEEE quotient a / B
+ *   const  B      a = 0 and every finite binary32 a >= 2^-100:
+ *                 fma(fma(-q, B, a), y, q) with y = RN(1/B), q = RN(a*y)
+ *                 equals the IEEE quotient a / B (below 2^-100 the quotient
+ *                 or the residual can be subnormal and the identity fails
+ *                 for some divisors; the kernels never divide such values)
  *   random N      N random pairs a, b with exponents in [-60, 60]: the same
  *                 with y = RN(1/b)
  *   tan LIB       with the oracle library LIB: tan_f(a) >= 1 for every
@@ -30,8 +33,8 @@ static int check_const(float b) {
   volatile float vb = b;
   const float y = 1.0f / vb;
   uint64_t bad = 0;
-  for (uint64_t u = 0; u < 0x7F800000ull; u++) {
-    const float a = fb((uint32_t)u);
+  for (uint64_t u = 0x0D800000ull; u < 0x7F800001ull; u = (u == 0x7F800000ull ? 0x7F800001ull : u + 1)) {
+    const float a = fb(u == 0x7F800000ull ? 0u : (uint32_t)u);   /* 2^-100 .. max finite, then +0 */
     volatile float va = a;
     if (bf(div_rcp(a, b, y)) != bf(va / vb)) bad++;
   }
