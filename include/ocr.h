@@ -165,6 +165,16 @@ int  ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri,
 /* object::g_flush (object.cpp:652-857): rewrite descriptors [first, first+n). */
 int  ocr_scene_update_objs(ocr_scene* s, uint32_t first, uint32_t n,
                            const ocr_obj_desc* objs);
+/* The same, ordered on `stream` like the reference's enqueued descriptor
+ * writes (object::g_flush, object.cpp:652-857, on the render queue): the
+ * descriptors are staged in pinned memory (the host array may be reused on
+ * return), copied and the scene's world-space copy refreshed on `stream`;
+ * renders, shadow passes and batches issued afterwards, on any stream, wait
+ * for it.  Enqueue it on the stream the scene's renders use (work already
+ * enqueued there that reads the scene completes first); the host blocks only
+ * when four earlier updates of the scene are still in flight. */
+int  ocr_scene_update_objs_async(ocr_scene* s, uint32_t first, uint32_t n,
+                                 const ocr_obj_desc* objs, void* stream);
 /* texture_context::alloc_gpu (texture_context.cpp:350-517) +
  * texture::update_me_to_gpu / update_gpu_mipmaps (texture.cpp:323-493):
  * packs the textures into 2048x2048 RGBA8 pages per power-of-two size class,

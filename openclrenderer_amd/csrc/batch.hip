@@ -113,6 +113,17 @@ int ocr::batch_reserve(ocr_batch* b, uint64_t sum_tris) {
   return rc;
 }
 
+// A batch reads its scenes' geometry and shadow maps: it follows their last
+// stream-ordered descriptor update and shadow pass, wherever enqueued.
+static hipError_t wait_scenes(ocr_batch* b, hipStream_t st) {
+  for (Scene* s : b->scenes) {
+    hipError_t e = s->wait_geometry(st);
+    if (e == hipSuccess) e = s->wait_shadow_maps(st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 static hipError_t enqueue_batch(ocr_batch* b, hipStream_t st) {
   const PassArgs pa = batch_pass(b);
   hipError_t e = launch_clear(pa, st);
@@ -246,6 +257,7 @@ int ocr_batch_render(ocr_batch* b, void* stream) {
   int rc = ensure_lists(b);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
+  OCR_HIP_CHECK(wait_scenes(b, st));
   const bool prof = b->profiling && b->n_profiled < kBatchRing;
   if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled], st));
   OCR_HIP_CHECK(enqueue_batch(b, st));
@@ -273,6 +285,7 @@ int ocr_batch_launch_graph(ocr_batch* b, void* stream) {
     OCR_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
   hipStream_t st = (hipStream_t)stream;
+  OCR_HIP_CHECK(wait_scenes(b, st));
   const bool prof = b->profiling && b->n_profiled < kBatchRing;
   if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled], st));
   OCR_HIP_CHECK(hipGraphLaunch(b->exec, st));

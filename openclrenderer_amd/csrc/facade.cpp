@@ -318,7 +318,9 @@ void object_context::flush_locations(bool force, compute_event* render_event) {
   if (!gpu_dat.scene || built_objects.empty()) return;
   std::vector<ocr_obj_desc> descs;
   write_descriptors(descs);
-  report("ocr_scene_update_objs", ocr_scene_update_objs(gpu_dat.scene, 0, (uint32_t)descs.size(), descs.data()));
+  // enqueued like the reference's descriptor writes: the next draw follows it
+  report("ocr_scene_update_objs_async",
+         ocr_scene_update_objs_async(gpu_dat.scene, 0, (uint32_t)descs.size(), descs.data(), nullptr));
 }
 
 bool object_context_data::ensure_screen_buffers(int w, int h) {

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -214,7 +215,26 @@ struct Scene {
   // recorded on the stream of the last shadow pass after its last launch;
   // renders wait for it (they run on streams of their frame)
   hipEvent_t written = nullptr;
-  bool written_valid = false;
+  std::atomic<bool> written_valid{false};
+  // recorded after the last stream-ordered descriptor update and its
+  // world-space refresh; renders and shadow passes wait for it before
+  // reading the geometry
+  hipEvent_t geom_written = nullptr;
+  std::atomic<bool> geom_valid{false};
+  // pinned staging of stream-ordered updates: a slot is reused once the copy
+  // that read it has completed
+  static constexpr int kUpdRing = 4;
+  ObjDesc* upd_host[kUpdRing] = {};
+  hipEvent_t upd_done[kUpdRing] = {};
+  bool upd_used[kUpdRing] = {};
+  int upd_next = 0;
+  // what the render and batch paths wait on before they read the scene
+  hipError_t wait_geometry(hipStream_t st) const {
+    return geom_valid ? hipStreamWaitEvent(st, geom_written, 0) : hipSuccess;
+  }
+  hipError_t wait_shadow_maps(hipStream_t st) const {
+    return written_valid ? hipStreamWaitEvent(st, written, 0) : hipSuccess;
+  }
 };
 
 struct Frame {

@@ -251,6 +251,7 @@ int ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri, cons
   if (e == hipSuccess) e = launch_world(s, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipEventCreateWithFlags(&s->written, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&s->geom_written, hipEventDisableTiming);
   if (e != hipSuccess) {
     ocr_scene_destroy(s);
     return fail(OCR_ERR_HIP, std::string("ocr_scene_create: ") + hipGetErrorString(e));
@@ -278,6 +279,37 @@ int ocr_scene_update_objs(ocr_scene* s, uint32_t first, uint32_t n, const ocr_ob
   return OCR_OK;
 }
 
+int ocr_scene_update_objs_async(ocr_scene* s, uint32_t first, uint32_t n, const ocr_obj_desc* objs, void* stream) {
+  if (!s || (n && !objs) || first > s->nobj || n > s->nobj - first)
+    return fail(OCR_ERR_INVALID, "ocr_scene_update_objs_async: range outside the scene");
+  if (!n) return OCR_OK;
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  const int k = s->upd_next;
+  s->upd_next = (k + 1) % Scene::kUpdRing;
+  if (!s->upd_host[k]) {
+    hipError_t e = hipHostMalloc((void**)&s->upd_host[k], sizeof(ObjDesc) * (size_t)s->nobj);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->upd_done[k], hipEventDisableTiming);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(OCR_ERR_OOM, std::string("ocr_scene_update_objs_async: pinned staging: ") + hipGetErrorString(e));
+    }
+  } else if (s->upd_used[k]) {
+    OCR_HIP_CHECK(hipEventSynchronize(s->upd_done[k]));   // the slot's last copy has read it
+  }
+  for (uint32_t o = 0; o < n; o++) {
+    pack_obj(objs[o], s->upd_host[k][o]);
+    s->max_tid = std::max(s->max_tid, objs[o].tid);
+  }
+  hipStream_t st = (hipStream_t)stream;
+  OCR_HIP_CHECK(hipMemcpyAsync(s->objs + first, s->upd_host[k], sizeof(ObjDesc) * n, hipMemcpyHostToDevice, st));
+  OCR_HIP_CHECK(hipEventRecord(s->upd_done[k], st));
+  s->upd_used[k] = true;
+  OCR_HIP_CHECK(launch_world(s, st));
+  OCR_HIP_CHECK(hipEventRecord(s->geom_written, st));
+  s->geom_valid = true;
+  return OCR_OK;
+}
+
 static void free_atlas(Atlas& a) {
   dev_free(a.pages);
   dev_free(a.nums);
@@ -300,6 +332,14 @@ void ocr_scene_destroy(ocr_scene* s) {
   dev_free(s->sh_cuts);
   dev_free(s->sh_counters);
   if (s->written) (void)hipEventDestroy(s->written);
+  if (s->geom_written) (void)hipEventDestroy(s->geom_written);
+  for (int k = 0; k < Scene::kUpdRing; k++) {
+    if (s->upd_done[k]) {
+      (void)hipEventSynchronize(s->upd_done[k]);
+      (void)hipEventDestroy(s->upd_done[k]);
+    }
+    if (s->upd_host[k]) (void)hipHostFree(s->upd_host[k]);
+  }
   delete s;
 }
 
@@ -467,6 +507,7 @@ int ocr_shadow_pass(ocr_scene* s, void* stream) {
   if (s->n_shadow == 0 && !(s->static_dirty && s->n_static)) return OCR_OK;
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   hipStream_t st = (hipStream_t)stream;
+  OCR_HIP_CHECK(s->wait_geometry(st));   // descriptor updates enqueued on another stream
   {
     // A triangle reaches the faces its three vertices fall in, each as at
     // most 2 clipped pieces, so 6 records per triangle always fit.
@@ -640,6 +681,8 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   // render follows the caller's stream.
   OCR_HIP_CHECK(hipEventRecord(f->entry[b], st));
   OCR_HIP_CHECK(hipStreamWaitEvent(sb, f->entry_valid[b ^ 1] ? f->entry[b ^ 1] : f->entry[b], 0));
+  // and the scene's last stream-ordered descriptor update
+  OCR_HIP_CHECK(s->wait_geometry(sb));
   f->entry_valid[b] = true;
   f->select(b);
   const Frame::OutBuf& o = f->out[b];
@@ -658,7 +701,7 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
   // Shading reads the shadow maps: it waits for the scene's last shadow
   // pass, wherever that was enqueued (the raster passes do not read them).
-  if (s->written_valid) OCR_HIP_CHECK(hipStreamWaitEvent(sb, s->written, 0));
+  OCR_HIP_CHECK(s->wait_shadow_maps(sb));
   OCR_HIP_CHECK(launch_shade(pa, s->ctx->face_sc, sb));
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[5], sb));
   OCR_HIP_CHECK(hipEventRecord(f->done[b], sb));

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libocr.so")
 # Every entry point include/ocr.h declares.
 EXPORTS = [
     "ocr_create", "ocr_destroy", "ocr_last_error", "ocr_query_free_hbm",
-    "ocr_scene_create", "ocr_scene_update_objs", "ocr_atlas_upload", "ocr_lights_upload",
+    "ocr_scene_create", "ocr_scene_update_objs", "ocr_scene_update_objs_async", "ocr_atlas_upload", "ocr_lights_upload",
     "ocr_shadow_pass", "ocr_scene_destroy", "ocr_scene_atlas_info", "ocr_scene_read_atlas",
     "ocr_scene_read_shadow_maps", "ocr_frame_create", "ocr_frame_destroy",
     "ocr_frame_set_profiling", "ocr_render", "ocr_frame_read", "ocr_frame_device_buffers",
@@ -92,6 +92,7 @@ def load_library(path=LIB_PATH):
     L.ocr_query_free_hbm.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     L.ocr_scene_create.argtypes = [vp, vp, u32, vp, u32, pp]
     L.ocr_scene_update_objs.argtypes = [vp, u32, u32, vp]
+    L.ocr_scene_update_objs_async.argtypes = [vp, u32, u32, vp, vp]
     L.ocr_atlas_upload.argtypes = [vp, vp, u32]
     L.ocr_lights_upload.argtypes = [vp, vp, u32, u32]
     L.ocr_shadow_pass.argtypes = [vp, vp]
@@ -199,6 +200,11 @@ class GpuScene:
     def update_objs(self, first, objs):
         objs = np.ascontiguousarray(objs, dtype=abi.OBJ_DESC)
         check(self.lib.ocr_scene_update_objs(self.h, first, objs.shape[0], _p(objs)))
+
+    def update_objs_async(self, first, objs, stream=None):
+        """Descriptor update ordered on `stream`; later renders see it."""
+        objs = np.ascontiguousarray(objs, dtype=abi.OBJ_DESC)
+        check(self.lib.ocr_scene_update_objs_async(self.h, first, objs.shape[0], _p(objs), stream))
 
     def shadow_pass(self, stream=None):
         check(self.lib.ocr_shadow_pass(self.h, stream))

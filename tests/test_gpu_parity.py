@@ -180,6 +180,38 @@ def test_back_to_back_renders_follow_scene_writes(gpu_ctx, oracle):
     assert_frame_parity(fr.read(), oracle.render(sc2, other, maps=maps), "other view")
 
 
+def test_stream_ordered_object_updates(gpu_ctx, oracle):
+    """Descriptor updates enqueued between renders with no host wait
+    (ocr_scene_update_objs_async): each frame shows exactly the objects and
+    shadow maps of the update issued before it, on three frames and on one
+    frame rendered three times back to back."""
+    sc, view = scenes.config1(shadow=1)
+    g = runtime.GpuScene(gpu_ctx, sc)
+    frames = [runtime.Frame(gpu_ctx, 800, 600) for _ in range(3)]
+    same = runtime.Frame(gpu_ctx, 800, 600)
+    objs = sc.objs.copy()
+    states = []
+    for k, fr in enumerate(frames):
+        objs["world_pos"][0, :3] = [60.0 * k - 50.0, -40.0 + 10.0 * k, 60.0]
+        objs["world_pos"][5, 1] += 25.0 * k
+        g.update_objs_async(0, objs)
+        g.shadow_pass()
+        fr.render(g, view)
+        same.render(g, view)
+        states.append(objs.copy())
+    outs = [fr.read() for fr in frames]
+    last = same.read()
+    for k in range(3):
+        sck = scenes.Scene(sc.tris, states[k], sc.lights, sc.textures, sc.light_buffer_dim)
+        maps = oracle.shadow_maps(sck)
+        ref = oracle.render(sck, view, maps=maps, nthreads=8)
+        assert_frame_parity(outs[k], ref, f"update {k}")
+    for key in ("depth", "tri_id", "rgba", "normals"):
+        assert np.array_equal(last[key], outs[2][key]), key
+    with pytest.raises(runtime.OcrError):
+        g.update_objs_async(len(sc.objs), sc.objs[:1])
+
+
 def test_invalid_inputs_are_rejected(gpu_ctx):
     sc, view = scenes.config1()
     g = runtime.GpuScene(gpu_ctx, sc)
