@@ -236,6 +236,12 @@ int  ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream);
  * OCR_ERR_UNSUPPORTED if it needed more than 2^31 - 1 raster samples. */
 int  ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id,
                     uint8_t* rgba8, uint16_t* normals, void* stream);
+/* engine::blend_with_depth -> blend_screens_with_depth (engine.cpp:2550-2570,
+ * cl2.cl:7842-7875): composites src onto dst (same size) on `stream`:
+ * wherever dst's depth is not nearer than src's, dst's RGBA8 becomes src
+ * over dst (alpha_blend, cl2.cl:1721-1728); dst's depth is unchanged.  Uses
+ * each frame's last render; enqueue it on the stream those renders used. */
+int  ocr_frame_blend_with_depth(ocr_frame* src, ocr_frame* dst, void* stream);
 /* Device pointers of the frame's buffers (zero-copy consumers). */
 int  ocr_frame_device_buffers(ocr_frame* f, void** depth, void** idkey,
                               void** rgba8, void** normals);

@@ -247,6 +247,9 @@ struct engine {
   void append_opencl_extra_command_line(const std::string&) {}
   compute_event generate_realtime_shadowing(object_context_data& dat);
   compute_event draw_bulk_objs_n(object_context_data& dat);
+  // engine.cpp:2550-2570: src's screen over dst's where src is at least as
+  // near (both contexts drawn at this engine's size)
+  compute_event blend_with_depth(object_context_data& src, object_context_data& dst);
   void blit_to_screen(object_context_data& dat);
   void render_block();
   void flip() { frame_count++; }

@@ -373,6 +373,13 @@ compute_event engine::generate_realtime_shadowing(object_context_data& dat) {
   return record_event(nullptr);
 }
 
+compute_event engine::blend_with_depth(object_context_data& src, object_context_data& dst) {
+  if (!src.frame || !dst.frame) return compute_event();
+  last_status = ocr_frame_blend_with_depth(src.frame, dst.frame, nullptr);
+  report("ocr_frame_blend_with_depth", last_status);
+  return record_event(nullptr);
+}
+
 compute_event engine::draw_bulk_objs_n(object_context_data& dat) {
   if (!dat.scene || !dat.ensure_screen_buffers((int)width, (int)height)) return compute_event();
   sync_lights(dat);

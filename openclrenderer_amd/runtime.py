@@ -21,6 +21,7 @@ EXPORTS = [
     "ocr_shadow_pass", "ocr_scene_destroy", "ocr_scene_atlas_info", "ocr_scene_read_atlas",
     "ocr_scene_read_shadow_maps", "ocr_frame_create", "ocr_frame_destroy",
     "ocr_frame_set_profiling", "ocr_render", "ocr_frame_read", "ocr_frame_device_buffers",
+    "ocr_frame_blend_with_depth",
     "ocr_frame_get_stats", "ocr_frame_count_texels", "ocr_frame_reserve",
     "ocr_batch_create", "ocr_batch_destroy", "ocr_batch_set_views", "ocr_batch_set_tuning",
     "ocr_batch_get_tuning", "ocr_batch_set_profiling", "ocr_batch_render", "ocr_batch_launch_graph",
@@ -109,6 +110,7 @@ def load_library(path=LIB_PATH):
     L.ocr_render.argtypes = [vp, vp, vp, vp]
     L.ocr_frame_read.argtypes = [vp, vp, vp, vp, vp, vp]
     L.ocr_frame_device_buffers.argtypes = [vp, pp, pp, pp, pp]
+    L.ocr_frame_blend_with_depth.argtypes = [vp, vp, vp]
     L.ocr_frame_get_stats.argtypes = [vp, vp, C.POINTER(FrameStats)]
     L.ocr_frame_count_texels.argtypes = [vp, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp]
     L.ocr_batch_create.argtypes = [vp, u32, pp]
@@ -285,6 +287,11 @@ class Frame:
         a, s = C.c_uint64(), C.c_uint64()
         check(self.lib.ocr_frame_count_texels(gscene.h, _p(v), self.h, C.byref(a), C.byref(s), stream))
         return a.value, s.value
+
+    def blend_from(self, src, stream=None):
+        """engine::blend_with_depth(src, self): src over this frame where src
+        is at least as near."""
+        check(self.lib.ocr_frame_blend_with_depth(src.h, self.h, stream))
 
     def device_buffers(self):
         ptrs = [C.c_void_p() for _ in range(4)]

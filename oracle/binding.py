@@ -105,6 +105,7 @@ class Oracle:
         L.oracle_encode_normal.argtypes = [fp]
         L.oracle_encode_normal.restype = u
         L.oracle_texture_mod.argtypes = [fp, fp]
+        L.oracle_blend_with_depth.argtypes = [u, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
 
     # --------------------------------------------------------------- helpers
     @staticmethod
@@ -120,6 +121,17 @@ class Oracle:
             ptrs.append(p)
         out, po = self._f(np.zeros(out_len, dtype=np.float32))
         getattr(self.lib, name)(*ptrs, po)
+        return out
+
+    def blend_with_depth(self, src_depth, dst_depth, src_rgba, dst_rgba):
+        """blend_screens_with_depth on host arrays; returns the new dst RGBA8
+        (H x W x 4 uint8), the inputs are left unchanged."""
+        sd = np.ascontiguousarray(src_depth, dtype=np.uint32)
+        dd = np.ascontiguousarray(dst_depth, dtype=np.uint32)
+        sr = np.ascontiguousarray(src_rgba, dtype=np.uint8)
+        out = np.array(dst_rgba, dtype=np.uint8, copy=True, order="C")
+        assert sd.size == dd.size == sr.size // 4 == out.size // 4
+        self.lib.oracle_blend_with_depth(sd.size, sd.ctypes.data, dd.ctypes.data, sr.ctypes.data, out.ctypes.data)
         return out
 
     def face_sincos(self):

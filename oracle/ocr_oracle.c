@@ -1302,3 +1302,24 @@ void oracle_get_barycentric(const float p[3], const float a[3], const float b[3]
 }
 uint32_t oracle_encode_normal(const float n[3]) { return encode_normal(ld3(n)); }
 void oracle_texture_mod(const float vt[2], float out[2]) { out[0] = tmod1(vt[0]); out[1] = tmod1(vt[1]); }
+
+/* blend_screens_with_depth, cl2.cl:7842-7875: the pixel is left alone when
+ * the destination is nearer (d2 < d1), else alpha_blend (cl2.cl:1721-1728):
+ * rgb = mad(src.rgb, src.a, dst.rgb * (1 - src.a)), a = src.a.  The images
+ * are RGBA8 (read_imagef gives c / 255); the result is quantised like the
+ * shading pass's output. */
+void oracle_blend_with_depth(uint32_t npix, const uint32_t* src_depth, const uint32_t* dst_depth,
+                             const uint8_t* src_rgba, uint8_t* dst_rgba) {
+    for (uint32_t p = 0; p < npix; p++) {
+        const uint32_t d1 = src_depth[p], d2 = dst_depth[p];
+        if (d2 < d1) continue;
+        float s[4], d[4];
+        for (int c = 0; c < 4; c++) {
+            s[c] = (float)src_rgba[4 * (size_t)p + c] / 255.f;
+            d[c] = (float)dst_rgba[4 * (size_t)p + c] / 255.f;
+        }
+        const float inv = 1.f - s[3];
+        for (int c = 0; c < 3; c++) dst_rgba[4 * (size_t)p + c] = (uint8_t)qz(fmaf(s[c], s[3], d[c] * inv));
+        dst_rgba[4 * (size_t)p + 3] = (uint8_t)qz(s[3]);
+    }
+}

@@ -129,6 +129,14 @@ void  oracle_get_barycentric(const float p[3], const float a[3], const float b[3
 uint32_t oracle_encode_normal(const float n[3]);
 void  oracle_texture_mod(const float vt[2], float out[2]);
 
+/* blend_screens_with_depth (cl2.cl:7842-7875) with alpha_blend
+ * (cl2.cl:1721-1728), engine::blend_with_depth (engine.cpp:2550-2570): for
+ * every pixel where dst_depth >= src_depth, dst_rgba = src over dst (RGBA8
+ * read as c / 255, written with the shading pass's quantisation); dst depth
+ * is left as it is. */
+void  oracle_blend_with_depth(uint32_t npix, const uint32_t* src_depth, const uint32_t* dst_depth,
+                              const uint8_t* src_rgba, uint8_t* dst_rgba);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,35 @@
+"""blend_screens_with_depth on the GPU (ocr_frame_blend_with_depth) against
+the oracle's restatement, on frames the path rendered: bit-exact RGBA8, the
+destination depth unchanged.  Sizes include one whose pixel count is not a
+multiple of 4 (the kernel's tail)."""
+import numpy as np
+import pytest
+
+from openclrenderer_amd import runtime, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("w,h,seed", [(37, 23, 1), (640, 480, 2), (1920, 1080, 3)])
+def test_blend_with_depth_matches_oracle(gpu_ctx, oracle, w, h, seed):
+    sa, va = scenes.random_scene(500 + seed, n_tris=400, width=w, height=h)
+    sb, vb = scenes.random_scene(600 + seed, n_tris=400, width=w, height=h)
+    ga, gb = runtime.GpuScene(gpu_ctx, sa), runtime.GpuScene(gpu_ctx, sb)
+    fa, fb = runtime.Frame(gpu_ctx, w, h), runtime.Frame(gpu_ctx, w, h)
+    src = fa.render_read(ga, va)
+    dst = fb.render_read(gb, vb)
+    fb.blend_from(fa)
+    out = fb.read()
+    want = oracle.blend_with_depth(src["depth"], dst["depth"], src["rgba"], dst["rgba"])
+    assert np.array_equal(out["rgba"], want.reshape(out["rgba"].shape))
+    assert np.array_equal(out["depth"], dst["depth"])
+    blended = dst["depth"].reshape(-1) >= src["depth"].reshape(-1)
+    assert blended.any() and (~blended).any()
+
+
+def test_blend_rejects_mismatched_frames(gpu_ctx):
+    fa, fb = runtime.Frame(gpu_ctx, 64, 48), runtime.Frame(gpu_ctx, 64, 40)
+    with pytest.raises(runtime.OcrError):
+        fb.blend_from(fa)
+    with pytest.raises(runtime.OcrError):
+        fa.blend_from(fa)
