@@ -93,19 +93,32 @@ struct object {
   void set_flag(cl_int flag, bool v) { feature_flag = v ? (feature_flag | flag) : (feature_flag & ~flag); }
 };
 
+struct objects_container;
+// obj_load (obj_load.cpp:181-568): the default load function, reading
+// `file` (Wavefront OBJ + MTL, include/ocr_obj.h) into one object per usemtl
+// group with its texture (a PNG map_Kd, a Kd colour or magenta).
+void obj_load(objects_container* c);
+
 struct objects_container {
   object_context* parent = nullptr;
   cl_uint id = 0;
   std::string file;
+  float requested_scale = 1.f;       // applied to positions at load
+  float current_scale = 1.f;
+  bool textures_are_unique = false;
+  std::string normal_map;
   bool isactive = false;
   bool isloaded = false;
   cl_float4 pos{{0, 0, 0, 0}};
   quaternion rot_quat;
   float dynamic_scale = 1.f;
   std::vector<object> objs;
-  std::function<void(objects_container*)> fp;
+  std::function<void(objects_container*)> fp = obj_load;   // objects_container.cpp:22
 
   void set_file(const std::string& f) { file = f; }
+  void request_scale(float f) { requested_scale = f; }
+  void set_unique_textures(bool u) { textures_are_unique = u; }
+  void set_normal(const std::string& n) { normal_map = n; }
   void set_load_func(std::function<void(objects_container*)> f) { fp = std::move(f); }
   void call_load_func(objects_container* c) { if (fp) fp(c); }
   void set_active(bool param) { isactive = param; }
@@ -126,6 +139,9 @@ struct objects_container {
 
 struct texture {
   int id = -1;
+  std::string texture_location;    // cache key of make_new_cached
+  bool cacheable = true;
+  int type = 0;                    // 1 = bump map
   cl_uint width = 0, height = 0;
   std::vector<uint8_t> rgba;       // row 0 first, tightly packed
   bool force_load = false;
@@ -133,6 +149,8 @@ struct texture {
   // texture::set_create_colour (texture.hpp): a solid-colour texture
   void set_create_colour(uint8_t r, uint8_t g, uint8_t b, uint8_t a, cl_uint w, cl_uint h);
   void set_raw(cl_uint w, cl_uint h, const uint8_t* data);
+  // decodes a PNG file (ocr_png_load); magenta 32x32 if it cannot
+  bool set_location(const std::string& path);
 };
 
 struct texture_context {
@@ -141,6 +159,8 @@ struct texture_context {
   int mipmap_start = 0;
   int gid = 0;
   texture* make_new();
+  // texture_context.cpp:27-45: the texture already made for `loc`, if any
+  texture* make_new_cached(const std::string& loc);
   texture* id_to_tex(int id);
   // position of a texture id in the last upload order (get_gpu_position_id)
   int get_gpu_position_id(int id) const;

@@ -64,7 +64,8 @@ def build(verbose=False, jobs=4):
     hip = ["-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
     tmp = FACADE_LIB + ".tmp"
     _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", inc, os.path.join(CSRC, "facade.cpp"),
-          "-L", HERE, "-locr", *hip, "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+          os.path.join(CSRC, "obj_load.cpp"), "-L", HERE, "-locr", *hip, "-lz", "-Wl,-rpath,$ORIGIN", "-o", tmp],
+         verbose)
     os.replace(tmp, FACADE_LIB)
     tmp = FACADE_DEMO + ".tmp"
     _run([CXX, "-O2", "-std=c++17", "-Wall", "-I", inc, os.path.join(CSRC, "facade_demo.cpp"),

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../../include/ocr_engine.hpp"
+#include "../../include/ocr_obj.h"
 
 // The HIP runtime header declares a `texture` template that collides with the
 // draw API's texture class, so the few runtime entry points used here are
@@ -101,6 +102,27 @@ void texture::set_raw(cl_uint w, cl_uint h, const uint8_t* data) {
   width = w;
   height = h;
   rgba.assign(data, data + (size_t)w * h * 4);
+}
+
+bool texture::set_location(const std::string& path) {
+  uint32_t w = 0, h = 0;
+  uint8_t* px = nullptr;
+  if (ocr_png_load(path.c_str(), &w, &h, &px) != OCR_OK) {
+    std::fprintf(stderr, "[ocr facade] texture %s could not be decoded\n", path.c_str());
+    set_create_colour(255, 0, 255, 255, 32, 32);
+    return false;
+  }
+  set_raw(w, h, px);
+  ocr_png_free(px);
+  return true;
+}
+
+texture* texture_context::make_new_cached(const std::string& loc) {
+  for (auto& t : all_textures)
+    if (t->cacheable && t->texture_location == loc) return t.get();
+  texture* t = make_new();
+  t->texture_location = loc;
+  return t;
 }
 
 texture* texture_context::make_new() {
@@ -215,6 +237,55 @@ object_context::object_context() {}
 object_context::~object_context() {
   if (gpu_dat.frame) ocr_frame_destroy(gpu_dat.frame);
   if (gpu_dat.scene) ocr_scene_destroy(gpu_dat.scene);
+}
+
+// obj_load.cpp:181-568 on the parsed mesh: one object per usemtl part.
+void obj_load(objects_container* c) {
+  if (!c || !c->parent) return;
+  ocr_obj_mesh* m = nullptr;
+  if (ocr_obj_load(c->file.c_str(), c->requested_scale, &m) != OCR_OK) {
+    std::fprintf(stderr, "[ocr facade] %s could not be loaded\n", c->file.c_str());
+    return;
+  }
+  std::vector<triangle> tris(ocr_obj_ntri(m));
+  std::vector<ocr_obj_part> parts(ocr_obj_nparts(m));
+  ocr_obj_get(m, tris.data(), parts.data());
+  ocr_obj_free(m);
+  texture_context& tc = c->parent->tex_ctx;
+  for (const ocr_obj_part& p : parts) {
+    texture* tex;
+    if (p.texture_kind == OCR_OBJ_TEX_FILE) {
+      tex = c->textures_are_unique ? tc.make_new() : tc.make_new_cached(p.texture_cache);
+      if (tex->rgba.empty()) tex->set_location(p.texture_path);
+    } else if (p.texture_kind == OCR_OBJ_TEX_COLOUR) {
+      tex = c->textures_are_unique ? tc.make_new() : tc.make_new_cached(p.texture_cache);
+      if (tex->rgba.empty()) tex->set_create_colour(p.colour[0], p.colour[1], p.colour[2], 255, 32, 32);
+    } else {
+      tex = tc.make_new();
+      tex->set_create_colour(255, 0, 255, 255, 32, 32);
+    }
+    object o;
+    o.tri_list.assign(tris.begin() + p.first_tri, tris.begin() + p.first_tri + p.ntri);
+    o.object_name = p.group_name;
+    o.tid = (cl_uint)tex->id;
+    if (p.bump_path[0]) {
+      texture* bump = tc.make_new_cached(p.bump_path);
+      bump->type = 1;
+      if (bump->rgba.empty()) bump->set_location(p.bump_path);
+      o.has_bump = 1;
+    }
+    if (!c->normal_map.empty()) {
+      texture* nm = tc.make_new_cached(c->normal_map);
+      if (nm->rgba.empty()) nm->set_location(c->normal_map);
+      o.rid = (cl_uint)nm->id;
+    }
+    o.pos = c->pos;
+    o.rot_quat = c->rot_quat;
+    o.isloaded = 1;
+    c->objs.push_back(o);
+  }
+  c->current_scale = c->requested_scale;
+  c->requested_scale = 1.f;
 }
 
 objects_container* object_context::make_new() {

@@ -72,3 +72,15 @@ def test_server_records_match_header(tmp_path):
             runtime.ServerConfig.hbm_fraction.offset, ctypes.sizeof(runtime.FrameStats)]
     assert got == want
     assert got[:3] == [24, 64, 56]
+
+
+def test_engine_library_exports_scene_ingest():
+    """include/ocr_obj.h (OBJ/MTL + PNG ingest) lives in libocr_engine.so."""
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "ocr_obj.h")).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(ocr_[a-z_0-9]+)\s*\(", text)))
+    assert names == ["ocr_obj_free", "ocr_obj_get", "ocr_obj_load", "ocr_obj_nparts", "ocr_obj_ntri",
+                     "ocr_png_free", "ocr_png_load"]
+    if not os.path.exists(b.FACADE_LIB):
+        b.build()
+    lib = ctypes.CDLL(b.FACADE_LIB)
+    assert not [n for n in names if not hasattr(lib, n)]
