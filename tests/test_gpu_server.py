@@ -116,3 +116,31 @@ def test_one_request_per_batch_matches(gpu_ctx):
         assert np.array_equal(o["rgba"], alone["rgba"])
         srv.release(rid)
     srv.close()
+
+
+def test_scene_added_while_serving(gpu_ctx):
+    """A scene created and registered while earlier requests are in flight
+    serves its own requests, and every result equals the request rendered
+    alone."""
+    sv, gs = _scenes(gpu_ctx)
+    srv = runtime.Server(gpu_ctx, max_batch_views=4, max_inflight=2, max_batch_tris=1 << 20,
+                         hbm_fraction=0.05, read_mask=runtime.READ_RGBA)
+    slot0 = srv.add_scene(gs[1])
+    first = [srv.submit(slot0, sv[1][1]) for _ in range(12)]
+    assert all(r is not None for r in first)
+    sc_new, view_new = scenes.random_scene(321, n_tris=150)
+    g_new = runtime.GpuScene(gpu_ctx, sc_new)
+    slot1 = srv.add_scene(g_new)
+    second = [srv.submit(slot1, view_new) for _ in range(6)]
+    assert all(r is not None for r in second)
+    got = _wait_all(srv, first + second)
+    alone0 = runtime.Frame(gpu_ctx, int(sv[1][1]["width"][0]), int(sv[1][1]["height"][0])).render_read(gs[1], sv[1][1])
+    alone1 = runtime.Frame(gpu_ctx, int(view_new["width"][0]), int(view_new["height"][0])).render_read(g_new, view_new)
+    for rid in first:
+        assert np.array_equal(got[rid][1]["rgba"], alone0["rgba"])
+        srv.release(rid)
+    for rid in second:
+        assert got[rid][0].scene_slot == slot1
+        assert np.array_equal(got[rid][1]["rgba"], alone1["rgba"])
+        srv.release(rid)
+    srv.close()
