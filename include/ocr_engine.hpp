@@ -164,6 +164,8 @@ struct texture_context {
   texture* id_to_tex(int id);
   // position of a texture id in the last upload order (get_gpu_position_id)
   int get_gpu_position_id(int id) const;
+  // the upload order and sources alloc_gpu would use, without uploading
+  void plan(object_context& ctx, std::vector<int>& order, std::vector<ocr_tex_src>& srcs);
   // uploads every texture used by an active object (or force_load) into the
   // scene's atlas, ordered by id as the reference's std::set is
   int alloc_gpu(object_context& ctx, ocr_scene* scene);
@@ -230,22 +232,33 @@ struct object_context {
   void load_active();
   // object_context::build (object_context.cpp:646): concatenates every active
   // container's objects, stamps object ids, writes descriptors, uploads
-  // geometry and atlas.  async / event are accepted and ignored: the upload is
-  // complete on return.
+  // geometry and atlas.  Synchronous by default: the new scene is current on
+  // return.  With async the upload runs on a builder thread while the
+  // current scene keeps rendering, and flip() makes it current once it has
+  // finished (the reference's second queue and ready_to_flip callback,
+  // object_context.cpp:596-612, :486-594); a further build first waits for
+  // the one in flight.  Textures must not change while a build is in flight.
   void build(bool force = false, bool async = false, compute_event* async_render_event = nullptr);
-  void build_tick(bool async = false, compute_event* async_render_event = nullptr) {}
+  void build_request() { request_dirty = true; }
+  void build_tick(bool async = false, compute_event* async_render_event = nullptr);
+  bool build_pending() const { return pending != nullptr; }
   object_context_data* fetch() { return &gpu_dat; }
   void set_clear_colour(const cl_float4& col) { gpu_dat.clear_col = col; }
   // pushes pos / rotation / scale / flags of every object (object::g_flush)
   void flush_locations(bool force = false, compute_event* render_event = nullptr);
-  void flip() {}
+  void flip();   // object_context.cpp:839-842: applies a finished async build
   void set_use_linear_rendering(cl_uint v) { gpu_dat.use_linear_rendering = v; }
   object_context_data gpu_dat;
 
  private:
   std::vector<std::unique_ptr<objects_container>> owned;
   std::vector<object*> built_objects;
+  bool request_dirty = false;
+  struct Pending;
+  std::shared_ptr<Pending> pending;
   void write_descriptors(std::vector<ocr_obj_desc>& out);
+  void write_descriptors(std::vector<ocr_obj_desc>& out, const std::vector<object*>& objs,
+                         const std::vector<int>& tex_order);
 };
 
 struct engine {

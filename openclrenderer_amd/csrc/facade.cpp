@@ -4,6 +4,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
+#include <thread>
 #include <algorithm>
 #include <map>
 #include <set>
@@ -143,7 +145,7 @@ int texture_context::get_gpu_position_id(int id) const {
   return -1;
 }
 
-int texture_context::alloc_gpu(object_context& ctx, ocr_scene* scene) {
+void texture_context::plan(object_context& ctx, std::vector<int>& order, std::vector<ocr_tex_src>& srcs) {
   // texture_context.cpp:350-420: textures used by active objects, plus
   // force_load ones, in ascending id order (a std::set in the reference)
   std::set<int> in_use;
@@ -154,14 +156,19 @@ int texture_context::alloc_gpu(object_context& ctx, ocr_scene* scene) {
   }
   for (auto& t : all_textures)
     if (t->force_load) in_use.insert(t->id);
-  texture_id_orders.assign(in_use.begin(), in_use.end());
-  mipmap_start = (int)texture_id_orders.size();
-  std::vector<ocr_tex_src> srcs;
-  for (size_t i = 0; i < texture_id_orders.size(); i++) {
-    texture* t = id_to_tex(texture_id_orders[i]);
-    t->gpu_id = (int)i;
+  order.assign(in_use.begin(), in_use.end());
+  srcs.clear();
+  for (int id : order) {
+    texture* t = id_to_tex(id);
     srcs.push_back(ocr_tex_src{t->width, t->height, t->rgba.data()});
   }
+}
+
+int texture_context::alloc_gpu(object_context& ctx, ocr_scene* scene) {
+  std::vector<ocr_tex_src> srcs;
+  plan(ctx, texture_id_orders, srcs);
+  mipmap_start = (int)texture_id_orders.size();
+  for (size_t i = 0; i < texture_id_orders.size(); i++) id_to_tex(texture_id_orders[i])->gpu_id = (int)i;
   if (!scene) return OCR_OK;
   int rc = ocr_atlas_upload(scene, srcs.data(), (uint32_t)srcs.size());
   report("ocr_atlas_upload", rc);
@@ -234,7 +241,24 @@ light_gpu light::build(light_gpu* old_dat) {
 // ---------------------------------------------------------- object_context
 object_context::object_context() {}
 
+// An async build: the scene is made on a builder thread from inputs it
+// owns; flip() makes it current once `done` is set.
+struct object_context::Pending {
+  std::thread th;
+  std::atomic<bool> done{false};
+  int rc = OCR_OK;
+  ocr_scene* scene = nullptr;
+  std::vector<object*> objs;
+  std::vector<int> order;
+  cl_uint tri_num = 0, obj_num = 0;
+};
+
 object_context::~object_context() {
+  if (pending) {
+    if (pending->th.joinable()) pending->th.join();
+    if (pending->scene) ocr_scene_destroy(pending->scene);
+    pending.reset();
+  }
   if (gpu_dat.frame) ocr_frame_destroy(gpu_dat.frame);
   if (gpu_dat.scene) ocr_scene_destroy(gpu_dat.scene);
 }
@@ -324,8 +348,13 @@ void object_context::load_active() {
 
 // generate_gpu_object_descriptor (object_context.cpp:228-339)
 void object_context::write_descriptors(std::vector<ocr_obj_desc>& out) {
+  write_descriptors(out, built_objects, tex_ctx.texture_id_orders);
+}
+
+void object_context::write_descriptors(std::vector<ocr_obj_desc>& out, const std::vector<object*>& objs,
+                                       const std::vector<int>& tex_order) {
   out.clear();
-  for (object* o : built_objects) {
+  for (object* o : objs) {
     ocr_obj_desc d;
     memset(&d, 0, sizeof(d));
     for (int k = 0; k < 3; k++) d.world_pos[k] = o->pos.s[k];
@@ -336,7 +365,9 @@ void object_context::write_descriptors(std::vector<ocr_obj_desc>& out) {
     memcpy(d.old_world_rot_quat_1, d.world_rot_quat, 16);
     memcpy(d.old_world_rot_quat_2, d.world_rot_quat, 16);
     d.scale = o->dynamic_scale;
-    int tid = tex_ctx.get_gpu_position_id((int)o->tid);
+    int tid = -1;
+    for (size_t i = 0; i < tex_order.size(); i++)
+      if (tex_order[i] == (int)o->tid) tid = (int)i;
     d.tid = tid < 0 ? 0u : (uint32_t)tid;
     d.rid = o->rid;
     d.ssid = o->ssid;
@@ -350,8 +381,13 @@ void object_context::write_descriptors(std::vector<ocr_obj_desc>& out) {
 }
 
 void object_context::build(bool force, bool async, compute_event* async_render_event) {
-  (void)force; (void)async; (void)async_render_event;
-  built_objects.clear();
+  (void)force; (void)async_render_event;
+  // object_context.cpp:656-664: a build in flight completes and flips first
+  if (pending) {
+    pending->th.join();
+    flip();
+  }
+  std::vector<object*> objs;
   std::vector<triangle> tris;
   cl_uint n = 0;
   // alloc_gpu (object_context.cpp:398-453): active containers' objects in
@@ -360,7 +396,7 @@ void object_context::build(bool force, bool async, compute_event* async_render_e
     if (!c->isactive) continue;
     for (auto& o : c->objs) {
       o.object_g_id = (cl_int)n;
-      built_objects.push_back(&o);
+      objs.push_back(&o);
       for (const triangle& t : o.tri_list) {
         tris.push_back(t);
         tris.back().vertices[0].object_id = n;
@@ -368,20 +404,63 @@ void object_context::build(bool force, bool async, compute_event* async_render_e
       n++;
     }
   }
-  tex_ctx.alloc_gpu(*this, nullptr);          // decides the texture order
+  std::vector<int> order;
+  std::vector<ocr_tex_src> srcs;
+  tex_ctx.plan(*this, order, srcs);
   std::vector<ocr_obj_desc> descs;
-  write_descriptors(descs);
+  write_descriptors(descs, objs, order);
+  auto p = std::make_shared<Pending>();
+  p->objs = std::move(objs);
+  p->order = std::move(order);
+  p->tri_num = (cl_uint)tris.size();
+  p->obj_num = (cl_uint)descs.size();
+  auto make = [p](std::vector<triangle> t, std::vector<ocr_obj_desc> d, std::vector<std::vector<uint8_t>> px,
+                  std::vector<ocr_tex_src> s) {
+    for (size_t i = 0; i < s.size() && !px.empty(); i++) s[i].rgba = px[i].data();
+    p->rc = ocr_scene_create(engine::ctx(), t.data(), (uint32_t)t.size(), d.data(), (uint32_t)d.size(), &p->scene);
+    report("ocr_scene_create", p->rc);
+    if (!p->rc) {
+      p->rc = ocr_atlas_upload(p->scene, s.data(), (uint32_t)s.size());
+      report("ocr_atlas_upload", p->rc);
+    }
+    p->done = true;
+  };
+  pending = p;
+  if (!async) {
+    make(std::move(tris), std::move(descs), {}, srcs);
+    flip();
+    return;
+  }
+  // the builder owns copies of the texture pixels
+  std::vector<std::vector<uint8_t>> px;
+  for (const ocr_tex_src& t : srcs) px.emplace_back(t.rgba, t.rgba + (size_t)t.width * t.height * 4);
+  p->th = std::thread(make, std::move(tris), std::move(descs), std::move(px), srcs);
+}
+
+void object_context::flip() {
+  if (!pending || !pending->done) return;
+  if (pending->th.joinable()) pending->th.join();
+  std::shared_ptr<Pending> p = std::move(pending);
+  pending.reset();
+  if (p->rc) {
+    if (p->scene) ocr_scene_destroy(p->scene);
+    return;
+  }
   if (gpu_dat.scene) ocr_scene_destroy(gpu_dat.scene);
-  gpu_dat.scene = nullptr;
-  ocr_scene* s = nullptr;
-  int rc = ocr_scene_create(engine::ctx(), tris.data(), (uint32_t)tris.size(), descs.data(),
-                            (uint32_t)descs.size(), &s);
-  report("ocr_scene_create", rc);
-  if (rc) return;
-  tex_ctx.alloc_gpu(*this, s);
-  gpu_dat.scene = s;
-  gpu_dat.tri_num = (cl_uint)tris.size();
-  gpu_dat.obj_num = (cl_uint)descs.size();
+  gpu_dat.scene = p->scene;
+  gpu_dat.tri_num = p->tri_num;
+  gpu_dat.obj_num = p->obj_num;
+  built_objects = std::move(p->objs);
+  tex_ctx.texture_id_orders = std::move(p->order);
+  tex_ctx.mipmap_start = (int)tex_ctx.texture_id_orders.size();
+  for (size_t i = 0; i < tex_ctx.texture_id_orders.size(); i++)
+    if (texture* t = tex_ctx.id_to_tex(tex_ctx.texture_id_orders[i])) t->gpu_id = (int)i;
+}
+
+void object_context::build_tick(bool async, compute_event* async_render_event) {
+  if (!request_dirty) return;
+  build(false, async, async_render_event);
+  request_dirty = false;
 }
 
 void object_context::flush_locations(bool force, compute_event* render_event) {

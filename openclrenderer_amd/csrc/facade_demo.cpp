@@ -28,10 +28,13 @@ static bool rd(FILE* f, void* p, size_t n) { return fread(p, 1, n, f) == n; }
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: %s scene.bin out.bin [frames]\n", argv[0]);
+    fprintf(stderr, "usage: %s scene.bin out.bin [frames] [async]\n", argv[0]);
     return 2;
   }
   int frames = argc > 3 ? atoi(argv[3]) : 2;
+  // async: build the first half of the containers, render, then add the
+  // rest with an async build while rendering, flipping when it is ready
+  const bool async_build = argc > 4 && !strcmp(argv[4], "async");
   FILE* f = fopen(argv[1], "rb");
   if (!f) return 3;
   char magic[8];
@@ -82,30 +85,33 @@ int main(int argc, char** argv) {
         self->objs.push_back(ob);
       }
     });
-    oc->set_active(true);
+    oc->set_active(!async_build || oc->id < conts.size() / 2);
   }
   engine window;
   window.set_fov(hfov);
   window.load(view.width, view.height, 1000, "demo", "cl2.cl", true);
   window.set_camera_pos(cl_float4{{view.c_pos[0], view.c_pos[1], view.c_pos[2], 0}});
   window.set_camera_rot(cl_float4{{view.c_rot[0], view.c_rot[1], view.c_rot[2], 0}});
-  context.load_active();
   // per-object placement and material after load (objects_container setters)
-  size_t ci = 0;
-  for (auto* oc : context.containers) {
-    for (size_t k = 0; k < oc->objs.size(); k++) {
-      const ocr_obj_desc& d = conts[ci][k].desc;
-      object& ob = oc->objs[k];
-      ob.set_pos(cl_float4{{d.world_pos[0], d.world_pos[1], d.world_pos[2], 0}});
-      ob.set_rot_quat(quaternion(d.world_rot_quat[0], d.world_rot_quat[1], d.world_rot_quat[2], d.world_rot_quat[3]));
-      ob.set_dynamic_scale(d.scale);
-      ob.specular = d.specular;
-      ob.spec_mult = d.spec_mult;
-      ob.diffuse = d.diffuse;
-      ob.feature_flag = d.feature_flag;
+  auto place = [&]() {
+    context.load_active();
+    size_t ci = 0;
+    for (auto* oc : context.containers) {
+      for (size_t k = 0; k < oc->objs.size(); k++) {
+        const ocr_obj_desc& d = conts[ci][k].desc;
+        object& ob = oc->objs[k];
+        ob.set_pos(cl_float4{{d.world_pos[0], d.world_pos[1], d.world_pos[2], 0}});
+        ob.set_rot_quat(quaternion(d.world_rot_quat[0], d.world_rot_quat[1], d.world_rot_quat[2], d.world_rot_quat[3]));
+        ob.set_dynamic_scale(d.scale);
+        ob.specular = d.specular;
+        ob.spec_mult = d.spec_mult;
+        ob.diffuse = d.diffuse;
+        ob.feature_flag = d.feature_flag;
+      }
+      ci++;
     }
-    ci++;
-  }
+  };
+  place();
   context.set_use_linear_rendering(view.use_linear_rendering);
   context.set_clear_colour(cl_float4{{view.clear_col[0], view.clear_col[1], view.clear_col[2], view.clear_col[3]}});
   context.build(true);
@@ -125,13 +131,31 @@ int main(int argc, char** argv) {
   light_gpu light_data = light::build();
   window.set_light_data(light_data);
   // main.cpp:243-342: the frame loop
-  for (int i = 0; i < frames; i++) {
+  auto frame = [&]() {
     window.generate_realtime_shadowing(*context.fetch());
     compute_event ev = window.draw_bulk_objs_n(*context.fetch());
     window.blit_to_screen(*context.fetch());
     window.render_block();
     window.flip();
+    context.flip();
     ev.wait();
+  };
+  for (int i = 0; i < frames; i++) frame();
+  if (async_build) {
+    for (auto* oc : context.containers) oc->set_active(true);
+    place();
+    context.build(false, true);
+    int during = 0;
+    while (context.build_pending() && during < 100000) {
+      frame();                     // the previous scene, until the flip
+      during++;
+    }
+    if (context.build_pending()) {
+      fprintf(stderr, "async build did not finish\n");
+      return 7;
+    }
+    printf("async build: %d frames rendered while it ran\n", during);
+    for (int i = 0; i < frames; i++) frame();
   }
   if (window.last_status != OCR_OK) {
     fprintf(stderr, "render failed: %s\n", ocr_last_error());

@@ -13,16 +13,21 @@ from openclrenderer_amd import build as b
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shadow", [0, 1])
-def test_facade_demo_matches_oracle(oracle, tmp_path, shadow):
+@pytest.mark.parametrize("shadow,mode", [(0, "sync"), (1, "sync"), (1, "async")])
+def test_facade_demo_matches_oracle(oracle, tmp_path, shadow, mode):
+    """mode async: half the containers are built and drawn first, the rest
+    are added by an asynchronous object_context::build while frames of the
+    previous scene are drawn, and flip() makes the full scene current."""
     sc, view = scenes.config1(shadow=shadow)
     scene_bin = tmp_path / "scene.bin"
     out_bin = tmp_path / "out.bin"
     scenes.dump_for_facade(sc, view, 60.0, str(scene_bin))
     assert os.path.exists(b.FACADE_DEMO), "build() first"
-    r = subprocess.run([b.FACADE_DEMO, str(scene_bin), str(out_bin), "2"], capture_output=True, text=True,
+    r = subprocess.run([b.FACADE_DEMO, str(scene_bin), str(out_bin), "2", mode], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    if mode == "async":
+        assert "async build:" in r.stdout, r.stdout
     raw = out_bin.read_bytes()
     used = np.frombuffer(raw[:64], dtype=abi.VIEW).copy()
     W, H = int(used["width"][0]), int(used["height"][0])
