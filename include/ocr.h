@@ -242,6 +242,12 @@ int  ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id,
  * over dst (alpha_blend, cl2.cl:1721-1728); dst's depth is unchanged.  Uses
  * each frame's last render; enqueue it on the stream those renders used. */
 int  ocr_frame_blend_with_depth(ocr_frame* src, ocr_frame* dst, void* stream);
+/* engine::do_pseudo_aa -> do_pseudo_aa (engine.cpp:1513-1516,
+ * cl2.cl:6438-6657): edge-aware smoothing of the frame's last render on
+ * `stream`, from its depth, encoded normals and colours (read as they were
+ * before the pass); RGBA8 is rewritten in place where an edge qualifies.
+ * Allocates a colour snapshot (4 bytes per pixel) on first use. */
+int  ocr_frame_pseudo_aa(ocr_frame* f, void* stream);
 /* Device pointers of the frame's buffers (zero-copy consumers). */
 int  ocr_frame_device_buffers(ocr_frame* f, void** depth, void** idkey,
                               void** rgba8, void** normals);

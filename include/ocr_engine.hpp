@@ -283,6 +283,8 @@ struct engine {
   // engine.cpp:2550-2570: src's screen over dst's where src is at least as
   // near (both contexts drawn at this engine's size)
   compute_event blend_with_depth(object_context_data& src, object_context_data& dst);
+  // engine.cpp:1513-1516: edge smoothing of the last draw's screen
+  compute_event do_pseudo_aa();
   void blit_to_screen(object_context_data& dat);
   void render_block();
   void flip() { frame_count++; }

@@ -530,6 +530,13 @@ compute_event engine::blend_with_depth(object_context_data& src, object_context_
   return record_event(nullptr);
 }
 
+compute_event engine::do_pseudo_aa() {
+  if (!obj_data || !obj_data->frame) return compute_event();
+  last_status = ocr_frame_pseudo_aa(obj_data->frame, nullptr);
+  report("ocr_frame_pseudo_aa", last_status);
+  return record_event(nullptr);
+}
+
 compute_event engine::draw_bulk_objs_n(object_context_data& dat) {
   if (!dat.scene || !dat.ensure_screen_buffers((int)width, (int)height)) return compute_event();
   sync_lights(dat);

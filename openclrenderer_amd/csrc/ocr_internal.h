@@ -263,6 +263,7 @@ struct Frame {
   hipEvent_t done[2] = {nullptr, nullptr};  // a render of the set finished
   bool entry_valid[2] = {false, false};
   uint32_t want_cut = 0;             // capacity set by ocr_frame_reserve (0 = automatic)
+  uint32_t* post_in = nullptr;       // colour snapshot of the screen-space post passes
   bool profiling = false;
   static constexpr int kEvents = 6;       // prearrange, clear, depth, id, shade boundaries
   static constexpr int kRing = 1024;      // renders averaged per profiling window

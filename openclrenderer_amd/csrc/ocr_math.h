@@ -84,6 +84,7 @@ constexpr float kInv125 = 1.0f / 125.0f;
 constexpr float kInvSsaoDiv = 1.0f / kSsaoDiv;
 constexpr float kInv255 = 1.0f / 255.0f;
 constexpr float kInvDepthFar = 1.0f / kDepthFar;
+constexpr float kInv65535 = 1.0f / 65535.0f;
 // A uniform divisor d takes the fast path when it and 1/d are comfortably
 // normal, which keeps every quotient the path forms in the normal range.
 __device__ __forceinline__ bool rcp_safe(float d) { return fabsf(d) >= 0x1p-60f && fabsf(d) <= 0x1p60f; }

@@ -598,6 +598,7 @@ void ocr_frame_destroy(ocr_frame* f) {
     dev_free(o.cuts);
     dev_free(o.counters);
   }
+  dev_free(f->post_in);
   for (int b = 0; b < 2; b++) {
     if (f->ss[b]) (void)hipStreamDestroy(f->ss[b]);
     if (f->entry[b]) (void)hipEventDestroy(f->entry[b]);

@@ -21,7 +21,7 @@ EXPORTS = [
     "ocr_shadow_pass", "ocr_scene_destroy", "ocr_scene_atlas_info", "ocr_scene_read_atlas",
     "ocr_scene_read_shadow_maps", "ocr_frame_create", "ocr_frame_destroy",
     "ocr_frame_set_profiling", "ocr_render", "ocr_frame_read", "ocr_frame_device_buffers",
-    "ocr_frame_blend_with_depth",
+    "ocr_frame_blend_with_depth", "ocr_frame_pseudo_aa",
     "ocr_frame_get_stats", "ocr_frame_count_texels", "ocr_frame_reserve",
     "ocr_batch_create", "ocr_batch_destroy", "ocr_batch_set_views", "ocr_batch_set_tuning",
     "ocr_batch_get_tuning", "ocr_batch_set_profiling", "ocr_batch_render", "ocr_batch_launch_graph",
@@ -111,6 +111,7 @@ def load_library(path=LIB_PATH):
     L.ocr_frame_read.argtypes = [vp, vp, vp, vp, vp, vp]
     L.ocr_frame_device_buffers.argtypes = [vp, pp, pp, pp, pp]
     L.ocr_frame_blend_with_depth.argtypes = [vp, vp, vp]
+    L.ocr_frame_pseudo_aa.argtypes = [vp, vp]
     L.ocr_frame_get_stats.argtypes = [vp, vp, C.POINTER(FrameStats)]
     L.ocr_frame_count_texels.argtypes = [vp, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp]
     L.ocr_batch_create.argtypes = [vp, u32, pp]
@@ -292,6 +293,10 @@ class Frame:
         """engine::blend_with_depth(src, self): src over this frame where src
         is at least as near."""
         check(self.lib.ocr_frame_blend_with_depth(src.h, self.h, stream))
+
+    def pseudo_aa(self, stream=None):
+        """engine::do_pseudo_aa on this frame's last render."""
+        check(self.lib.ocr_frame_pseudo_aa(self.h, stream))
 
     def device_buffers(self):
         ptrs = [C.c_void_p() for _ in range(4)]

@@ -106,6 +106,7 @@ class Oracle:
         L.oracle_encode_normal.restype = u
         L.oracle_texture_mod.argtypes = [fp, fp]
         L.oracle_blend_with_depth.argtypes = [u, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_pseudo_aa.argtypes = [u, u, C.c_void_p, C.c_void_p, C.c_void_p]
 
     # --------------------------------------------------------------- helpers
     @staticmethod
@@ -132,6 +133,17 @@ class Oracle:
         out = np.array(dst_rgba, dtype=np.uint8, copy=True, order="C")
         assert sd.size == dd.size == sr.size // 4 == out.size // 4
         self.lib.oracle_blend_with_depth(sd.size, sd.ctypes.data, dd.ctypes.data, sr.ctypes.data, out.ctypes.data)
+        return out
+
+    def pseudo_aa(self, depth, normals, rgba):
+        """do_pseudo_aa on a frame's host buffers (depth H x W u32, normals
+        H x W x 2 u16, rgba H x W x 4 u8); returns the new RGBA8."""
+        H, W = depth.shape
+        d = np.ascontiguousarray(depth, dtype=np.uint32)
+        n = np.ascontiguousarray(normals, dtype=np.uint16)
+        out = np.array(rgba, dtype=np.uint8, copy=True, order="C")
+        assert n.size == 2 * W * H and out.size == 4 * W * H
+        self.lib.oracle_pseudo_aa(W, H, d.ctypes.data, n.ctypes.data, out.ctypes.data)
         return out
 
     def face_sincos(self):

@@ -1323,3 +1323,77 @@ void oracle_blend_with_depth(uint32_t npix, const uint32_t* src_depth, const uin
         dst_rgba[4 * (size_t)p + 3] = (uint8_t)qz(s[3]);
     }
 }
+
+/* decode_normal(ushort2), cl2.cl:5611-5646: short_to_float is
+ * (s / 65535) * 2 - 1 (the POW2e16-1 macro binds as 65536 - 1). */
+static f3 decode_normal_u16(uint16_t sx, uint16_t sy) {
+    float vx = (float)sx / 65535.f * 2.f - 1.f;
+    float vy = (float)sy / 65535.f * 2.f - 1.f;
+    float z = (vx * vx + vy * vy) * 2.f - 1.f;
+    float r = 1.0f / sqrtf(vx * vx + vy * vy);
+    float k = sqrtf(fmaxf(1.f - z * z, 0.f));
+    return V3((vx * r) * k, (vy * r) * k, z);
+}
+
+/* do_pseudo_aa, cl2.cl:6438-6657.  Per interior pixel: the eight
+ * neighbours are split by two tests, depth difference > 100 and a normal
+ * turned by more than 20 degrees; for the first test (then the second) with
+ * exactly one differing x neighbour, one differing y neighbour and at least
+ * one differing corner, the pixel becomes 0.65 * mean(same side) + 0.35 *
+ * mean(other side), alpha 1.  The reference's avg_depth / avg_normal feed
+ * only the disabled REDUCED_AA exit and are not formed. */
+void oracle_pseudo_aa(uint32_t W, uint32_t H, const uint32_t* depth, const uint16_t* normals, uint8_t* rgba) {
+    const size_t npix = (size_t)W * H;
+    uint8_t* in = (uint8_t*)malloc(npix * 4);
+    if (!in) return;
+    memcpy(in, rgba, npix * 4);
+    /* cos(20 * 2 * M_PI / 360) in single precision: the argument rounds to
+     * 0.34906587f and its cosine to 0.9396926f (0x1.e11f64p-1) */
+    const float cosrad = (float)cos((double)(20.f * 2.f * 3.14159265f / 360.f));
+    for (uint32_t y = 1; y + 1 < H; y++)
+        for (uint32_t x = 1; x + 1 < W; x++) {
+            const size_t p = (size_t)y * W + x;
+            const uint32_t my_raw = depth[p];
+            if ((float)my_raw == MULINT_F) continue;
+            f3 mn = decode_normal_u16(normals[2 * p], normals[2 * p + 1]);
+            mn = fnorm3(mn);
+            const float my_depth = ((float)my_raw / MULINT_F) * DEPTH_FAR;
+            int nx[2] = {0, 0}, ny[2] = {0, 0}, nc[2] = {0, 0};
+            f3 mine[2] = {{0, 0, 0}, {0, 0, 0}}, theirs[2] = {{0, 0, 0}, {0, 0, 0}};
+            float ms[2] = {0, 0}, ts[2] = {0, 0};
+            for (int j = -1; j < 2; j++)
+                for (int i = -1; i < 2; i++) {
+                    if (i == 0 && j == 0) continue;
+                    const size_t q = (size_t)(y + j) * W + (x + i);
+                    const float d = ((float)depth[q] / MULINT_F) * DEPTH_FAR;
+                    const f3 fn = decode_normal_u16(normals[2 * q], normals[2 * q + 1]);
+                    const f3 val = V3((float)in[4 * q] / 255.f, (float)in[4 * q + 1] / 255.f,
+                                      (float)in[4 * q + 2] / 255.f);
+                    const int tests[2] = {fabsf(d - my_depth) > 100.f, dot3(mn, fn) < cosrad};
+                    for (int k = 0; k < 2; k++) {
+                        if (tests[k]) {
+                            if (i == j || i == -j) nc[k]++;
+                            else if (i == 1 || i == -1) nx[k]++;
+                            else if (j == 1 || j == -1) ny[k]++;
+                            theirs[k] = add3(theirs[k], val);
+                            ts[k] += 1.f;
+                        } else {
+                            mine[k] = add3(mine[k], val);
+                            ms[k] += 1.f;
+                        }
+                    }
+                }
+            for (int k = 0; k < 2; k++) {
+                if (nx[k] == 1 && ny[k] == 1 && nc[k] >= 1) {
+                    const f3 a = div3(mine[k], ms[k]), b = div3(theirs[k], ts[k]);
+                    const f3 acc = add3(mul3(a, 0.65f), mul3(b, 0.35f));
+                    rgba[4 * p] = (uint8_t)qz(acc.x);
+                    rgba[4 * p + 1] = (uint8_t)qz(acc.y);
+                    rgba[4 * p + 2] = (uint8_t)qz(acc.z);
+                    rgba[4 * p + 3] = 255;
+                    break;
+                }
+            }
+        }
+    free(in);
+}

@@ -137,6 +137,13 @@ void  oracle_texture_mod(const float vt[2], float out[2]);
 void  oracle_blend_with_depth(uint32_t npix, const uint32_t* src_depth, const uint32_t* dst_depth,
                               const uint8_t* src_rgba, uint8_t* dst_rgba);
 
+/* do_pseudo_aa (cl2.cl:6438-6657), engine::do_pseudo_aa (engine.cpp:1513-1516)
+ * on one frame: depth (u32), encoded normals (ushort2 per pixel) and RGBA8
+ * of a render.  Reads the colours as they were before the pass and rewrites
+ * rgba in place where an edge qualifies. */
+void  oracle_pseudo_aa(uint32_t width, uint32_t height, const uint32_t* depth, const uint16_t* normals,
+                       uint8_t* rgba);
+
 #ifdef __cplusplus
 }
 #endif

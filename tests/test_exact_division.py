@@ -33,7 +33,7 @@ def checker(tmp_path_factory):
 
 def test_constant_divisors_exhaustive(checker):
     procs = [subprocess.Popen([checker, "const", b], stdout=subprocess.PIPE, text=True)
-             for b in ("9", "125", "2.5", "255", "350000", "300", "500")]
+             for b in ("9", "125", "2.5", "255", "350000", "300", "500", "65535")]
     outs = [(p.communicate(timeout=600)[0], p.returncode) for p in procs]
     for out, rc in outs:
         sys.stdout.write(out)

@@ -33,3 +33,23 @@ def test_blend_rejects_mismatched_frames(gpu_ctx):
         fb.blend_from(fa)
     with pytest.raises(runtime.OcrError):
         fa.blend_from(fa)
+
+
+@pytest.mark.parametrize("w,h,seed", [(37, 23, 4), (640, 480, 5), (1920, 1080, 6)])
+def test_pseudo_aa_matches_oracle(gpu_ctx, oracle, w, h, seed):
+    sc, view = scenes.random_scene(700 + seed, n_tris=600, width=w, height=h)
+    g = runtime.GpuScene(gpu_ctx, sc)
+    fr = runtime.Frame(gpu_ctx, w, h)
+    before = fr.render_read(g, view)
+    fr.pseudo_aa()
+    after = fr.read()
+    want = oracle.pseudo_aa(before["depth"], before["normals"], before["rgba"])
+    assert np.array_equal(after["rgba"], want.reshape(after["rgba"].shape))
+    assert np.array_equal(after["depth"], before["depth"])
+    if w >= 640:
+        assert (after["rgba"] != before["rgba"]).any()
+    # a second pass reads the smoothed colours
+    fr.pseudo_aa()
+    again = fr.read()
+    assert np.array_equal(again["rgba"], oracle.pseudo_aa(after["depth"], after["normals"], after["rgba"]).reshape(
+        again["rgba"].shape))
