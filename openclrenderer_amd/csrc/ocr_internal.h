@@ -240,9 +240,9 @@ struct Scene {
 struct Frame {
   Ctx* ctx = nullptr;
   uint32_t width = 0, height = 0;
-  // Output and cut-list sets, double-buffered: render N uses set N mod 2 on
-  // its own stream, so it can overlap render N-1.  The pointers below alias
-  // the set of the last render.
+  // Output and cut-list sets: render N uses set N mod kSets on that set's
+  // stream, so it can overlap the renders before it.  The pointers below
+  // alias the set of the last render.
   struct OutBuf {
     uint32_t* depth = nullptr;
     unsigned long long* idkey = nullptr;
@@ -252,16 +252,17 @@ struct Frame {
     uint32_t* counters = nullptr;
     uint32_t cap_cut = 0;
   };
-  OutBuf out[2];
+  static constexpr int kSets = 2;   // three measured slower: renders compete for VALU
+  OutBuf out[kSets];
   int cur = 0;
   uint32_t* depth = nullptr;
   unsigned long long* idkey = nullptr;
   uint32_t* rgba = nullptr;
   uint32_t* normals = nullptr;
-  hipStream_t ss[2] = {nullptr, nullptr};   // the streams renders of each set run on
-  hipEvent_t entry[2] = {nullptr, nullptr}; // caller's stream when a render of the set was issued
-  hipEvent_t done[2] = {nullptr, nullptr};  // a render of the set finished
-  bool entry_valid[2] = {false, false};
+  hipStream_t ss[kSets] = {};   // the streams renders of each set run on
+  hipEvent_t entry[kSets] = {}; // caller's stream when a render of the set was issued
+  hipEvent_t done[kSets] = {};  // a render of the set finished
+  bool entry_valid[kSets] = {};
   uint32_t want_cut = 0;             // capacity set by ocr_frame_reserve (0 = automatic)
   uint32_t* post_in = nullptr;       // colour snapshot of the screen-space post passes
   bool profiling = false;

@@ -573,7 +573,7 @@ int ocr_frame_create(ocr_ctx* ctx, uint32_t width, uint32_t height, ocr_frame** 
   }
   f->select(0);
   hipError_t e = hipSuccess;
-  for (int b = 0; b < 2 && e == hipSuccess; b++) {
+  for (int b = 0; b < Frame::kSets && e == hipSuccess; b++) {
     e = hipStreamCreateWithFlags(&f->ss[b], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&f->entry[b], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&f->done[b], hipEventDisableTiming);
@@ -599,7 +599,7 @@ void ocr_frame_destroy(ocr_frame* f) {
     dev_free(o.counters);
   }
   dev_free(f->post_in);
-  for (int b = 0; b < 2; b++) {
+  for (int b = 0; b < Frame::kSets; b++) {
     if (f->ss[b]) (void)hipStreamDestroy(f->ss[b]);
     if (f->entry[b]) (void)hipEventDestroy(f->entry[b]);
     if (f->done[b]) (void)hipEventDestroy(f->done[b]);
@@ -662,7 +662,7 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   if (!s->lights && s->nlights) return fail(OCR_ERR_INVALID, "ocr_render: lights missing");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   // this render's buffer set: the one the previous render did not use
-  const int b = f->cur ^ 1;
+  const int prev = f->cur, b = (f->cur + 1) % Frame::kSets;
   int rc = frame_alloc_set(f, b);
   if (!rc) rc = ensure_frame_list(f, b, s->ntri);
   if (rc) return rc;
@@ -675,13 +675,13 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   const bool prof = f->profiling && f->n_profiled < Frame::kRing;
   hipEvent_t* ev = prof ? &f->ev[(size_t)f->n_profiled * Frame::kEvents] : nullptr;
   hipStream_t sb = f->ss[b];
-  // Render N runs on the stream of its set once render N-2 (same buffers)
-  // and everything enqueued on the caller's stream before render N-1 was
-  // issued have completed (the event recorded at that point), so it can
-  // overlap render N-1; the caller's stream then waits for it.  The first
-  // render follows the caller's stream.
+  // Render N runs on the stream of its set once render N-kSets (same
+  // buffers, same stream) and everything enqueued on the caller's stream
+  // before render N-1 was issued have completed (the event recorded at that
+  // point), so it can overlap the renders before it; the caller's stream
+  // then waits for it.  The first render follows the caller's stream.
   OCR_HIP_CHECK(hipEventRecord(f->entry[b], st));
-  OCR_HIP_CHECK(hipStreamWaitEvent(sb, f->entry_valid[b ^ 1] ? f->entry[b ^ 1] : f->entry[b], 0));
+  OCR_HIP_CHECK(hipStreamWaitEvent(sb, f->entry_valid[prev] ? f->entry[prev] : f->entry[b], 0));
   // and the scene's last stream-ordered descriptor update
   OCR_HIP_CHECK(s->wait_geometry(sb));
   f->entry_valid[b] = true;
