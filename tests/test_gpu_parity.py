@@ -1,9 +1,9 @@
 """Parity of the HIP path with the oracle, through the C ABI.
 
 Bar: depth and triangle-id buffers bit-exact; RGBA8 within 1/255 per channel
-(the arithmetic contract makes it exact in practice, the tolerance is the
-north-star bar); encoded normals within 1 unit; shadow maps and atlas bytes
-bit-exact.
+(the arithmetic contract makes it exact in practice; the tolerance is the
+path's stated bar for shaded colour); encoded normals within 1 unit; shadow
+maps and atlas bytes bit-exact.
 """
 import numpy as np
 import pytest
