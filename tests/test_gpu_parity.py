@@ -276,3 +276,17 @@ def test_config3_full_size_is_stable(gpu_ctx):
     assert vis.sum() > 1_000_000
     assert out["tri_id"][vis].max() < sc.ntri
     assert np.array_equal(vis, out["depth"] != UINT_MAX)
+
+
+def test_largest_frame_matches_oracle(gpu_ctx, oracle):
+    """5120 x 2880 (14.7 M pixels, near the 16,000,000-pixel limit): the whole
+    frame against the oracle, and the buffer invariants."""
+    w, h = 5120, 2880
+    sc, view = scenes.random_scene(909, n_tris=300, width=w, height=h)
+    g, fr, out = render_gpu(gpu_ctx, sc, view)
+    ref = oracle.render(sc, view, nthreads=16)
+    assert_frame_parity(out, ref, "5120x2880")
+    vis = out["tri_id"] != UINT_MAX
+    assert vis.sum() > 100000
+    assert np.array_equal(vis, out["depth"] != UINT_MAX)
+    assert out["tri_id"][vis].max() < sc.ntri
