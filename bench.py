@@ -188,8 +188,10 @@ def run_blend(ctx, g, fr, sc, view, W, H, reps=100):
     """blend_screens_with_depth (ocr_frame_blend_with_depth) of a second
     view of the scene onto the frame: average device time per launch and the
     HBM rate of its 20 algorithmic bytes per pixel (two depths and two
-    colours read, one colour written); and do_pseudo_aa (ocr_frame_pseudo_aa,
-    colour snapshot + kernel) on the frame."""
+    colours read, one colour written); do_pseudo_aa (ocr_frame_pseudo_aa,
+    colour snapshot + kernel) and do_motion_blur (ocr_frame_motion_blur,
+    snapshot + kernel; the previous camera 40 units to the side and turned
+    by 0.005 rad, frame ids alternating) on the frame."""
     from openclrenderer_amd import runtime
     other = view.copy()
     other["c_pos"][0, 0] += 40.0
@@ -212,8 +214,18 @@ def run_blend(ctx, g, fr, sc, view, W, H, reps=100):
     ev.stop()
     aa_us = ev.ms() * 1e3 / reps
     fr.render_read(g, view)
+    c_pos_old = view["c_pos"][0] + np.array([40.0, 0.0, 0.0, 0.0], np.float32)
+    c_rot_old = view["c_rot"][0] + np.array([0.0, 0.005, 0.0, 0.0], np.float32)
+    fr.motion_blur(g, view, c_pos_old, c_rot_old, 0)
+    hip_sync()
+    ev.start()
+    for i in range(reps):
+        fr.motion_blur(g, view, c_pos_old, c_rot_old, i)
+    ev.stop()
+    mb_us = ev.ms() * 1e3 / reps
+    fr.render_read(g, view)
     return {"kernel": "k_blend_depth", "us_per_launch": us, "bytes_per_pixel": 20,
-            "pseudo_aa_us_per_call": aa_us,
+            "pseudo_aa_us_per_call": aa_us, "motion_blur_us_per_call": mb_us,
             "achieved_GBs": 20.0 * W * H / (us * 1e-6) / 1e9, "peak_GBs": HBM_PEAK_GBS,
             "note": "back-to-back launches over the same two frames, whose 20 B/pixel stay in the "
                     "256 MB Infinity Cache, so the rate can exceed the HBM peak"}

@@ -199,6 +199,12 @@ int  ocr_scene_read_atlas(ocr_scene* s, uint8_t* pages, uint32_t* nums,
                           uint32_t* sizes);
 int  ocr_scene_read_shadow_maps(ocr_scene* s, uint32_t* dynamic_maps,
                                 uint32_t* static_maps);
+/* The motion-blur poses of objects [first, first+n): 16 floats each,
+ * old_world_pos_1, old_world_pos_2, old_world_rot_quat_1,
+ * old_world_rot_quat_2 (obj_g_descriptor.hpp:15-18).  They start as the
+ * descriptors given to ocr_scene_create; ocr_scene_update_objs leaves them
+ * alone (as object::g_flush does) and ocr_frame_motion_blur records them. */
+int  ocr_scene_read_motion(ocr_scene* s, uint32_t first, uint32_t n, float* poses);
 
 /* ---- frames: offscreen framebuffer (replaces cl_gl_interop_texture) ----- */
 
@@ -248,6 +254,24 @@ int  ocr_frame_blend_with_depth(ocr_frame* src, ocr_frame* dst, void* stream);
  * before the pass); RGBA8 is rewritten in place where an edge qualifies.
  * Allocates a colour snapshot (4 bytes per pixel) on first use. */
 int  ocr_frame_pseudo_aa(ocr_frame* f, void* stream);
+/* engine::do_motion_blur -> do_motion_blur (engine.cpp:1518-1536,
+ * cl2.cl:6715-6858): blurs the frame's last render, drawn from scene `s`
+ * with `view`, on `stream`.  Each covered pixel is carried back to where it
+ * was in the previous frame (camera c_pos_old / c_rot_old, its object's pose
+ * recorded by the last pass of the other frame_id parity) and replaced by
+ * the mean of up to 50 bilinear samples of the colours (as they were before
+ * the pass) along the line from there, scaled by `strength`;
+ * camera_contribution weighs camera motion against object motion.  Pixels
+ * whose earlier position lies behind the previous camera's near plane keep
+ * their colour; background pixels are left alone (the reference writes
+ * nothing there).  Records every drawn object's current pose in the slot
+ * the other parity reads (old_world_*_2 when frame_id is even, _1 when odd).
+ * Passes on one scene race unless ordered on one stream.  Allocates a
+ * colour snapshot (4 bytes per pixel) on first use. */
+int  ocr_frame_motion_blur(ocr_frame* f, ocr_scene* s, const ocr_view* view,
+                           const float c_pos_old[4], const float c_rot_old[4],
+                           uint32_t frame_id, float strength,
+                           float camera_contribution, void* stream);
 /* Device pointers of the frame's buffers (zero-copy consumers). */
 int  ocr_frame_device_buffers(ocr_frame* f, void** depth, void** idkey,
                               void** rgba8, void** normals);

@@ -178,9 +178,16 @@ struct object_context_data {
   cl_uint frame_id = 0;
   cl_uint use_linear_rendering = 1;
   cl_float4 clear_col{{0, 0, 0, 0}};
+  // camera of the last swap_buffers (object_context.hpp:89-90), the
+  // previous frame's camera for do_motion_blur
+  cl_float4 c_pos_old{{0, 0, 0, 0}};
+  cl_float4 c_rot_old{{0, 0, 0, 0}};
   ocr_frame* frame = nullptr;        // offscreen framebuffer (ensure_screen_buffers)
   int s_w = 0, s_h = 0;
   bool ensure_screen_buffers(int w, int h);
+  // object_context.cpp:17-25: records engine::c_pos / c_rot as the old
+  // camera (the depth double buffer is the frame's own)
+  void swap_buffers();
   // host read-back of the last blit (blit_to_screen)
   std::vector<uint32_t> depth, tri_id;
   std::vector<uint8_t> rgba8;
@@ -285,6 +292,9 @@ struct engine {
   compute_event blend_with_depth(object_context_data& src, object_context_data& dst);
   // engine.cpp:1513-1516: edge smoothing of the last draw's screen
   compute_event do_pseudo_aa();
+  // engine.cpp:1518-1536: motion blur of dat's last draw against its old
+  // camera and each object's pose of the last pass (frame parity dat.frame_id)
+  compute_event do_motion_blur(object_context_data& dat, cl_float strength, cl_float camera_contribution);
   void blit_to_screen(object_context_data& dat);
   void render_block();
   void flip() { frame_count++; }

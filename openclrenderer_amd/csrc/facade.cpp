@@ -537,6 +537,19 @@ compute_event engine::do_pseudo_aa() {
   return record_event(nullptr);
 }
 
+void object_context_data::swap_buffers() {
+  c_pos_old = engine::c_pos;
+  c_rot_old = engine::c_rot;
+}
+
+compute_event engine::do_motion_blur(object_context_data& dat, cl_float strength, cl_float camera_contribution) {
+  if (!dat.scene || !dat.frame) return compute_event();
+  last_status = ocr_frame_motion_blur(dat.frame, dat.scene, &last_view, dat.c_pos_old.s, dat.c_rot_old.s,
+                                      dat.frame_id, strength, camera_contribution, nullptr);
+  report("ocr_frame_motion_blur", last_status);
+  return record_event(nullptr);
+}
+
 compute_event engine::draw_bulk_objs_n(object_context_data& dat) {
   if (!dat.scene || !dat.ensure_screen_buffers((int)width, (int)height)) return compute_event();
   sync_lights(dat);

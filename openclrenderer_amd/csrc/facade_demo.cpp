@@ -28,13 +28,16 @@ static bool rd(FILE* f, void* p, size_t n) { return fread(p, 1, n, f) == n; }
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: %s scene.bin out.bin [frames] [async]\n", argv[0]);
+    fprintf(stderr, "usage: %s scene.bin out.bin [frames] [async|motion]\n", argv[0]);
     return 2;
   }
   int frames = argc > 3 ? atoi(argv[3]) : 2;
   // async: build the first half of the containers, render, then add the
   // rest with an async build while rendering, flipping when it is ready
   const bool async_build = argc > 4 && !strcmp(argv[4], "async");
+  // motion: after the frames, swap_buffers (main.cpp:283), move the camera
+  // by (40, -25, 30), draw once more and apply do_motion_blur (main.cpp:270)
+  const bool motion = argc > 4 && !strcmp(argv[4], "motion");
   FILE* f = fopen(argv[1], "rb");
   if (!f) return 3;
   char magic[8];
@@ -157,6 +160,22 @@ int main(int argc, char** argv) {
     printf("async build: %d frames rendered while it ran\n", during);
     for (int i = 0; i < frames; i++) frame();
   }
+  if (motion) {
+    context.fetch()->swap_buffers();
+    cl_float4 p = engine::c_pos;
+    p.s[0] += 40.f;
+    p.s[1] -= 25.f;
+    p.s[2] += 30.f;
+    window.set_camera_pos(p);
+    window.generate_realtime_shadowing(*context.fetch());
+    window.draw_bulk_objs_n(*context.fetch());
+    compute_event ev = window.do_motion_blur(*context.fetch(), 1.f, 1.f);
+    window.blit_to_screen(*context.fetch());
+    window.render_block();
+    window.flip();
+    ev.wait();
+    printf("motion blur: frame_id %u\n", context.fetch()->frame_id);
+  }
   if (window.last_status != OCR_OK) {
     fprintf(stderr, "render failed: %s\n", ocr_last_error());
     return 5;
@@ -166,6 +185,7 @@ int main(int argc, char** argv) {
   if (!o) return 6;
   ocr_view used = view;
   used.fov_const = window.get_fov_const();
+  for (int k = 0; k < 4; k++) used.c_pos[k] = engine::c_pos.s[k];
   fwrite(&used, sizeof(used), 1, o);
   fwrite(dat->depth.data(), 4, dat->depth.size(), o);
   fwrite(dat->tri_id.data(), 4, dat->tri_id.size(), o);

@@ -200,6 +200,10 @@ struct Scene {
   TriPos* world = nullptr;           // world-space copy, refreshed when objects change
   TriAttr* tri_attr = nullptr;
   ObjDesc* objs = nullptr;
+  // motion-blur poses, 16 floats per object: old_world_pos_1, old_world_pos_2,
+  // old_world_rot_quat_1, old_world_rot_quat_2 (obj_g_descriptor.hpp:15-18);
+  // set at creation, then written only by the motion-blur pass
+  float* motion = nullptr;
   Atlas atlas;
   ocr_light* lights = nullptr;
   float* light_lin = nullptr;        // 4 floats per light
@@ -298,6 +302,8 @@ hipError_t launch_clear(const PassArgs& a, hipStream_t s);
 hipError_t launch_prearrange(const PassArgs& a, hipStream_t s);
 hipError_t launch_depth(const PassArgs& a, int blocks_per_cu, hipStream_t s);
 hipError_t launch_id(const PassArgs& a, int blocks_per_cu, hipStream_t s);
+// sin/cos of a rotation's three angles: binary64 evaluation rounded once
+void sincos_of(const float ang[3], SinCos& sc);
 hipError_t launch_shade(const PassArgs& a, const SinCos face[6], hipStream_t s);
 hipError_t launch_shade_count(const PassArgs& a, const SinCos face[6], uint32_t* tex_bits,
                               uint32_t* shadow_bits, uint32_t* static_bits, hipStream_t s);

@@ -48,7 +48,7 @@ static void dev_free(T*& p) {
 }
 
 // sin/cos of an angle: binary64 evaluation rounded once to binary32.
-static void sincos_of(const float ang[3], SinCos& sc) {
+void sincos_of(const float ang[3], SinCos& sc) {
   for (int i = 0; i < 3; i++) {
     sc.s[i] = (float)sin((double)ang[i]);
     sc.c[i] = (float)cos((double)ang[i]);
@@ -233,14 +233,20 @@ int ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri, cons
     pos[t].oid = T.vertices[0].object_id;
   }
   std::vector<ObjDesc> od(nobj);
+  std::vector<float> motion((size_t)nobj * 16);
   for (uint32_t o = 0; o < nobj; o++) {
     pack_obj(objs[o], od[o]);
     s->max_tid = std::max(s->max_tid, objs[o].tid);
+    memcpy(&motion[16 * (size_t)o], objs[o].old_world_pos_1, 16);
+    memcpy(&motion[16 * (size_t)o + 4], objs[o].old_world_pos_2, 16);
+    memcpy(&motion[16 * (size_t)o + 8], objs[o].old_world_rot_quat_1, 16);
+    memcpy(&motion[16 * (size_t)o + 12], objs[o].old_world_rot_quat_2, 16);
   }
   int rc;
   if ((rc = dev_alloc(&s->tri_pos, ntri, "triangles")) || (rc = dev_alloc(&s->world, ntri, "world-space triangles")) ||
       (rc = dev_alloc(&s->tri_attr, ntri, "attributes")) ||
-      (rc = dev_alloc(&s->objs, nobj, "descriptors")) || (rc = dev_alloc(&s->sh_counters, kCounterWords, "counters"))) {
+      (rc = dev_alloc(&s->objs, nobj, "descriptors")) || (rc = dev_alloc(&s->motion, (size_t)nobj * 16, "motion poses")) ||
+      (rc = dev_alloc(&s->sh_counters, kCounterWords, "counters"))) {
     ocr_scene_destroy(s);
     return rc;
   }
@@ -248,6 +254,7 @@ int ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri, cons
   if (ntri) e = hipMemcpy(s->tri_pos, pos.data(), sizeof(TriPos) * ntri, hipMemcpyHostToDevice);
   if (e == hipSuccess && ntri) e = hipMemcpy(s->tri_attr, attr.data(), sizeof(TriAttr) * ntri, hipMemcpyHostToDevice);
   if (e == hipSuccess && nobj) e = hipMemcpy(s->objs, od.data(), sizeof(ObjDesc) * nobj, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nobj) e = hipMemcpy(s->motion, motion.data(), 4 * motion.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = launch_world(s, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipEventCreateWithFlags(&s->written, hipEventDisableTiming);
@@ -324,6 +331,7 @@ void ocr_scene_destroy(ocr_scene* s) {
   dev_free(s->world);
   dev_free(s->tri_attr);
   dev_free(s->objs);
+  dev_free(s->motion);
   free_atlas(s->atlas);
   dev_free(s->lights);
   dev_free(s->light_lin);
@@ -551,6 +559,16 @@ int ocr_scene_read_shadow_maps(ocr_scene* s, uint32_t* dynamic_maps, uint32_t* s
   if (static_maps && s->n_static)
     OCR_HIP_CHECK(hipMemcpy(static_maps, s->static_maps, per * s->n_static, hipMemcpyDeviceToHost));
   if (ov) return fail(OCR_ERR_UNSUPPORTED, "shadow pass: more raster samples than one pass can hold");
+  return OCR_OK;
+}
+
+int ocr_scene_read_motion(ocr_scene* s, uint32_t first, uint32_t n, float* poses) {
+  if (!s || (n && !poses) || first > s->nobj || n > s->nobj - first)
+    return fail(OCR_ERR_INVALID, "ocr_scene_read_motion: range outside the scene");
+  if (!n) return OCR_OK;
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  OCR_HIP_CHECK(hipDeviceSynchronize());
+  OCR_HIP_CHECK(hipMemcpy(poses, s->motion + 16 * (size_t)first, 64 * (size_t)n, hipMemcpyDeviceToHost));
   return OCR_OK;
 }
 

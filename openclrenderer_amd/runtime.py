@@ -19,9 +19,9 @@ EXPORTS = [
     "ocr_create", "ocr_destroy", "ocr_last_error", "ocr_query_free_hbm",
     "ocr_scene_create", "ocr_scene_update_objs", "ocr_scene_update_objs_async", "ocr_atlas_upload", "ocr_lights_upload",
     "ocr_shadow_pass", "ocr_scene_destroy", "ocr_scene_atlas_info", "ocr_scene_read_atlas",
-    "ocr_scene_read_shadow_maps", "ocr_frame_create", "ocr_frame_destroy",
+    "ocr_scene_read_shadow_maps", "ocr_scene_read_motion", "ocr_frame_create", "ocr_frame_destroy",
     "ocr_frame_set_profiling", "ocr_render", "ocr_frame_read", "ocr_frame_device_buffers",
-    "ocr_frame_blend_with_depth", "ocr_frame_pseudo_aa",
+    "ocr_frame_blend_with_depth", "ocr_frame_pseudo_aa", "ocr_frame_motion_blur",
     "ocr_frame_get_stats", "ocr_frame_count_texels", "ocr_frame_reserve",
     "ocr_batch_create", "ocr_batch_destroy", "ocr_batch_set_views", "ocr_batch_set_tuning",
     "ocr_batch_get_tuning", "ocr_batch_set_profiling", "ocr_batch_render", "ocr_batch_launch_graph",
@@ -102,6 +102,7 @@ def load_library(path=LIB_PATH):
     L.ocr_scene_atlas_info.argtypes = [vp] + [C.POINTER(C.c_uint32)] * 4
     L.ocr_scene_read_atlas.argtypes = [vp, vp, vp, vp]
     L.ocr_scene_read_shadow_maps.argtypes = [vp, vp, vp]
+    L.ocr_scene_read_motion.argtypes = [vp, u32, u32, vp]
     L.ocr_frame_create.argtypes = [vp, u32, u32, pp]
     L.ocr_frame_destroy.argtypes = [vp]
     L.ocr_frame_destroy.restype = None
@@ -112,6 +113,7 @@ def load_library(path=LIB_PATH):
     L.ocr_frame_device_buffers.argtypes = [vp, pp, pp, pp, pp]
     L.ocr_frame_blend_with_depth.argtypes = [vp, vp, vp]
     L.ocr_frame_pseudo_aa.argtypes = [vp, vp]
+    L.ocr_frame_motion_blur.argtypes = [vp, vp, vp, vp, vp, u32, C.c_float, C.c_float, vp]
     L.ocr_frame_get_stats.argtypes = [vp, vp, C.POINTER(FrameStats)]
     L.ocr_frame_count_texels.argtypes = [vp, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp]
     L.ocr_batch_create.argtypes = [vp, u32, pp]
@@ -180,6 +182,7 @@ class GpuScene:
         check(self.lib.ocr_scene_create(ctx.h, _p(tris), tris.shape[0], _p(objs), objs.shape[0],
                                         C.byref(h)))
         self.h = h
+        self.nobj = objs.shape[0]
         self.upload_textures(scene.textures)
         self.upload_lights(scene.lights, scene.light_buffer_dim)
 
@@ -228,6 +231,13 @@ class GpuScene:
         stat = np.zeros((max(self.n_static, 1), 6, L, L), np.uint32)
         check(self.lib.ocr_scene_read_shadow_maps(self.h, _p(dyn), _p(stat)))
         return dyn, stat
+
+    def read_motion(self):
+        """(nobj, 4, 4) float32: old_world_pos_1, old_world_pos_2,
+        old_world_rot_quat_1, old_world_rot_quat_2 of every object."""
+        out = np.zeros((self.nobj, 4, 4), np.float32)
+        check(self.lib.ocr_scene_read_motion(self.h, 0, self.nobj, _p(out)))
+        return out
 
     def close(self):
         if self.h:
@@ -297,6 +307,15 @@ class Frame:
     def pseudo_aa(self, stream=None):
         """engine::do_pseudo_aa on this frame's last render."""
         check(self.lib.ocr_frame_pseudo_aa(self.h, stream))
+
+    def motion_blur(self, gscene, view, c_pos_old, c_rot_old, frame_id, strength=1.0,
+                    camera_contribution=1.0, stream=None):
+        """engine::do_motion_blur on this frame's last render of gscene."""
+        v = np.ascontiguousarray(view, dtype=abi.VIEW)
+        po = np.ascontiguousarray(np.resize(np.asarray(c_pos_old, np.float32), 4))
+        ro = np.ascontiguousarray(np.resize(np.asarray(c_rot_old, np.float32), 4))
+        check(self.lib.ocr_frame_motion_blur(self.h, gscene.h, _p(v), _p(po), _p(ro), frame_id,
+                                             strength, camera_contribution, stream))
 
     def device_buffers(self):
         ptrs = [C.c_void_p() for _ in range(4)]

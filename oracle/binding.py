@@ -107,6 +107,9 @@ class Oracle:
         L.oracle_texture_mod.argtypes = [fp, fp]
         L.oracle_blend_with_depth.argtypes = [u, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_pseudo_aa.argtypes = [u, u, C.c_void_p, C.c_void_p, C.c_void_p]
+        vp = C.c_void_p
+        L.oracle_motion_blur.argtypes = [vp, vp, fp, vp, vp, vp, vp, u, vp, u, u, C.c_float, C.c_float, vp]
+        L.oracle_sample_linear.argtypes = [vp, u, u, C.c_float, C.c_float, fp]
 
     # --------------------------------------------------------------- helpers
     @staticmethod
@@ -144,6 +147,34 @@ class Oracle:
         out = np.array(rgba, dtype=np.uint8, copy=True, order="C")
         assert n.size == 2 * W * H and out.size == 4 * W * H
         self.lib.oracle_pseudo_aa(W, H, d.ctypes.data, n.ctypes.data, out.ctypes.data)
+        return out
+
+    def motion_blur(self, view, frame, tris, objs, c_pos_old, c_rot_old, frame_id, strength=1.0,
+                    camera_contribution=1.0):
+        """do_motion_blur on a frame's host buffers (dict with depth, tri_id,
+        rgba as Frame.read returns).  Returns (new RGBA8, descriptors with the
+        recorded poses); the inputs are left unchanged."""
+        v = np.ascontiguousarray(view)
+        W, H = int(v["width"][0]), int(v["height"][0])
+        d = np.ascontiguousarray(frame["depth"], dtype=np.uint32)
+        t = np.ascontiguousarray(frame["tri_id"], dtype=np.uint32)
+        out = np.array(frame["rgba"], dtype=np.uint8, copy=True, order="C")
+        assert d.size == t.size == W * H and out.size == 4 * W * H
+        tr = np.ascontiguousarray(tris)
+        ob = np.array(objs, copy=True, order="C")
+        cam = camera_sincos(v["c_rot"][0])
+        cam_old = camera_sincos(np.asarray(c_rot_old, np.float32))
+        po, ppo = self._f(np.asarray(c_pos_old, np.float32)[:3])
+        self.lib.oracle_motion_blur(v.ctypes.data, C.addressof(cam), ppo, C.addressof(cam_old), d.ctypes.data,
+                                    t.ctypes.data, tr.ctypes.data, tr.shape[0], ob.ctypes.data, ob.shape[0],
+                                    frame_id, strength, camera_contribution, out.ctypes.data)
+        return out, ob
+
+    def sample_linear(self, rgba, u, v):
+        img = np.ascontiguousarray(rgba, dtype=np.uint8)
+        H, W = img.shape[:2]
+        out, po = self._f(np.zeros(4, np.float32))
+        self.lib.oracle_sample_linear(img.ctypes.data, W, H, u, v, po)
         return out
 
     def face_sincos(self):

@@ -1397,3 +1397,134 @@ void oracle_pseudo_aa(uint32_t W, uint32_t H, const uint32_t* depth, const uint1
         }
     free(in);
 }
+
+/* read_imagef with CLK_FILTER_LINEAR on unnormalised coordinates (u, v),
+ * OpenCL 1.2 section 8.2: i0 = floor(u - 0.5), a = frac(u - 0.5) (and the
+ * same for v), the four texels weighted (1-a)(1-b), a(1-b), (1-a)b, ab and
+ * summed in that order; RGBA8 texels read as c / 255; indices beyond the
+ * image edge read the edge. */
+static void sample_linear(const uint8_t* img, uint32_t W, uint32_t H, float u, float v, float out[4]) {
+    const float tx = u - 0.5f, ty = v - 0.5f;
+    const float fx = floorf(tx), fy = floorf(ty);
+    const float a = tx - fx, b = ty - fy;
+    int i0 = (int)fx, j0 = (int)fy, i1 = i0 + 1, j1 = j0 + 1;
+    i0 = i0 < 0 ? 0 : (i0 > (int)W - 1 ? (int)W - 1 : i0);
+    i1 = i1 < 0 ? 0 : (i1 > (int)W - 1 ? (int)W - 1 : i1);
+    j0 = j0 < 0 ? 0 : (j0 > (int)H - 1 ? (int)H - 1 : j0);
+    j1 = j1 < 0 ? 0 : (j1 > (int)H - 1 ? (int)H - 1 : j1);
+    const uint8_t* t00 = img + 4 * ((size_t)j0 * W + i0);
+    const uint8_t* t10 = img + 4 * ((size_t)j0 * W + i1);
+    const uint8_t* t01 = img + 4 * ((size_t)j1 * W + i0);
+    const uint8_t* t11 = img + 4 * ((size_t)j1 * W + i1);
+    const float w00 = (1.f - a) * (1.f - b), w10 = a * (1.f - b), w01 = (1.f - a) * b, w11 = a * b;
+    for (int c = 0; c < 4; c++)
+        out[c] = ((w00 * ((float)t00[c] / 255.f) + w10 * ((float)t10[c] / 255.f)) + w01 * ((float)t01[c] / 255.f))
+               + w11 * ((float)t11[c] / 255.f);
+}
+
+void oracle_sample_linear(const uint8_t* rgba, uint32_t W, uint32_t H, float u, float v, float out[4]) {
+    sample_linear(rgba, W, H, u, v, out);
+}
+
+/* (int) of a float as the GPU converts it: NaN -> 0, saturating (the
+ * reference's implicit conversion is undefined out of range). */
+static int to_int_sat(float m) {
+    if (m != m) return 0;
+    if (m >= 2147483648.f) return 0x7FFFFFFF;
+    if (m <= -2147483648.f) return (int)0x80000000u;
+    return (int)m;
+}
+
+/* do_motion_blur, cl2.cl:6715-6858, as engine::do_motion_blur binds it
+ * (engine.cpp:1518-1536), over the frame's host buffers.  Pixels are visited
+ * in row order; the pose writes (cl2.cl:6770-6785) go to the slot that no
+ * pixel of the pass reads, so the order does not matter. */
+void oracle_motion_blur(const ocr_view* view, const oracle_sincos* cam, const float c_pos_old[3],
+                        const oracle_sincos* cam_old, const uint32_t* depth, const uint32_t* tri_id,
+                        const ocr_triangle* tris, uint32_t ntri, ocr_obj_desc* objs, uint32_t nobj,
+                        uint32_t frame_id, float strength, float camera_contribution, uint8_t* rgba) {
+    const uint32_t W = view->width, H = view->height;
+    const float fov = view->fov_const;
+    const size_t npix = (size_t)W * H;
+    uint8_t* in = (uint8_t*)malloc(npix * 4);
+    if (!in) return;
+    memcpy(in, rgba, npix * 4);
+    const f3 c_pos = ld3(view->c_pos), c_old = ld3(c_pos_old);
+    const int even = (frame_id & 1) == 0;
+    for (uint32_t y = 0; y < H; y++)
+        for (uint32_t x = 0; x < W; x++) {
+            const size_t p = (size_t)y * W + x;
+            const uint32_t dbuf_val = depth[p], t = tri_id[p];
+            if (dbuf_val == 0xFFFFFFFFu || t >= ntri) continue;   /* background: nothing written */
+            const uint32_t o_id = tris[t].vertices[0].object_id;
+            if (o_id >= nobj) continue;
+            ocr_obj_desc* G = &objs[o_id];
+            const float ldepth = ((float)dbuf_val / MULINT_F) * DEPTH_FAR;   /* idcalc */
+            const float actual_depth = ldepth;
+            const f3 local_position = V3(((float)x - (float)W / 2.0f) * actual_depth / fov,
+                                         ((float)y - (float)H / 2.0f) * actual_depth / fov, actual_depth);
+            const f3 global_position = add3(back_rot(local_position, cam), c_pos);
+            const f4 q = {G->world_rot_quat[0], G->world_rot_quat[1], G->world_rot_quat[2], G->world_rot_quat[3]};
+            const f3 object_local = back_rot_quat(sub3(global_position, ld3(G->world_pos)), q);
+            f3 old_world_pos;
+            f4 old_world_quat;
+            if (even) {
+                old_world_pos = ld3(G->old_world_pos_1);
+                old_world_quat = (f4){G->old_world_rot_quat_1[0], G->old_world_rot_quat_1[1],
+                                      G->old_world_rot_quat_1[2], G->old_world_rot_quat_1[3]};
+                memcpy(G->old_world_pos_2, G->world_pos, 12);
+                memcpy(G->old_world_rot_quat_2, G->world_rot_quat, 16);
+            } else {
+                old_world_pos = ld3(G->old_world_pos_2);
+                old_world_quat = (f4){G->old_world_rot_quat_2[0], G->old_world_rot_quat_2[1],
+                                      G->old_world_rot_quat_2[2], G->old_world_rot_quat_2[3]};
+                memcpy(G->old_world_pos_1, G->world_pos, 12);
+                memcpy(G->old_world_rot_quat_1, G->world_rot_quat, 16);
+            }
+            f3 last_frame_pos = add3(rot_quat(object_local, old_world_quat), old_world_pos);
+            f3 last_frame_no_camera = rot(last_frame_pos, c_pos, cam);
+            last_frame_pos = rot(last_frame_pos, c_old, cam_old);
+            last_frame_no_camera = depth_project1(last_frame_no_camera, (float)W, (float)H, fov);
+            last_frame_pos = depth_project1(last_frame_pos, (float)W, (float)H, fov);
+            if (last_frame_pos.z < 20.f) {      /* depth_icutoff: the pixel's own colour */
+                float c[4];
+                sample_linear(in, W, H, (float)x + 0.5f, (float)y + 0.5f, c);
+                for (int k = 0; k < 4; k++) rgba[4 * p + k] = (uint8_t)qz(c[k]);
+                continue;
+            }
+            const float csx = (float)x, csy = (float)y;
+            float tmx = csx - last_frame_pos.x, tmy = csy - last_frame_pos.y;
+            const float ncx = csx - last_frame_no_camera.x, ncy = csy - last_frame_no_camera.y;
+            tmx = camera_contribution * tmx + (1.f - camera_contribution) * ncx;
+            tmy = camera_contribution * tmy + (1.f - camera_contribution) * ncy;
+            tmx *= strength;
+            tmy *= strength;
+            int n = to_int_sat(fmaxf(fabsf(tmx), fabsf(tmy)) + 1.f);
+            const int bound = 50;
+            if (n > bound) {
+                tmx /= (float)n;
+                tmy /= (float)n;
+                tmx *= (float)bound;
+                tmy *= (float)bound;
+                n = bound;
+            }
+            float dx = 0.f, dy = 0.f;
+            if (n != 0) {
+                dx = tmx / (float)n;
+                dy = tmy / (float)n;
+            }
+            float cx = csx - tmx / 2.f, cy = csy - tmy / 2.f;
+            float acc[4] = {0.f, 0.f, 0.f, 0.f}, fcount = 0.f;
+            for (int i = 0; i < n; i++, cx += dx, cy += dy) {
+                if (cx < 0.f || cx >= (float)W || cy < 0.f || cy >= (float)H) continue;
+                float c[4];
+                sample_linear(in, W, H, cx + 0.5f, cy + 0.5f, c);
+                for (int k = 0; k < 4; k++) acc[k] += c[k] * 1.f;
+                fcount += 1.f;
+            }
+            if (fcount != 0.f)
+                for (int k = 0; k < 4; k++) acc[k] /= fcount;
+            for (int k = 0; k < 4; k++) rgba[4 * p + k] = (uint8_t)qz(acc[k]);
+        }
+    free(in);
+}

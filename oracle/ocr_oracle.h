@@ -144,6 +144,21 @@ void  oracle_blend_with_depth(uint32_t npix, const uint32_t* src_depth, const ui
 void  oracle_pseudo_aa(uint32_t width, uint32_t height, const uint32_t* depth, const uint16_t* normals,
                        uint8_t* rgba);
 
+/* do_motion_blur (cl2.cl:6715-6858), engine::do_motion_blur
+ * (engine.cpp:1518-1536) on one frame: its depth, triangle ids
+ * (OCR_NO_TRIANGLE where nothing is visible) and RGBA8 (rewritten in place,
+ * from the colours as they were before the pass), the scene's triangles (for
+ * each visible triangle's object id) and descriptors (old_world_* written
+ * for every object with a covered pixel), the view it was drawn with and its
+ * camera sin/cos, and the previous camera. */
+void  oracle_motion_blur(const ocr_view* view, const oracle_sincos* cam, const float c_pos_old[3],
+                         const oracle_sincos* cam_old, const uint32_t* depth, const uint32_t* tri_id,
+                         const ocr_triangle* tris, uint32_t ntri, ocr_obj_desc* objs, uint32_t nobj,
+                         uint32_t frame_id, float strength, float camera_contribution, uint8_t* rgba);
+/* The bilinear sampler it reads the colours with (unnormalised u, v). */
+void  oracle_sample_linear(const uint8_t* rgba, uint32_t width, uint32_t height, float u, float v,
+                           float out[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,11 +13,14 @@ from openclrenderer_amd import build as b
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shadow,mode", [(0, "sync"), (1, "sync"), (1, "async")])
+@pytest.mark.parametrize("shadow,mode", [(0, "sync"), (1, "sync"), (1, "async"), (0, "motion")])
 def test_facade_demo_matches_oracle(oracle, tmp_path, shadow, mode):
     """mode async: half the containers are built and drawn first, the rest
     are added by an asynchronous object_context::build while frames of the
-    previous scene are drawn, and flip() makes the full scene current."""
+    previous scene are drawn, and flip() makes the full scene current.
+    mode motion: swap_buffers records the camera, the camera moves, one more
+    frame is drawn and engine::do_motion_blur blurs it (objects still, so the
+    motion is the camera's alone)."""
     sc, view = scenes.config1(shadow=shadow)
     scene_bin = tmp_path / "scene.bin"
     out_bin = tmp_path / "out.bin"
@@ -39,4 +42,14 @@ def test_facade_demo_matches_oracle(oracle, tmp_path, shadow, mode):
     ref = oracle.render(sc, used)
     assert np.array_equal(depth, ref["depth"])
     assert np.array_equal(tri, ref["tri_id"])
+    if mode == "motion":
+        assert "motion blur: frame_id 3" in r.stdout, r.stdout
+        objs = sc.objs.copy()            # generate_gpu_object_descriptor: old poses = current
+        for k in (1, 2):
+            objs[f"old_world_pos_{k}"] = objs["world_pos"]
+            objs[f"old_world_rot_quat_{k}"] = objs["world_rot_quat"]
+        blurred, _ = oracle.motion_blur(used, ref, sc.tris, objs, view["c_pos"][0], view["c_rot"][0], 3)
+        assert np.abs(rgba.astype(int) - blurred.astype(int)).max() <= 1
+        assert (blurred != ref["rgba"]).any()
+        return
     assert np.abs(rgba.astype(int) - ref["rgba"].astype(int)).max() <= 1
