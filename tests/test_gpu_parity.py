@@ -290,3 +290,58 @@ def test_largest_frame_matches_oracle(gpu_ctx, oracle):
     assert vis.sum() > 100000
     assert np.array_equal(vis, out["depth"] != UINT_MAX)
     assert out["tri_id"][vis].max() < sc.ntri
+
+
+def _screen_filling_scene(copies=0):
+    """One two-sided object of large triangles in front of a camera at the
+    origin: two crossing screen-filling triangles (vertices far outside the
+    view, so bounding boxes are clamped and one triangle is the whole frame's
+    8.3 M samples at 3840 x 2160), one crossing the near plane, a one-pixel
+    sliver across the full width, and `copies` extra copies of the first."""
+    big = [((-1e4, -1e4, 1000.0), (1e4, -1e4, 1000.0), (0.0, 2e4, 1000.0)),
+           ((-1e4, -1e4, 600.0), (1e4, -1e4, 1400.0), (0.0, 2e4, 1000.0)),
+           ((-400.0, 100.0, 10.0), (-50.0, 100.0, 300.0), (-400.0, 400.0, 300.0)),
+           ((-1e4, 10.0, 300.0), (1e4, 10.6, 300.0), (1e4, 11.4, 300.5))]
+    big += [big[0]] * copies
+    tris = np.zeros(len(big), abi.TRIANGLE)
+    rng = np.random.default_rng(31)
+    for t, vs in enumerate(big):
+        for v, p in enumerate(vs):
+            tris["v"]["pos"][t, v, :3] = p
+            n = rng.normal(size=3) + (0.0, 0.0, -3.0)
+            tris["v"]["normal"][t, v, :3] = n / np.linalg.norm(n)
+            tris["v"]["vt"][t, v] = rng.uniform(-2.0, 3.0, 2)
+    objs = scenes._desc((0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0), 1.0, flags=abi.FEATURE_TWO_SIDED)
+    lights = np.concatenate([scenes.make_light((200.0, -300.0, 100.0)),
+                             scenes.make_light((-400.0, 200.0, 700.0), col=(0.4, 0.6, 1.0), radius=900.0)])
+    tex = scenes.value_noise_texture(256, 77)
+    return scenes.Scene(tris, objs, lights, [tex], 1024, "screen-filling")
+
+
+def test_screen_filling_triangles_match_oracle(gpu_ctx, oracle):
+    sc = _screen_filling_scene()
+    view = scenes.make_view(3840, 2160, (0.0, 0.0, 0.0), hfov=70.0)
+    g, fr, out = render_gpu(gpu_ctx, sc, view)
+    ref = oracle.render(sc, view, nthreads=16)
+    assert_frame_parity(out, ref, "screen-filling")
+    # covered but for the last row and column, which calc_min_max's
+    # bounding box (max edge exclusive) never reaches
+    assert (out["depth"] != UINT_MAX).mean() > 0.999
+    assert np.all(out["depth"][:, -1] == UINT_MAX) and np.all(out["depth"][-1, :] == UINT_MAX)
+    assert set(np.unique(out["tri_id"]).tolist()) == {0, 1, 2, 3, UINT_MAX}
+
+
+def test_more_than_2_31_samples_is_refused_then_recovers(gpu_ctx, oracle):
+    # 300 copies of a whole-frame triangle: ~2.5 G samples at 3840 x 2160
+    sc = _screen_filling_scene(copies=300)
+    view = scenes.make_view(3840, 2160, (0.0, 0.0, 0.0), hfov=70.0)
+    g = runtime.GpuScene(gpu_ctx, sc)
+    fr = runtime.Frame(gpu_ctx, 3840, 2160)
+    fr.render(g, view)
+    with pytest.raises(runtime.OcrError) as ei:
+        fr.read()
+    assert ei.value.code == abi.OCR_ERR_UNSUPPORTED
+    # the same frame renders a scene within the limit afterwards
+    sc2 = _screen_filling_scene()
+    out = fr.render_read(runtime.GpuScene(gpu_ctx, sc2), view)
+    assert_frame_parity(out, oracle.render(sc2, view, nthreads=16), "after refusal")
