@@ -180,8 +180,9 @@ def run_animated(g, fr, sc, view, steps, warmup):
     g.update_objs(0, sc.objs)
     ms = (t1 - t0) * 1e3 / steps
     return {"value": 1e3 / ms, "unit": "frames/s", "ms_per_step": ms, "objects_updated_per_frame": int(sc.objs.shape[0]),
-            "note": "every frame: all descriptors re-uploaded and the world-space copy refreshed on the render "
-                    "stream, then rendered; same scene and view otherwise"}
+            "note": "every frame: all descriptors re-uploaded (a new geometry version: descriptors and the "
+                    "world-space copy, built while the previous render shades), then rendered; same scene and "
+                    "view otherwise"}
 
 
 def run_blend(ctx, g, fr, sc, view, W, H, reps=100):
@@ -311,7 +312,11 @@ def run_plain(args):
                 "frac": n / (st["ms_shade"] * 1e-3) / VALU_PEAK_WAVE_INSTS,
                 "frac_isolated": n / (st_iso["ms_shade"] * 1e-3) / VALU_PEAK_WAVE_INSTS}
 
-    animated = run_animated(g, fr, sc, view, args.steps, args.warmup) if args.config == 2 else None
+    # the legs below time whole calls; per-pass timing events stay off
+    fr.set_profiling(False)
+    # the first update of a scene allocates its second geometry version and
+    # the first waits across streams create their events: a longer warm-up
+    animated = run_animated(g, fr, sc, view, max(args.steps, 200), max(args.warmup, 20)) if args.config == 2 else None
     post = run_blend(ctx, g, fr, sc, view, W, H) if args.config == 2 else None
 
     if args.calib:

@@ -165,14 +165,19 @@ int  ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri,
 /* object::g_flush (object.cpp:652-857): rewrite descriptors [first, first+n). */
 int  ocr_scene_update_objs(ocr_scene* s, uint32_t first, uint32_t n,
                            const ocr_obj_desc* objs);
-/* The same, ordered on `stream` like the reference's enqueued descriptor
- * writes (object::g_flush, object.cpp:652-857, on the render queue): the
- * descriptors are staged in pinned memory (the host array may be reused on
- * return), copied and the scene's world-space copy refreshed on `stream`;
- * renders, shadow passes and batches issued afterwards, on any stream, wait
- * for it.  Enqueue it on the stream the scene's renders use (work already
- * enqueued there that reads the scene completes first); the host blocks only
- * when four earlier updates of the scene are still in flight. */
+/* The same, stream-ordered like the reference's enqueued descriptor writes
+ * (object::g_flush, object.cpp:652-857, on the render queue): renders,
+ * shadow passes, batches and post passes issued afterwards, on any stream,
+ * see the update, and work enqueued on `stream` afterwards follows it; work
+ * issued before it keeps seeing the state before it.  The scene holds two
+ * versions of its descriptors and world-space triangles: the update builds
+ * the one no pass issued since the previous update reads (a copy of the
+ * current version plus the new records, on a stream of the scene's own,
+ * after the passes that read that version have finished), so it overlaps
+ * the renders in flight.  The descriptors are staged in pinned memory (the
+ * host array may be reused on return); the host blocks only when four
+ * earlier updates of the scene are still in flight.  The second version is
+ * allocated on the first call (40 bytes per triangle + 64 per object). */
 int  ocr_scene_update_objs_async(ocr_scene* s, uint32_t first, uint32_t n,
                                  const ocr_obj_desc* objs, void* stream);
 /* texture_context::alloc_gpu (texture_context.cpp:350-517) +

@@ -28,6 +28,12 @@ struct ocr_batch {
   std::vector<DevView> host_views;
   DevView* d_views = nullptr;
   DevScene* d_scenes = nullptr;
+  // the geometry version of each scene that d_scenes points at, and the
+  // pinned staging of its refresh (reused once the last refresh copy ended)
+  std::vector<int> bound_ver;
+  DevScene* ds_host = nullptr;
+  hipEvent_t ds_done = nullptr;
+  bool ds_pending = false;
   uint32_t tri_slots = 0, pix_slots = 0, tiles = 0;
   uint64_t sum_tris = 0, sum_pix = 0;
   CutTri* cuts = nullptr;
@@ -124,6 +130,49 @@ static hipError_t wait_scenes(ocr_batch* b, hipStream_t st) {
   return hipSuccess;
 }
 
+// Holds the version locks of the batch's scenes (address order, so two
+// batches sharing scenes cannot deadlock).
+static void lock_scenes(const std::vector<Scene*>& scenes, std::vector<std::unique_lock<std::mutex>>& locks) {
+  std::vector<Scene*> order = scenes;
+  std::sort(order.begin(), order.end());
+  order.erase(std::unique(order.begin(), order.end()), order.end());
+  for (Scene* s : order) locks.emplace_back(s->ver_mu);
+}
+
+// With the scenes locked: points d_scenes at the scenes' current geometry
+// versions (a copy on `st` when a stream-ordered update made another one
+// current since the last launch) and makes `st` follow the scenes' updates
+// and shadow passes.
+static hipError_t bind_scenes(ocr_batch* b, hipStream_t st) {
+  bool changed = false;
+  for (size_t i = 0; i < b->scenes.size(); i++) changed |= b->bound_ver[i] != b->scenes[i]->ver;
+  if (changed) {
+    if (b->ds_pending) {
+      hipError_t e = hipEventSynchronize(b->ds_done);
+      if (e != hipSuccess) return e;
+    }
+    for (size_t i = 0; i < b->scenes.size(); i++) {
+      b->ds_host[i] = dev_scene(b->scenes[i]);
+      b->bound_ver[i] = b->scenes[i]->ver;
+    }
+    hipError_t e = hipMemcpyAsync(b->d_scenes, b->ds_host, sizeof(DevScene) * b->scenes.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(b->ds_done, st);
+    if (e != hipSuccess) return e;
+    b->ds_pending = true;
+  }
+  return wait_scenes(b, st);
+}
+
+// With the scenes locked, after the launch: the scenes' current versions
+// are read by work on `st`.
+static hipError_t note_scene_reads(ocr_batch* b, hipStream_t st) {
+  for (Scene* s : b->scenes) {
+    hipError_t e = s->note_read(st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 static hipError_t enqueue_batch(ocr_batch* b, hipStream_t st) {
   const PassArgs pa = batch_pass(b);
   hipError_t e = launch_clear(pa, st);
@@ -151,6 +200,8 @@ int ocr_batch_create(ocr_ctx* ctx, uint32_t max_views, ocr_batch** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&b->counters, 4 * kCounterWords);
   if (e == hipSuccess) e = hipMemset(b->counters, 0, 4 * kCounterWords);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->ds_host, sizeof(DevScene) * max_views);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ds_done, hipEventDisableTiming);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     ocr_batch_destroy(b);
@@ -170,6 +221,8 @@ void ocr_batch_destroy(ocr_batch* b) {
   if (b->cuts) (void)hipFree(b->cuts);
   if (b->counters) (void)hipFree(b->counters);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  if (b->ds_host) (void)hipHostFree(b->ds_host);
+  if (b->ds_done) (void)hipEventDestroy(b->ds_done);
   for (hipEvent_t e : b->ev)
     if (e) (void)hipEventDestroy(e);
   delete b;
@@ -204,16 +257,29 @@ int ocr_batch_set_views(ocr_batch* b, ocr_scene* const* scenes, const ocr_view* 
     sum_tris += s->ntri;
     sum_pix += (uint64_t)views[i].width * views[i].height;
   }
-  std::vector<DevScene> ds(uniq.size());
   for (size_t i = 0; i < uniq.size(); i++) {
     if (!uniq[i]->lights) {
       int rc = ocr_lights_upload(static_cast<ocr_scene*>(uniq[i]), nullptr, 0, uniq[i]->lbuf_dim);
       if (rc) return rc;
     }
-    ds[i] = dev_scene(uniq[i]);
+  }
+  std::vector<DevScene> ds(uniq.size());
+  std::vector<int> vers(uniq.size());
+  {
+    std::vector<std::unique_lock<std::mutex>> locks;
+    lock_scenes(uniq, locks);
+    for (size_t i = 0; i < uniq.size(); i++) {
+      ds[i] = dev_scene(uniq[i]);
+      vers[i] = uniq[i]->ver;
+    }
   }
   drop_graph(b);
   b->scenes = uniq;
+  b->bound_ver = vers;
+  if (b->ds_pending) {
+    OCR_HIP_CHECK(hipEventSynchronize(b->ds_done));   // a refresh copy still reading the old table
+    b->ds_pending = false;
+  }
   b->host_views = dv;
   b->nviews = n;
   b->sum_tris = sum_tris;
@@ -257,10 +323,13 @@ int ocr_batch_render(ocr_batch* b, void* stream) {
   int rc = ensure_lists(b);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
-  OCR_HIP_CHECK(wait_scenes(b, st));
+  std::vector<std::unique_lock<std::mutex>> locks;
+  lock_scenes(b->scenes, locks);
+  OCR_HIP_CHECK(bind_scenes(b, st));
   const bool prof = b->profiling && b->n_profiled < kBatchRing;
   if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled], st));
   OCR_HIP_CHECK(enqueue_batch(b, st));
+  OCR_HIP_CHECK(note_scene_reads(b, st));
   if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled++ + 1], st));
   return OCR_OK;
 }
@@ -285,10 +354,13 @@ int ocr_batch_launch_graph(ocr_batch* b, void* stream) {
     OCR_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
   hipStream_t st = (hipStream_t)stream;
-  OCR_HIP_CHECK(wait_scenes(b, st));
+  std::vector<std::unique_lock<std::mutex>> locks;
+  lock_scenes(b->scenes, locks);
+  OCR_HIP_CHECK(bind_scenes(b, st));   // the graph reads d_scenes when it runs
   const bool prof = b->profiling && b->n_profiled < kBatchRing;
   if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled], st));
   OCR_HIP_CHECK(hipGraphLaunch(b->exec, st));
+  OCR_HIP_CHECK(note_scene_reads(b, st));
   if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled++ + 1], st));
   return OCR_OK;
 }
@@ -341,9 +413,17 @@ int ocr_batch_autotune(ocr_batch* b, void* stream, uint32_t reps, ocr_batch_tuni
       ocr_batch_tuning t{bl, tl, {0, 0}};
       if ((status = ocr_batch_set_tuning(b, &t))) break;
       if ((status = ocr_batch_launch_graph(b, stream))) break;
-      hipError_t e = hipEventRecord(e0, st);
-      for (uint32_t r = 0; r < reps && e == hipSuccess; r++)
-        e = hipGraphLaunch(b->exec, st);
+      hipError_t e;
+      {
+        // the timed replays read the scenes like any launch
+        std::vector<std::unique_lock<std::mutex>> locks;
+        lock_scenes(b->scenes, locks);
+        e = bind_scenes(b, st);
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        for (uint32_t r = 0; r < reps && e == hipSuccess; r++)
+          e = hipGraphLaunch(b->exec, st);
+        if (e == hipSuccess) e = note_scene_reads(b, st);
+      }
       if (e == hipSuccess) e = hipEventRecord(e1, st);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
       float ms = 0.f;

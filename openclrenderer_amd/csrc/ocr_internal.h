@@ -232,6 +232,39 @@ struct Scene {
   hipEvent_t upd_done[kUpdRing] = {};
   bool upd_used[kUpdRing] = {};
   int upd_next = 0;
+  // Geometry versions.  objs / world above alias objs_v[ver] / world_v[ver].
+  // A stream-ordered update builds the other version on upd_stream, after
+  // the passes that read that version (its readers) have finished, and then
+  // makes it current, so it does not wait for the passes still reading the
+  // current one.  Every pass that reads objs or world enqueues its kernels
+  // and records its reader event with ver_mu held (a server's dispatcher
+  // thread renders while the application updates).
+  struct Reader {
+    hipStream_t stream;
+    hipEvent_t ev;   // recorded on `stream` after the stream's last pass that read the version
+  };
+  ObjDesc* objs_v[2] = {};
+  TriPos* world_v[2] = {};
+  int ver = 0;
+  std::vector<Reader> readers[2];
+  std::vector<hipEvent_t> ev_pool;
+  hipStream_t upd_stream = nullptr;
+  std::mutex ver_mu;
+  // Records that work enqueued on `st` so far reads the current version.
+  hipError_t note_read(hipStream_t st) {
+    for (Reader& r : readers[ver])
+      if (r.stream == st) return hipEventRecord(r.ev, st);
+    hipEvent_t e = nullptr;
+    if (!ev_pool.empty()) {
+      e = ev_pool.back();
+      ev_pool.pop_back();
+    } else {
+      hipError_t c = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+      if (c != hipSuccess) return c;
+    }
+    readers[ver].push_back(Reader{st, e});
+    return hipEventRecord(e, st);
+  }
   // what the render and batch paths wait on before they read the scene
   hipError_t wait_geometry(hipStream_t st) const {
     return geom_valid ? hipStreamWaitEvent(st, geom_written, 0) : hipSuccess;

@@ -358,7 +358,9 @@ extern "C" int ocr_frame_motion_blur(ocr_frame* f, ocr_scene* s, const ocr_view*
     }
   }
   hipStream_t st = (hipStream_t)stream;
-  // descriptors written by stream-ordered updates are read here too
+  // reads the scene's current descriptors (the version of its last
+  // stream-ordered update), registered as their reader below
+  std::lock_guard<std::mutex> lk(s->ver_mu);
   OCR_HIP_CHECK(s->wait_geometry(st));
   OCR_HIP_CHECK(hipMemcpyAsync(f->post_in, f->rgba, 4 * npix, hipMemcpyDeviceToDevice, st));
   MotionArgs a;
@@ -386,5 +388,6 @@ extern "C" int ocr_frame_motion_blur(ocr_frame* f, ocr_scene* s, const ocr_view*
   a.camera_contribution = camera_contribution;
   hipLaunchKernelGGL(k_motion_blur, dim3((a.W + 15) / 16, (a.H + 15) / 16), dim3(256), 0, st, a);
   OCR_HIP_CHECK(hipGetLastError());
+  OCR_HIP_CHECK(s->note_read(st));
   return OCR_OK;
 }

@@ -250,6 +250,8 @@ int ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri, cons
     ocr_scene_destroy(s);
     return rc;
   }
+  s->objs_v[0] = s->objs;
+  s->world_v[0] = s->world;
   hipError_t e = hipSuccess;
   if (ntri) e = hipMemcpy(s->tri_pos, pos.data(), sizeof(TriPos) * ntri, hipMemcpyHostToDevice);
   if (e == hipSuccess && ntri) e = hipMemcpy(s->tri_attr, attr.data(), sizeof(TriAttr) * ntri, hipMemcpyHostToDevice);
@@ -277,6 +279,7 @@ int ocr_scene_update_objs(ocr_scene* s, uint32_t first, uint32_t n, const ocr_ob
     s->max_tid = std::max(s->max_tid, objs[o].tid);
   }
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  std::lock_guard<std::mutex> lk(s->ver_mu);
   // renders run on streams of their frames: let those reading the scene end
   OCR_HIP_CHECK(hipDeviceSynchronize());
   OCR_HIP_CHECK(hipMemcpy(s->objs + first, od.data(), sizeof(ObjDesc) * n, hipMemcpyHostToDevice));
@@ -291,6 +294,25 @@ int ocr_scene_update_objs_async(ocr_scene* s, uint32_t first, uint32_t n, const 
     return fail(OCR_ERR_INVALID, "ocr_scene_update_objs_async: range outside the scene");
   if (!n) return OCR_OK;
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  std::lock_guard<std::mutex> lk(s->ver_mu);
+  const int nv = 1 - s->ver;   // the version this update builds
+  if (!s->objs_v[nv]) {
+    int rc;
+    if ((rc = dev_alloc(&s->objs_v[nv], s->nobj, "descriptors (second version)")) ||
+        (rc = dev_alloc(&s->world_v[nv], s->ntri, "world-space triangles (second version)"))) {
+      dev_free(s->objs_v[nv]);
+      dev_free(s->world_v[nv]);
+      return rc;
+    }
+  }
+  if (!s->upd_stream) OCR_HIP_CHECK(hipStreamCreateWithFlags(&s->upd_stream, hipStreamNonBlocking));
+  hipStream_t us = s->upd_stream;
+  // the passes that read version nv when it was current end first
+  for (const Scene::Reader& r : s->readers[nv]) {
+    OCR_HIP_CHECK(hipStreamWaitEvent(us, r.ev, 0));
+    s->ev_pool.push_back(r.ev);
+  }
+  s->readers[nv].clear();
   const int k = s->upd_next;
   s->upd_next = (k + 1) % Scene::kUpdRing;
   if (!s->upd_host[k]) {
@@ -308,12 +330,23 @@ int ocr_scene_update_objs_async(ocr_scene* s, uint32_t first, uint32_t n, const 
     s->max_tid = std::max(s->max_tid, objs[o].tid);
   }
   hipStream_t st = (hipStream_t)stream;
-  OCR_HIP_CHECK(hipMemcpyAsync(s->objs + first, s->upd_host[k], sizeof(ObjDesc) * n, hipMemcpyHostToDevice, st));
-  OCR_HIP_CHECK(hipEventRecord(s->upd_done[k], st));
+  // Work enqueued on `stream` before this call that writes what the update
+  // reads (nothing of the scene's own) is not waited for: the update reads
+  // only the current version and the staged records.  Version nv starts as
+  // a copy of the current one, then takes the new records.
+  OCR_HIP_CHECK(hipMemcpyAsync(s->objs_v[nv], s->objs_v[s->ver], sizeof(ObjDesc) * s->nobj,
+                               hipMemcpyDeviceToDevice, us));
+  OCR_HIP_CHECK(hipMemcpyAsync(s->objs_v[nv] + first, s->upd_host[k], sizeof(ObjDesc) * n, hipMemcpyHostToDevice, us));
+  OCR_HIP_CHECK(hipEventRecord(s->upd_done[k], us));
   s->upd_used[k] = true;
-  OCR_HIP_CHECK(launch_world(s, st));
-  OCR_HIP_CHECK(hipEventRecord(s->geom_written, st));
+  s->ver = nv;
+  s->objs = s->objs_v[nv];
+  s->world = s->world_v[nv];
+  OCR_HIP_CHECK(launch_world(s, us));
+  OCR_HIP_CHECK(hipEventRecord(s->geom_written, us));
   s->geom_valid = true;
+  // later work on the caller's stream sees the update
+  OCR_HIP_CHECK(hipStreamWaitEvent(st, s->geom_written, 0));
   return OCR_OK;
 }
 
@@ -328,9 +361,15 @@ void ocr_scene_destroy(ocr_scene* s) {
   if (!s) return;
   (void)hipSetDevice(s->ctx->device);
   dev_free(s->tri_pos);
-  dev_free(s->world);
+  if (!s->world_v[0]) s->world_v[0] = s->world;   // creation failed before the versions were set
+  if (!s->objs_v[0]) s->objs_v[0] = s->objs;
+  for (int v = 0; v < 2; v++) {
+    dev_free(s->world_v[v]);
+    dev_free(s->objs_v[v]);
+  }
+  s->world = nullptr;
+  s->objs = nullptr;
   dev_free(s->tri_attr);
-  dev_free(s->objs);
   dev_free(s->motion);
   free_atlas(s->atlas);
   dev_free(s->lights);
@@ -348,6 +387,10 @@ void ocr_scene_destroy(ocr_scene* s) {
     }
     if (s->upd_host[k]) (void)hipHostFree(s->upd_host[k]);
   }
+  for (int v = 0; v < 2; v++)
+    for (const Scene::Reader& r : s->readers[v]) (void)hipEventDestroy(r.ev);
+  for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+  if (s->upd_stream) (void)hipStreamDestroy(s->upd_stream);
   delete s;
 }
 
@@ -515,6 +558,7 @@ int ocr_shadow_pass(ocr_scene* s, void* stream) {
   if (s->n_shadow == 0 && !(s->static_dirty && s->n_static)) return OCR_OK;
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(s->ver_mu);   // reads the current geometry version
   OCR_HIP_CHECK(s->wait_geometry(st));   // descriptor updates enqueued on another stream
   {
     // A triangle reaches the faces its three vertices fall in, each as at
@@ -542,6 +586,7 @@ int ocr_shadow_pass(ocr_scene* s, void* stream) {
     if (l.shadow && l.is_static) kk++;
   }
   s->static_dirty = false;
+  OCR_HIP_CHECK(s->note_read(st));
   OCR_HIP_CHECK(hipEventRecord(s->written, st));
   s->written_valid = true;
   return OCR_OK;
@@ -700,7 +745,9 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   // then waits for it.  The first render follows the caller's stream.
   OCR_HIP_CHECK(hipEventRecord(f->entry[b], st));
   OCR_HIP_CHECK(hipStreamWaitEvent(sb, f->entry_valid[prev] ? f->entry[prev] : f->entry[b], 0));
-  // and the scene's last stream-ordered descriptor update
+  // and the scene's last stream-ordered descriptor update; the render reads
+  // the current geometry version, registered as its reader below
+  std::lock_guard<std::mutex> lk(s->ver_mu);
   OCR_HIP_CHECK(s->wait_geometry(sb));
   f->entry_valid[b] = true;
   f->select(b);
@@ -722,6 +769,7 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   // pass, wherever that was enqueued (the raster passes do not read them).
   OCR_HIP_CHECK(s->wait_shadow_maps(sb));
   OCR_HIP_CHECK(launch_shade(pa, s->ctx->face_sc, sb));
+  OCR_HIP_CHECK(s->note_read(sb));
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[5], sb));
   OCR_HIP_CHECK(hipEventRecord(f->done[b], sb));
   OCR_HIP_CHECK(hipStreamWaitEvent(st, f->done[b], 0));
@@ -777,6 +825,9 @@ int ocr_frame_count_texels(ocr_scene* s, const ocr_view* view, ocr_frame* f, uin
     return fail(OCR_ERR_INVALID, "ocr_frame_count_texels: view size differs from the frame");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   hipStream_t st = (hipStream_t)stream;
+  // reads the current geometry version until the synchronisation below
+  std::lock_guard<std::mutex> lk(s->ver_mu);
+  OCR_HIP_CHECK(s->wait_geometry(st));
   const size_t tex_words = ((size_t)s->atlas.npages * OCR_ATLAS_DIM * OCR_ATLAS_DIM + 31) / 32;
   const size_t per = (size_t)6 * s->lbuf_dim * s->lbuf_dim;
   const size_t sh_words = (per * std::max<uint32_t>(s->n_shadow, 1) + 31) / 32;

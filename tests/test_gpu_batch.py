@@ -93,3 +93,25 @@ def test_batch_with_config1_and_shadows(gpu_ctx, oracle):
     assert np.array_equal(got["tri_id"], ref["tri_id"])
     assert np.abs(got["rgba"].astype(int) - ref["rgba"].astype(int)).max() <= 1
     b.close()
+
+
+def test_batch_follows_stream_ordered_updates(gpu_ctx, oracle):
+    """A batch (plain and graph launches) after ocr_scene_update_objs_async
+    renders the updated objects: its scene table follows the update."""
+    scs, gs, views, frames, owners = _make(gpu_ctx, seed=5, n=6)
+    b = runtime.Batch(gpu_ctx, 16)
+    b.set_views([gs[o] for o in owners], views, frames)
+    b.render_checked(graph=True)
+    for step, graph in ((1, True), (2, False), (3, True)):
+        for si, (s, g) in enumerate(zip(scs, gs)):
+            objs = s.objs.copy()
+            objs["world_pos"][:, 0] += 7.0 * step * (si + 1)
+            g.update_objs_async(0, objs)
+            s.objs = objs
+        b.render_checked(graph=graph)
+        for i, (v, f, o) in enumerate(zip(views, frames, owners)):
+            got = f.read()
+            ref = oracle.render(scs[o], v)
+            assert np.array_equal(got["depth"], ref["depth"]), (step, i)
+            assert np.array_equal(got["tri_id"], ref["tri_id"]), (step, i)
+    b.close()

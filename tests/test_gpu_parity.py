@@ -345,3 +345,36 @@ def test_more_than_2_31_samples_is_refused_then_recovers(gpu_ctx, oracle):
     sc2 = _screen_filling_scene()
     out = fr.render_read(runtime.GpuScene(gpu_ctx, sc2), view)
     assert_frame_parity(out, oracle.render(sc2, view, nthreads=16), "after refusal")
+
+
+def test_partial_updates_between_pipelined_renders(gpu_ctx, oracle):
+    """Six partial descriptor updates (one or two objects each, moved and
+    turned) interleaved with renders into six frames and, back to back, into
+    one more frame, with no host wait: every frame shows the state of all
+    updates issued before it (an update builds on the previous one while
+    earlier renders may still read the state before it)."""
+    sc, view = scenes.random_scene(4321, n_tris=500, width=160, height=120)
+    g = runtime.GpuScene(gpu_ctx, sc)
+    frames = [runtime.Frame(gpu_ctx, 160, 120) for _ in range(6)]
+    same = runtime.Frame(gpu_ctx, 160, 120)
+    objs = sc.objs.copy()
+    states = []
+    rng = np.random.default_rng(9)
+    for k, fr in enumerate(frames):
+        o = k % len(objs)
+        objs["world_pos"][o, :3] += rng.uniform(-30.0, 30.0, 3).astype(np.float32)
+        q = objs["world_rot_quat"][o] + rng.normal(scale=0.2, size=4).astype(np.float32)
+        objs["world_rot_quat"][o] = q / np.linalg.norm(q)
+        n = 2 if k % 2 else 1
+        g.update_objs_async(o, objs[o:o + n] if o + n <= len(objs) else objs[o:])
+        fr.render(g, view)
+        same.render(g, view)
+        states.append(objs.copy())
+    outs = [fr.read() for fr in frames]
+    last = same.read()
+    for k in range(6):
+        sck = scenes.Scene(sc.tris, states[k], sc.lights, sc.textures, sc.light_buffer_dim)
+        assert_frame_parity(outs[k], oracle.render(sck, view), f"update {k}")
+    for key in ("depth", "tri_id", "rgba", "normals"):
+        assert np.array_equal(last[key], outs[5][key]), key
+    assert any(not np.array_equal(outs[k]["depth"], outs[k + 1]["depth"]) for k in range(5))
