@@ -333,9 +333,11 @@ int ocr_scene_update_objs_async(ocr_scene* s, uint32_t first, uint32_t n, const 
   // Work enqueued on `stream` before this call that writes what the update
   // reads (nothing of the scene's own) is not waited for: the update reads
   // only the current version and the staged records.  Version nv starts as
-  // a copy of the current one, then takes the new records.
-  OCR_HIP_CHECK(hipMemcpyAsync(s->objs_v[nv], s->objs_v[s->ver], sizeof(ObjDesc) * s->nobj,
-                               hipMemcpyDeviceToDevice, us));
+  // a copy of the current one (unless every record is replaced), then takes
+  // the new records.
+  if (first > 0 || n < s->nobj)
+    OCR_HIP_CHECK(hipMemcpyAsync(s->objs_v[nv], s->objs_v[s->ver], sizeof(ObjDesc) * s->nobj,
+                                 hipMemcpyDeviceToDevice, us));
   OCR_HIP_CHECK(hipMemcpyAsync(s->objs_v[nv] + first, s->upd_host[k], sizeof(ObjDesc) * n, hipMemcpyHostToDevice, us));
   OCR_HIP_CHECK(hipEventRecord(s->upd_done[k], us));
   s->upd_used[k] = true;
