@@ -144,3 +144,39 @@ def test_scene_added_while_serving(gpu_ctx):
         assert np.array_equal(got[rid][1]["rgba"], alone1["rgba"])
         srv.release(rid)
     srv.close()
+
+
+def test_updates_while_serving(gpu_ctx):
+    """The application thread updates a served scene's objects (stream
+    ordered, alternating between two states) while the server's dispatcher
+    renders requests for it: every result is the scene in one state or the
+    other, never a mixture (a batch reads one geometry version)."""
+    sc, view = scenes.random_scene(330, n_tris=300, width=96, height=72)
+    objs_a = sc.objs.copy()
+    objs_b = sc.objs.copy()
+    objs_b["world_pos"][:, 0] += np.array([35.0, -25.0, 15.0, -45.0], np.float32)[: len(objs_b)]
+    objs_b["world_pos"][:, 1] -= 20.0
+    refs = []
+    for objs in (objs_a, objs_b):
+        s2 = scenes.Scene(sc.tris, objs, sc.lights, sc.textures, sc.light_buffer_dim)
+        refs.append(runtime.Frame(gpu_ctx, 96, 72).render_read(runtime.GpuScene(gpu_ctx, s2), view))
+    assert not np.array_equal(refs[0]["depth"], refs[1]["depth"])
+    g = runtime.GpuScene(gpu_ctx, sc)
+    srv = runtime.Server(gpu_ctx, max_batch_views=4, max_inflight=2, max_batch_tris=1 << 20,
+                         hbm_fraction=0.05, read_mask=runtime.READ_RGBA | runtime.READ_DEPTH)
+    slot = srv.add_scene(g)
+    ids = []
+    for i in range(48):
+        ids.append(srv.submit(slot, view))
+        g.update_objs_async(0, objs_b if i % 2 == 0 else objs_a)
+    assert all(r is not None for r in ids)
+    got = _wait_all(srv, ids)
+    seen = [0, 0]
+    for rid in ids:
+        o = got[rid][1]
+        match = [np.array_equal(o["depth"], r["depth"]) and np.array_equal(o["rgba"], r["rgba"]) for r in refs]
+        assert any(match), rid
+        seen[match.index(True)] += 1
+        srv.release(rid)
+    srv.close()
+    print("requests per state", seen)
