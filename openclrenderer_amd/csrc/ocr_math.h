@@ -83,6 +83,16 @@ constexpr float kInv9 = 1.0f / 9.0f;       // RN(1/9), folded by the compiler
 constexpr float kInv125 = 1.0f / 125.0f;
 constexpr float kInvSsaoDiv = 1.0f / kSsaoDiv;
 constexpr float kInv255 = 1.0f / 255.0f;
+// b / 255 correctly rounded for every byte b in two operations: b*hi + b*lo
+// with hi + lo the two-term split of 1/255 (hi = RN(1/255), lo = RN(1/255 -
+// hi)), the sum formed by one fma; exact for all 256 values (checked with
+// rational arithmetic).
+constexpr float kInv255Hi = 0x1.010102p-8f;
+constexpr float kInv255Lo = -0x1.fdfdfep-33f;
+__device__ __forceinline__ float unorm8(uint32_t b) {
+  const float f = (float)b;
+  return fmaf(f, kInv255Hi, f * kInv255Lo);
+}
 constexpr float kInvDepthFar = 1.0f / kDepthFar;
 constexpr float kInv65535 = 1.0f / 65535.0f;
 // A uniform divisor d takes the fast path when it and 1/d are comfortably

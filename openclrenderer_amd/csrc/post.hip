@@ -19,9 +19,9 @@ __device__ __forceinline__ uint32_t blend_pixel(uint32_t d1, uint32_t d2, uint32
   float sc[4], dc[4];
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    // c / 255 correctly rounded (div_rcp is exact for these integers)
-    sc[c] = div_rcp((float)((s >> (8 * c)) & 0xFFu), 255.f, kInv255);
-    dc[c] = div_rcp((float)((d >> (8 * c)) & 0xFFu), 255.f, kInv255);
+    // c / 255 correctly rounded
+    sc[c] = unorm8((s >> (8 * c)) & 0xFFu);
+    dc[c] = unorm8((d >> (8 * c)) & 0xFFu);
   }
   const float inv = 1.f - sc[3];
   uint32_t out = quant8(sc[3]) << 24;
@@ -74,8 +74,7 @@ __device__ __forceinline__ f3 decode_normal_u16(uint32_t packed) {
 }
 
 __device__ __forceinline__ f3 unorm_rgb(uint32_t c) {
-  return mk3(div_rcp((float)(c & 0xFFu), 255.f, kInv255), div_rcp((float)((c >> 8) & 0xFFu), 255.f, kInv255),
-             div_rcp((float)((c >> 16) & 0xFFu), 255.f, kInv255));
+  return mk3(unorm8(c & 0xFFu), unorm8((c >> 8) & 0xFFu), unorm8((c >> 16) & 0xFFu));
 }
 
 // do_pseudo_aa, cl2.cl:6438-6657: one work item per interior pixel.  The
@@ -173,8 +172,7 @@ struct MotionArgs {
 };
 
 __device__ __forceinline__ f4 unorm4(uint32_t c) {
-  return f4{div_rcp((float)(c & 0xFFu), 255.f, kInv255), div_rcp((float)((c >> 8) & 0xFFu), 255.f, kInv255),
-            div_rcp((float)((c >> 16) & 0xFFu), 255.f, kInv255), div_rcp((float)(c >> 24), 255.f, kInv255)};
+  return f4{unorm8(c & 0xFFu), unorm8((c >> 8) & 0xFFu), unorm8((c >> 16) & 0xFFu), unorm8(c >> 24)};
 }
 
 // read_imagef with CLK_FILTER_LINEAR, unnormalised coordinates (u, v)

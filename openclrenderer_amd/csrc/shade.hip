@@ -285,8 +285,7 @@ __device__ __forceinline__ float shadow_taps(const ShadeArgs& a, const ShadowSet
 }
 
 __device__ __forceinline__ f4 vcol(uint32_t c) {
-  return f4{div_rcp((float)(c >> 24), 255.f, kInv255), div_rcp((float)((c >> 16) & 0xFF), 255.f, kInv255),
-            div_rcp((float)((c >> 8) & 0xFF), 255.f, kInv255), div_rcp((float)(c & 0xFF), 255.f, kInv255)};
+  return f4{unorm8(c >> 24), unorm8((c >> 16) & 0xFF), unorm8((c >> 8) & 0xFF), unorm8(c & 0xFF)};
 }
 
 // The view and scene records reach the kernel through a reference that may

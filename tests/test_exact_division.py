@@ -10,6 +10,7 @@ that both changes leave every result bit-identical:
 * divisors known only at run time (fov, light radius, the 1/z plane value):
   Markstein's correction theorem for normal operands, exercised on random
   pairs over exponents -60..60;
+* bytes over 255 in two operations (unorm8), for all 256 bytes;
 * tan(a) >= 1 for every binary32 a from float(pi/4) rounded up to
   rational_acos(0.05), so the bias clamp gives SHADOWBIAS there.
 """
@@ -49,3 +50,26 @@ def test_tan_saturates_from_quarter_pi(checker, oracle):
     from oracle.binding import LIB_PATH
     r = subprocess.run([checker, "tan", LIB_PATH], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout
+
+
+def test_byte_over_255_two_term_form():
+    """unorm8 (the device's b / 255 for a byte b): fma(b, hi, RN(b * lo))
+    with hi = RN(1/255) and lo = RN(1/255 - hi) equals RN(b / 255) for all
+    256 bytes, in exact rational arithmetic (each step rounded once)."""
+    from fractions import Fraction
+
+    import numpy as np
+
+    def rn32(x):
+        f = np.float32(float(x))
+        cands = [np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))]
+        # nearest, ties to the even significand
+        return min(cands, key=lambda v: (abs(Fraction(float(v)) - x), int(v.view(np.uint32)) & 1))
+
+    hi = rn32(Fraction(1, 255))
+    lo = rn32(Fraction(1, 255) - Fraction(float(hi)))
+    assert float(hi) == float.fromhex("0x1.010102p-8") and float(lo) == float.fromhex("-0x1.fdfdfep-33")
+    for b in range(256):
+        p = rn32(Fraction(b) * Fraction(float(lo)))
+        q = rn32(Fraction(b) * Fraction(float(hi)) + Fraction(float(p)))
+        assert q == rn32(Fraction(b, 255)), b
