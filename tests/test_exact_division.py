@@ -8,8 +8,9 @@ that both changes leave every result bit-identical:
   a >= 2^-100, exhaustively (the values divided are integers, depths of at
   least 20 or sums of weights in [0, 9]);
 * divisors known only at run time (fov, light radius, the 1/z plane value):
-  Markstein's correction theorem for normal operands, exercised on random
-  pairs over exponents -60..60;
+  Markstein's correction step, exercised on random pairs over exponents
+  -60..60 and exhaustively over every dividend for sixteen divisor
+  significands (extremes, badly rounded reciprocals, seeded random ones);
 * bytes over 255 in two operations (unorm8), for all 256 bytes;
 * tan(a) >= 1 for every binary32 a from float(pi/4) rounded up to
   rational_acos(0.05), so the bias clamp gives SHADOWBIAS there.
@@ -44,6 +45,25 @@ def test_constant_divisors_exhaustive(checker):
 def test_runtime_divisors_random_pairs(checker):
     r = subprocess.run([checker, "random", "200000000"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0 and " 0 mismatches" in r.stdout, r.stdout
+
+
+# Divisor significands checked exhaustively over every dividend: the
+# largest and smallest significands, ones whose reciprocal rounds far from
+# 1/b, and seeded random ones.  A divisor's exponent only scales the
+# quotient, so each covers its whole significand class.  (Offline, 56 more
+# random significands from random.seed(2026) gave 0 mismatches as well.)
+RUNTIME_SIGNIFICANDS = ("0x1.fffffep+0", "0x1.000002p+0", "0x1.555556p+0", "0x1.800000p+0", "0x1.fffffcp+0",
+                        "0x1.c71c72p+0", "0x1.99999ap+0", "0x1.6a09e6p+0", "0x1.2f6a3cp+0", "0x1.b3e0d2p+0",
+                        "0x1.07c1f0p+0", "0x1.e7d5a8p+0", "0x1.4c2b96p+0", "0x1.91d3e4p+0", "0x1.3a0f5ep+0",
+                        "0x1.d8e47ap+0")
+
+
+def test_runtime_divisor_significands_exhaustive(checker):
+    procs = [subprocess.Popen([checker, "const", b], stdout=subprocess.PIPE, text=True)
+             for b in RUNTIME_SIGNIFICANDS]
+    for p in procs:
+        out = p.communicate(timeout=900)[0]
+        assert p.returncode == 0 and " 0 mismatches" in out, out
 
 
 def test_tan_saturates_from_quarter_pi(checker, oracle):
