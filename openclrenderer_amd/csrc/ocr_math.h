@@ -81,6 +81,7 @@ __device__ __forceinline__ float rcp_cr(float b) {
 }
 constexpr float kInv9 = 1.0f / 9.0f;       // RN(1/9), folded by the compiler
 constexpr float kInv125 = 1.0f / 125.0f;
+constexpr float kInv100 = 1.0f / 100.0f;
 constexpr float kInvSsaoDiv = 1.0f / kSsaoDiv;
 constexpr float kInv255 = 1.0f / 255.0f;
 // b / 255 correctly rounded for every byte b in two operations: b*hi + b*lo

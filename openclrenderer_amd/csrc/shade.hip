@@ -489,7 +489,10 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
   if (!is_front && two_sided) normal = -normal;
   const float occ_ssao = ssao(a, x, y);
   normal = fast_normalize(normal);
-  const f3 lighting_normal = fast_normalize(normal + rseed / 100.f);
+  // rseed / 100: each component is 0 or of magnitude >= 2^-25, where the
+  // short division by 100 is exact
+  const f3 lighting_normal = fast_normalize(
+      normal + mk3(div_rcp(rseed.x, 100.f, kInv100), div_rcp(rseed.y, 100.f, kInv100), div_rcp(rseed.z, 100.f, kInv100)));
   const float ambient = linear ? gamma_fwd(kAmbient) : kAmbient;
 
   int shnum = 0, static_num = 0;

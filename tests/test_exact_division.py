@@ -3,7 +3,7 @@ three-instruction form (div_rcp: q = a*y, r = fma(-q, b, a), fma(r, y, q)
 with y = RN(1/b)) and skips tan where the bias saturates.  These checks fix
 that both changes leave every result bit-identical:
 
-* constant divisors 9, 125, 2.5, 255 (shading) and 350000, 300, 500
+* constant divisors 9, 125, 2.5, 255, 100 (shading) and 350000, 300, 500
   (prearrange: z / depth_far, area / op_size): a = 0 and every finite
   a >= 2^-100, exhaustively (the values divided are integers, depths of at
   least 20 or sums of weights in [0, 9]);
@@ -35,7 +35,7 @@ def checker(tmp_path_factory):
 
 def test_constant_divisors_exhaustive(checker):
     procs = [subprocess.Popen([checker, "const", b], stdout=subprocess.PIPE, text=True)
-             for b in ("9", "125", "2.5", "255", "350000", "300", "500", "65535")]
+             for b in ("9", "125", "2.5", "255", "350000", "300", "500", "65535", "100")]
     outs = [(p.communicate(timeout=600)[0], p.returncode) for p in procs]
     for out, rc in outs:
         sys.stdout.write(out)
