@@ -19,7 +19,7 @@ Other workloads (one JSON line each):
               resident scenes: throughput and p50/p99 latency, batched vs one
               request at a time
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C] [--no-cpu] [--plain-only]
 """
 import argparse
 import ctypes as C
@@ -316,8 +316,9 @@ def run_plain(args):
     fr.set_profiling(False)
     # the first update of a scene allocates its second geometry version and
     # the first waits across streams create their events: a longer warm-up
-    animated = run_animated(g, fr, sc, view, max(args.steps, 200), max(args.warmup, 20)) if args.config == 2 else None
-    post = run_blend(ctx, g, fr, sc, view, W, H) if args.config == 2 else None
+    extras = args.config == 2 and not args.plain_only
+    animated = run_animated(g, fr, sc, view, max(args.steps, 200), max(args.warmup, 20)) if extras else None
+    post = run_blend(ctx, g, fr, sc, view, W, H) if extras else None
 
     if args.calib:
         cal = C.CDLL(os.path.join(ROOT, "openclrenderer_amd", "libocr_calib.so"))
@@ -521,6 +522,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calib", action="store_true", help="also run the copy kernel used to scale PMC counters")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="config 2: skip the animated and post-pass legs (profiling runs)")
     ap.add_argument("--requests", type=int, default=2000, help="served: requests per arrival rate")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="served: arrival rates (req/s)")
     ap.add_argument("--modes", default="8x4,16x3,1x1",
