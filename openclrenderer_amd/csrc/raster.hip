@@ -38,15 +38,23 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
-// Prearrange blocks: 1024 work items, one triangle each.  Cut records and
-// sample ranges are allocated once per block (one 64-bit atomicAdd on the
-// packed counter (cuts << 32 | samples) per 1024 triangles): every wavefront
-// of the render appending to one counter would serialise on that address.
-constexpr int kPreThreads = 1024;
-constexpr int kPreWaves = kPreThreads / kWave;
+// Prearrange blocks: one triangle per work item.  Cut records and sample
+// ranges are allocated once per block (one 64-bit atomicAdd on the packed
+// counter (cuts << 32 | samples) per block): every wavefront of the render
+// appending to one counter would serialise on that address (~12 ns per
+// same-address atomic).  The camera pass uses 512 work items: the next
+// render's prearrange runs beside the current render's shading, and a
+// 512-item block (8 wavefronts) fits on a CU beside shading wavefronts where
+// a 1024-item one waits for the CU to drain (measured: 4083-4094 vs
+// 4019-4024 frames/s on the 1080p, 1 M-triangle frame; 256 items pays for
+// twice the atomics, 57 vs 40 us alone).  The shadow pass runs alone and keeps
+// 1024 (one atomic per 1024 triangles and face round).
+constexpr int kPreThreads = 512;
+constexpr int kShadowPreThreads = 1024;
 
+template <int kWaves>
 struct BlockAllocLds {
-  uint32_t cuts[kPreWaves], smps[kPreWaves], frags[kPreWaves];
+  uint32_t cuts[kWaves], smps[kWaves], frags[kWaves];
   unsigned long long base;
   uint32_t ok;
 };
@@ -56,7 +64,8 @@ struct BlockAllocLds {
 // sample total is checked before its atomic, and each atomic adds less than
 // kMaxSamples, so the low half reaches the threshold and sets the flag before
 // it could carry into the record count.
-__device__ __forceinline__ bool block_alloc(const Lists& L, BlockAllocLds& sh, uint32_t ncut, uint32_t nsmp,
+template <int kWaves>
+__device__ __forceinline__ bool block_alloc(const Lists& L, BlockAllocLds<kWaves>& sh, uint32_t ncut, uint32_t nsmp,
                                             uint32_t nfrag, uint32_t& cut0, uint32_t& smp0) {
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const uint32_t ic = wave_incl_scan(ncut), is = wave_incl_scan(nsmp), ifr = wave_incl_scan(nfrag);
@@ -69,7 +78,7 @@ __device__ __forceinline__ bool block_alloc(const Lists& L, BlockAllocLds& sh, u
   if (threadIdx.x == 0) {
     uint32_t tc = 0, tf = 0;
     unsigned long long ts = 0;
-    for (int w = 0; w < kPreWaves; w++) {
+    for (int w = 0; w < kWaves; w++) {
       const uint32_t c = sh.cuts[w], sm = sh.smps[w];
       sh.cuts[w] = tc;
       sh.smps[w] = (uint32_t)ts;
@@ -282,8 +291,8 @@ __device__ __forceinline__ int face_clip_project(const f3 w[3], f3 cpos, const S
 // pass the list capacity or the sample limit writes nothing and sets the
 // overflow word, which every later pass of the render checks before reading
 // the list.
-template <bool kCamera>
-__device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds& sh, const f3 tp[2][3],
+template <bool kCamera, int kWaves>
+__device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds<kWaves>& sh, const f3 tp[2][3],
                                             const float rc[2], RasterTri rt[2], const uint32_t nf[2],
                                             const uint32_t ns[2], uint32_t key0, uint32_t key_step, uint32_t oid,
                                             uint32_t view) {
@@ -378,7 +387,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
 
 // ------------------------------------------------------------- prearrange
 __global__ void __launch_bounds__(kPreThreads) k_prearrange(PassArgs a) {
-  __shared__ BlockAllocLds sh;
+  __shared__ BlockAllocLds<kPreThreads / kWave> sh;
   // per-view triangle ranges are 256-aligned: each wavefront has one view
   const uint32_t slot = blockIdx.x * (uint32_t)kPreThreads + (threadIdx.x & ~(uint32_t)(kWave - 1));
   const uint32_t vi = view_of<Range::Tri>(a, slot);
@@ -637,17 +646,17 @@ struct FaceTable { SinCos f[6]; };
 // record allocation per round.  (Looping over the six faces per triangle
 // would leave most lanes idle: a triangle touches one to three faces.)
 struct ShadowLds {
-  float wpos[kPreThreads][9];
-  uint32_t oid[kPreThreads];
-  uint32_t items[kPreThreads * 3];   // local triangle << 4 | two_sided << 3 | face
+  float wpos[kShadowPreThreads][9];
+  uint32_t oid[kShadowPreThreads];
+  uint32_t items[kShadowPreThreads * 3];   // local triangle << 4 | two_sided << 3 | face
   uint32_t n_items;
 };
 
-__global__ void __launch_bounds__(kPreThreads) k_prearrange_shadow(const TriPos* __restrict__ world, uint32_t ntri,
+__global__ void __launch_bounds__(kShadowPreThreads) k_prearrange_shadow(const TriPos* __restrict__ world, uint32_t ntri,
                                                            const ObjDesc* __restrict__ objs, uint32_t nobj,
                                                            float lx, float ly, float lz, uint32_t L,
                                                            int only_static, FaceTable ft, Lists lists) {
-  __shared__ BlockAllocLds sh;
+  __shared__ BlockAllocLds<kShadowPreThreads / kWave> sh;
   __shared__ ShadowLds S;
   const uint32_t t = threadIdx.x;
   const uint32_t id = blockIdx.x * blockDim.x + t;
@@ -691,7 +700,7 @@ __global__ void __launch_bounds__(kPreThreads) k_prearrange_shadow(const TriPos*
   __syncthreads();
   const uint32_t n_items = S.n_items;
   // ---- phase 2: same rounds for every work item of the block
-  for (uint32_t r0 = 0; r0 < n_items; r0 += kPreThreads) {
+  for (uint32_t r0 = 0; r0 < n_items; r0 += kShadowPreThreads) {
     f3 tp[2][3];
     float rc[2] = {0.f, 0.f};
     RasterTri rt[2];
@@ -728,7 +737,8 @@ hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, 
   const Lists lists = make_lists(sc->sh_cuts, sc->sh_counters, sc->sh_cap_cut, false);
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, sc->sh_counters);
   if (sc->ntri)
-    hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + kPreThreads - 1) / kPreThreads), dim3(kPreThreads), 0, s, sc->world,
+    hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + kShadowPreThreads - 1) / kShadowPreThreads),
+                       dim3(kShadowPreThreads), 0, s, sc->world,
                        sc->ntri, sc->objs, sc->nobj, lpos[0], lpos[1], lpos[2], sc->lbuf_dim, only_static,
                        ft, lists);
   PassArgs a;
