@@ -28,9 +28,11 @@ struct ocr_batch {
   std::vector<DevView> host_views;
   DevView* d_views = nullptr;
   DevScene* d_scenes = nullptr;
-  // the geometry version of each scene that d_scenes points at, and the
-  // pinned staging of its refresh (reused once the last refresh copy ended)
-  std::vector<int> bound_ver;
+  // the generation of each scene that d_scenes was built from (a scene's
+  // generation moves when its geometry version, atlas or lights are
+  // replaced), and the pinned staging of its refresh (reused once the last
+  // refresh copy ended)
+  std::vector<uint64_t> bound_gen;
   DevScene* ds_host = nullptr;
   hipEvent_t ds_done = nullptr;
   bool ds_pending = false;
@@ -43,6 +45,7 @@ struct ocr_batch {
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint32_t graph_unshadowed = 0;     // the shading variant the graph was captured with
   bool profiling = false;
   std::vector<hipEvent_t> ev;        // kRing x 2 (begin, end of a batch)
   int n_profiled = 0;
@@ -67,6 +70,9 @@ static PassArgs batch_pass(const ocr_batch* b) {
   a.pix_slots = b->pix_slots;
   a.tiles = b->tiles;
   a.tile_w_log2 = b->tune.shade_tile_w_log2;
+  a.unshadowed = 1u;
+  for (const Scene* s : b->scenes)
+    if (s->n_shadow) a.unshadowed = 0u;
   a.lists = make_lists(b->cuts, b->counters, b->cap_cut);
   return a;
 }
@@ -139,13 +145,12 @@ static void lock_scenes(const std::vector<Scene*>& scenes, std::vector<std::uniq
   for (Scene* s : order) locks.emplace_back(s->ver_mu);
 }
 
-// With the scenes locked: points d_scenes at the scenes' current geometry
-// versions (a copy on `st` when a stream-ordered update made another one
-// current since the last launch) and makes `st` follow the scenes' updates
-// and shadow passes.
+// With the scenes locked: points d_scenes at the scenes' current buffers (a
+// copy on `st` when an update, atlas or light upload replaced one since the
+// last launch) and makes `st` follow the scenes' updates and shadow passes.
 static hipError_t bind_scenes(ocr_batch* b, hipStream_t st) {
   bool changed = false;
-  for (size_t i = 0; i < b->scenes.size(); i++) changed |= b->bound_ver[i] != b->scenes[i]->ver;
+  for (size_t i = 0; i < b->scenes.size(); i++) changed |= b->bound_gen[i] != b->scenes[i]->gen;
   if (changed) {
     if (b->ds_pending) {
       hipError_t e = hipEventSynchronize(b->ds_done);
@@ -153,7 +158,7 @@ static hipError_t bind_scenes(ocr_batch* b, hipStream_t st) {
     }
     for (size_t i = 0; i < b->scenes.size(); i++) {
       b->ds_host[i] = dev_scene(b->scenes[i]);
-      b->bound_ver[i] = b->scenes[i]->ver;
+      b->bound_gen[i] = b->scenes[i]->gen;
     }
     hipError_t e = hipMemcpyAsync(b->d_scenes, b->ds_host, sizeof(DevScene) * b->scenes.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipEventRecord(b->ds_done, st);
@@ -264,18 +269,18 @@ int ocr_batch_set_views(ocr_batch* b, ocr_scene* const* scenes, const ocr_view* 
     }
   }
   std::vector<DevScene> ds(uniq.size());
-  std::vector<int> vers(uniq.size());
+  std::vector<uint64_t> gens(uniq.size());
   {
     std::vector<std::unique_lock<std::mutex>> locks;
     lock_scenes(uniq, locks);
     for (size_t i = 0; i < uniq.size(); i++) {
       ds[i] = dev_scene(uniq[i]);
-      vers[i] = uniq[i]->ver;
+      gens[i] = uniq[i]->gen;
     }
   }
   drop_graph(b);
   b->scenes = uniq;
-  b->bound_ver = vers;
+  b->bound_gen = gens;
   if (b->ds_pending) {
     OCR_HIP_CHECK(hipEventSynchronize(b->ds_done));   // a refresh copy still reading the old table
     b->ds_pending = false;
@@ -339,6 +344,9 @@ int ocr_batch_launch_graph(ocr_batch* b, void* stream) {
   OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
   int rc = ensure_lists(b);
   if (rc) return rc;
+  // the graph holds one shading kernel: capture again when a light upload
+  // changed whether any scene casts shadows
+  if (b->exec && batch_pass(b).unshadowed != b->graph_unshadowed) drop_graph(b);
   if (!b->exec) {
     // Capture on the batch's own stream (the legacy stream cannot be
     // captured); the passes use fill kernels, never memset nodes.
@@ -351,6 +359,7 @@ int ocr_batch_launch_graph(ocr_batch* b, void* stream) {
       return fail(OCR_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
     }
     b->graph = g;
+    b->graph_unshadowed = batch_pass(b).unshadowed;
     OCR_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
   hipStream_t st = (hipStream_t)stream;

@@ -150,6 +150,7 @@ struct PassArgs {
   uint32_t nviews;
   uint32_t tri_slots, pix_slots, tiles;
   uint32_t tile_w_log2;    // shading tile = (1 << tile_w_log2) x (256 >> tile_w_log2)
+  uint32_t unshadowed;     // no scene of the pass has a shadow-casting light
   Lists lists;
 };
 
@@ -246,6 +247,10 @@ struct Scene {
   ObjDesc* objs_v[2] = {};
   TriPos* world_v[2] = {};
   int ver = 0;
+  // bumped whenever a buffer the kernels reach through DevScene is replaced
+  // (atlas or light upload, geometry version), so that scene tables kept in
+  // device memory (batches) are rebuilt before their next launch
+  uint64_t gen = 0;
   std::vector<Reader> readers[2];
   std::vector<hipEvent_t> ev_pool;
   hipStream_t upd_stream = nullptr;

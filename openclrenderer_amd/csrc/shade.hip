@@ -353,8 +353,10 @@ __device__ __forceinline__ const DevView& shade_view(const PassArgs& p, uint32_t
   return p.view0;
 }
 
-template <bool kCount>
-__global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
+// kShadows = false: no light of the pass casts shadows (every light.shadow
+// is 0), so the occlusion code is compiled out.
+template <bool kCount, bool kShadows>
+__device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
   uint32_t vi;
   const DevView& V = shade_view(sl.p, vi);
   const DevScene& S = sl.p.scenes ? sl.p.scenes[V.scene] : sl.p.scene0;
@@ -504,7 +506,7 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
     float occlusion = 1;
     // the static and the dynamic map of a light share the face, position
     // and bias; both use the normalised point-to-light vector shading uses
-    const bool stat = l.shadow && l.is_static;
+    const bool stat = kShadows && l.shadow && l.is_static;
     f3 ptl_n;
     ShadowSetup ss;
     if (stat) {
@@ -525,7 +527,7 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
     f3 light_col = mk3(l.col[0], l.col[1], l.col[2]);
     if (linear) light_col = mk3(ldg(a.light_lin, 4 * i), ldg(a.light_lin, 4 * i + 1), ldg(a.light_lin, 4 * i + 2));
     if (!stat) ptl_n = fast_normalize(ptl);
-    if (l.shadow && recv_dyn) {
+    if (kShadows && l.shadow && recv_dyn) {
       if (!stat) ss = shadow_setup(a, lpos, normal, ptl_n, global_position);
       float dyn = 1.f - shadow_taps<kCount>(a, ss, a.shadow_maps, a.shadow_touch, shnum);
       occlusion = fminf(occlusion, dyn);
@@ -571,6 +573,21 @@ __global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
   a.normals[pix] = encode_normal(normal);
 }
 
+template <bool kCount>
+__global__ void __launch_bounds__(256) k_shade(ShadeLaunch sl) {
+  shade_pixel<kCount, true>(sl);
+}
+
+// Passes whose scenes have no shadow-casting light: without the occlusion
+// code the kernel fits 96 VGPRs, five wavefronts per SIMD instead of the
+// three of the general kernel's 139, which hides more of its gathers and
+// leaves room for the next render's raster passes (1080p, 1 M triangles,
+// 4 lights: shading alone 124 -> 108 us).  With shadows, even four
+// wavefronts per SIMD spill inside the light loop and run slower.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_shade_unshadowed(ShadeLaunch sl) {
+  shade_pixel<false, false>(sl);
+}
+
 // The linear-space light colours (gamma_fwd of each light's colour, the same
 // for every pixel), once per light upload.
 __global__ void k_light_prep(const ocr_light* __restrict__ lights, uint32_t n, float* __restrict__ lin) {
@@ -594,7 +611,10 @@ hipError_t launch_shade(const PassArgs& p, const SinCos face[6], hipStream_t s) 
   sl.p = p;
   for (int i = 0; i < 6; i++) sl.face[i] = face[i];
   sl.tex_touch = sl.shadow_touch = sl.static_touch = nullptr;
-  hipLaunchKernelGGL(k_shade<false>, dim3(p.tiles > 0 ? p.tiles : 1), dim3(256), 0, s, sl);
+  if (p.unshadowed)
+    hipLaunchKernelGGL(k_shade_unshadowed, dim3(p.tiles > 0 ? p.tiles : 1), dim3(256), 0, s, sl);
+  else
+    hipLaunchKernelGGL(k_shade<false>, dim3(p.tiles > 0 ? p.tiles : 1), dim3(256), 0, s, sl);
   return hipGetLastError();
 }
 

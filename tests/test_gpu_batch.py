@@ -115,3 +115,34 @@ def test_batch_follows_stream_ordered_updates(gpu_ctx, oracle):
             assert np.array_equal(got["depth"], ref["depth"]), (step, i)
             assert np.array_equal(got["tri_id"], ref["tri_id"]), (step, i)
     b.close()
+
+
+def test_batch_after_light_and_atlas_uploads(gpu_ctx):
+    """Uploads that replace a scene's buffers after set_views — lights (one
+    now casting shadows, which also switches the shading kernel) and the
+    atlas — are followed by the next batch launch, graph and plain: every
+    view equals the view rendered alone after the same uploads."""
+    scs, gs, views, frames, owners = _make(gpu_ctx, seed=11, n=6)
+    b = runtime.Batch(gpu_ctx, 8)
+    b.set_views([gs[o] for o in owners], views, frames)
+    b.render_checked(graph=True)
+    lights = scs[0].lights.copy()
+    lights["shadow"][0] = 1
+    gs[0].upload_lights(lights, 256)
+    gs[0].shadow_pass()
+    gs[1].upload_textures([scenes.value_noise_texture(64, 99, lattice=4)])
+    for graph in (True, False, True):
+        b.render_checked(graph=graph)
+        for i, (v, f, o) in enumerate(zip(views, frames, owners)):
+            got = f.read()
+            alone = runtime.Frame(gpu_ctx, f.width, f.height).render_read(gs[o], v)
+            for k in ("depth", "tri_id", "rgba", "normals"):
+                assert np.array_equal(got[k], alone[k]), (graph, i, k)
+    # and back to no shadow-casting light
+    lights["shadow"][0] = 0
+    gs[0].upload_lights(lights, 256)
+    b.render_checked(graph=True)
+    for i, (v, f, o) in enumerate(zip(views, frames, owners)):
+        alone = runtime.Frame(gpu_ctx, f.width, f.height).render_read(gs[o], v)
+        assert np.array_equal(f.read()["rgba"], alone["rgba"]), i
+    b.close()
