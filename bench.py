@@ -302,7 +302,10 @@ def run_plain(args):
     x_tex, x_shadow = fr.count_texels(g, view)
     B = shade_algorithmic_bytes(W, H, V, O, x_tex, x_shadow, int(sc.lights.shape[0]))
     achieved = B / (st["ms_shade"] * 1e-3) / 1e9
-    prof = profile_entry("k_shade<false>", workload)
+    # the shading kernel the path launches: without shadow-casting lights the
+    # one with the occlusion code compiled out
+    shade_kernel = "k_shade<false>" if np.any(sc.lights["shadow"] != 0) else "k_shade_unshadowed"
+    prof = profile_entry(shade_kernel, workload)
     traffic = prof.get("bytes_per_launch")
     achieved_iso = B / (st_iso["ms_shade"] * 1e-3) / 1e9
     valu = None
@@ -341,7 +344,7 @@ def run_plain(args):
         "passes_ms_isolated": passes_iso,
         "work": {"cut_tris": st["n_cut"], "fragments": st["n_frag"], "visible_tris": V, "visible_objs": O,
                  "atlas_texels": x_tex, "shadow_texels": x_shadow},
-        "roofline": {"bound": "hbm", "kernel": "k_shade", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": shade_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": B, "valu_issue": valu,
                      "isolated": {"ms": st_iso["ms_shade"], "achieved": achieved_iso,
