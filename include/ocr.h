@@ -193,7 +193,13 @@ int  ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n,
                        uint32_t light_buffer_dim);
 /* engine::generate_realtime_shadowing (engine.cpp:1601-1790): clears and
  * renders every shadow-casting light's cube map from dynamic geometry, and
- * the static-light maps from static geometry when lights were re-uploaded. */
+ * the static-light maps from static geometry when lights were re-uploaded.
+ * Stream-ordered like ocr_scene_update_objs_async: renders, batches and work
+ * on `stream` issued afterwards see the new maps, renders issued before keep
+ * the maps they were issued with.  The scene holds up to two sets of dynamic
+ * maps (the second allocated when a pass follows a render): a pass renders
+ * the set no render issued since the previous pass reads, on a stream of the
+ * scene's own, so it overlaps the shading of the frame before it. */
 int  ocr_shadow_pass(ocr_scene* s, void* stream);
 void ocr_scene_destroy(ocr_scene* s);
 

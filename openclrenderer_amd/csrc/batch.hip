@@ -168,11 +168,11 @@ static hipError_t bind_scenes(ocr_batch* b, hipStream_t st) {
   return wait_scenes(b, st);
 }
 
-// With the scenes locked, after the launch: the scenes' current versions
-// are read by work on `st`.
+// With the scenes locked, after the launch: the scenes' current geometry
+// versions and shadow maps are read by work on `st`.
 static hipError_t note_scene_reads(ocr_batch* b, hipStream_t st) {
   for (Scene* s : b->scenes) {
-    hipError_t e = s->note_read(st);
+    hipError_t e = s->note_read(st, true);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

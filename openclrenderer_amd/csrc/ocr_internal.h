@@ -255,9 +255,17 @@ struct Scene {
   std::vector<hipEvent_t> ev_pool;
   hipStream_t upd_stream = nullptr;
   std::mutex ver_mu;
-  // Records that work enqueued on `st` so far reads the current version.
-  hipError_t note_read(hipStream_t st) {
-    for (Reader& r : readers[ver])
+  // Dynamic shadow-map versions, the same way: shadow_maps aliases
+  // maps_v[mver]; a shadow pass renders the other version on sh_stream once
+  // the passes that read it (map_readers) have finished, so it overlaps the
+  // shading of the render before it.  Static maps have one version (they
+  // are rendered only after a light upload).
+  uint32_t* maps_v[2] = {};
+  int mver = 0;
+  std::vector<Reader> map_readers[2];
+  hipStream_t sh_stream = nullptr;
+  hipError_t note_into(std::vector<Reader>& list, hipStream_t st) {
+    for (Reader& r : list)
       if (r.stream == st) return hipEventRecord(r.ev, st);
     hipEvent_t e = nullptr;
     if (!ev_pool.empty()) {
@@ -267,8 +275,25 @@ struct Scene {
       hipError_t c = hipEventCreateWithFlags(&e, hipEventDisableTiming);
       if (c != hipSuccess) return c;
     }
-    readers[ver].push_back(Reader{st, e});
+    list.push_back(Reader{st, e});
     return hipEventRecord(e, st);
+  }
+  // Records that work enqueued on `st` so far reads the current geometry
+  // version (and, for passes that shade, the current shadow maps).
+  hipError_t note_read(hipStream_t st, bool maps = false) {
+    hipError_t e = note_into(readers[ver], st);
+    if (e == hipSuccess && maps) e = note_into(map_readers[mver], st);
+    return e;
+  }
+  // Makes `st` wait for every pass recorded in `list`, then forgets them.
+  hipError_t wait_readers(std::vector<Reader>& list, hipStream_t st) {
+    for (const Reader& r : list) {
+      hipError_t e = hipStreamWaitEvent(st, r.ev, 0);
+      if (e != hipSuccess) return e;
+      ev_pool.push_back(r.ev);
+    }
+    list.clear();
+    return hipSuccess;
   }
   // what the render and batch paths wait on before they read the scene
   hipError_t wait_geometry(hipStream_t st) const {

@@ -313,11 +313,7 @@ int ocr_scene_update_objs_async(ocr_scene* s, uint32_t first, uint32_t n, const 
   if (!s->upd_stream) OCR_HIP_CHECK(hipStreamCreateWithFlags(&s->upd_stream, hipStreamNonBlocking));
   hipStream_t us = s->upd_stream;
   // the passes that read version nv when it was current end first
-  for (const Scene::Reader& r : s->readers[nv]) {
-    OCR_HIP_CHECK(hipStreamWaitEvent(us, r.ev, 0));
-    s->ev_pool.push_back(r.ev);
-  }
-  s->readers[nv].clear();
+  OCR_HIP_CHECK(s->wait_readers(s->readers[nv], us));
   const int k = s->upd_next;
   s->upd_next = (k + 1) % Scene::kUpdRing;
   if (!s->upd_host[k]) {
@@ -382,7 +378,9 @@ void ocr_scene_destroy(ocr_scene* s) {
   free_atlas(s->atlas);
   dev_free(s->lights);
   dev_free(s->light_lin);
-  dev_free(s->shadow_maps);
+  if (!s->maps_v[0] && !s->maps_v[1]) s->maps_v[0] = s->shadow_maps;
+  for (int v = 0; v < 2; v++) dev_free(s->maps_v[v]);
+  s->shadow_maps = nullptr;
   dev_free(s->static_maps);
   dev_free(s->sh_cuts);
   dev_free(s->sh_counters);
@@ -395,10 +393,13 @@ void ocr_scene_destroy(ocr_scene* s) {
     }
     if (s->upd_host[k]) (void)hipHostFree(s->upd_host[k]);
   }
-  for (int v = 0; v < 2; v++)
+  for (int v = 0; v < 2; v++) {
     for (const Scene::Reader& r : s->readers[v]) (void)hipEventDestroy(r.ev);
+    for (const Scene::Reader& r : s->map_readers[v]) (void)hipEventDestroy(r.ev);
+  }
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
   if (s->upd_stream) (void)hipStreamDestroy(s->upd_stream);
+  if (s->sh_stream) (void)hipStreamDestroy(s->sh_stream);
   delete s;
 }
 
@@ -531,7 +532,13 @@ int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_
   }
   dev_free(s->lights);
   dev_free(s->light_lin);
-  dev_free(s->shadow_maps);
+  for (int v = 0; v < 2; v++) {
+    dev_free(s->maps_v[v]);
+    for (const Scene::Reader& r : s->map_readers[v]) s->ev_pool.push_back(r.ev);   // all complete
+    s->map_readers[v].clear();
+  }
+  s->shadow_maps = nullptr;
+  s->mver = 0;
   dev_free(s->static_maps);
   for (uint32_t i = 0; i < n; i++)
     if (lights[i].shadow > 1)
@@ -547,9 +554,10 @@ int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_
   int rc;
   size_t per = (size_t)6 * light_buffer_dim * light_buffer_dim;
   if ((rc = dev_alloc(&s->lights, n, "lights")) || (rc = dev_alloc(&s->light_lin, (size_t)4 * n, "light colours")) ||
-      (rc = dev_alloc(&s->shadow_maps, per * std::max<uint32_t>(s->n_shadow, 1), "shadow maps")) ||
+      (rc = dev_alloc(&s->maps_v[0], per * std::max<uint32_t>(s->n_shadow, 1), "shadow maps")) ||
       (rc = dev_alloc(&s->static_maps, per * std::max<uint32_t>(s->n_static, 1), "static shadow maps")))
     return rc;
+  s->shadow_maps = s->maps_v[0];
   if (n) OCR_HIP_CHECK(hipMemcpy(s->lights, lights, sizeof(ocr_light) * n, hipMemcpyHostToDevice));
   if (n) OCR_HIP_CHECK(launch_light_prep(s->lights, n, s->light_lin, nullptr));
   OCR_HIP_CHECK(launch_fill_u32(s->shadow_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_shadow, 1), nullptr));
@@ -573,8 +581,23 @@ int ocr_shadow_pass(ocr_scene* s, void* stream) {
   if (!s) return fail(OCR_ERR_INVALID, "ocr_shadow_pass: null scene");
   if (s->n_shadow == 0 && !(s->static_dirty && s->n_static)) return OCR_OK;
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
-  hipStream_t st = (hipStream_t)stream;
   std::lock_guard<std::mutex> lk(s->ver_mu);   // reads the current geometry version
+  const size_t per = (size_t)6 * s->lbuf_dim * s->lbuf_dim;
+  // the map version this pass renders: the current one when no render has
+  // been issued against it (renders register under this lock), else the
+  // other one, on the scene's own stream once its readers have finished
+  const int nm = s->map_readers[s->mver].empty() ? s->mver : 1 - s->mver;
+  if (!s->maps_v[nm]) {
+    int rc = dev_alloc(&s->maps_v[nm], per * std::max<uint32_t>(s->n_shadow, 1), "shadow maps (second version)");
+    if (rc) return rc;
+  }
+  if (!s->sh_stream) OCR_HIP_CHECK(hipStreamCreateWithFlags(&s->sh_stream, hipStreamNonBlocking));
+  const hipStream_t caller = (hipStream_t)stream;
+  const hipStream_t st = s->sh_stream;
+  OCR_HIP_CHECK(s->wait_readers(s->map_readers[nm], st));
+  // static maps have one version: when they are rendered, every pass that
+  // may read them ends first
+  if (s->static_dirty && s->n_static) OCR_HIP_CHECK(s->wait_readers(s->map_readers[s->mver], st));
   OCR_HIP_CHECK(s->wait_geometry(st));   // descriptor updates enqueued on another stream
   {
     // A triangle reaches the faces its three vertices fall in, each as at
@@ -583,16 +606,16 @@ int ocr_shadow_pass(ocr_scene* s, void* stream) {
     int rc = ensure_shadow_lists(s, (uint32_t)cut);
     if (rc) return rc;
   }
-  const size_t per = (size_t)6 * s->lbuf_dim * s->lbuf_dim;
+  uint32_t* const maps = s->maps_v[nm];
   // engine.cpp:1613-1625: clear every dynamic map, then render each
   // shadow-casting light from the dynamic geometry; static lights' maps are
   // rendered from static geometry when the lights were (re)uploaded.
-  OCR_HIP_CHECK(launch_fill_u32(s->shadow_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_shadow, 1), st));
+  OCR_HIP_CHECK(launch_fill_u32(maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_shadow, 1), st));
   uint32_t nn = 0, kk = 0;
   for (uint32_t i = 0; i < s->nlights; i++) {
     const ocr_light& l = s->host_lights[i];
     if (l.shadow == 1) {
-      OCR_HIP_CHECK(launch_shadow_light(s, l.pos, 0, s->shadow_maps + per * nn, st));
+      OCR_HIP_CHECK(launch_shadow_light(s, l.pos, 0, maps + per * nn, st));
       nn++;
     }
     if (l.shadow && l.is_static && s->static_dirty) {
@@ -605,6 +628,11 @@ int ocr_shadow_pass(ocr_scene* s, void* stream) {
   OCR_HIP_CHECK(s->note_read(st));
   OCR_HIP_CHECK(hipEventRecord(s->written, st));
   s->written_valid = true;
+  s->mver = nm;
+  s->shadow_maps = maps;
+  s->gen++;   // batches' scene tables point at the new maps
+  // later work on the caller's stream sees the maps
+  OCR_HIP_CHECK(hipStreamWaitEvent(caller, s->written, 0));
   return OCR_OK;
 }
 
@@ -785,7 +813,7 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   // pass, wherever that was enqueued (the raster passes do not read them).
   OCR_HIP_CHECK(s->wait_shadow_maps(sb));
   OCR_HIP_CHECK(launch_shade(pa, s->ctx->face_sc, sb));
-  OCR_HIP_CHECK(s->note_read(sb));
+  OCR_HIP_CHECK(s->note_read(sb, true));
   if (prof) OCR_HIP_CHECK(hipEventRecord(ev[5], sb));
   OCR_HIP_CHECK(hipEventRecord(f->done[b], sb));
   OCR_HIP_CHECK(hipStreamWaitEvent(st, f->done[b], 0));
