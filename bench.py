@@ -79,8 +79,10 @@ class EventPair:
         return t.value
 
 
-# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per
-# 4 cycles per SIMD
+# VALU issue model: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per
+# 4 cycles per SIMD.  Some instruction mixes issue faster than this (the
+# unshadowed shading kernel measures ~1.05 of it), so it is a lower bound of
+# the ceiling and fractions above 1 are possible.
 VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 
 
