@@ -413,7 +413,7 @@ def run_batched(args):
                       "path": "batched (hipGraph, autotuned)"})
     line.update({"mpix_per_s": npix / t_batch / 1e6, "sequential_views_per_s": len(views) / t_seq,
                  "sequential_ms_per_step": t_seq * 1e3, "speedup_vs_sequential": t_seq / t_batch,
-                 "tuning": {"raster_blocks_per_cu": tuning[0], "shade_tile_w_log2": tuning[1]},
+                 "tuning": {"raster_blocks_per_cu": tuning[0], "shade_tile_w_log2": tuning[1], "groups": tuning[2]},
                  "batched_equals_sequential": bool(same)})
     print(json.dumps(line), flush=True)
 

@@ -304,7 +304,11 @@ typedef struct ocr_batch ocr_batch;
 typedef struct ocr_batch_tuning {
     uint32_t raster_blocks_per_cu;   /* persistent depth / id grid: blocks per CU (1..32) */
     uint32_t shade_tile_w_log2;      /* shading tile: 2^k wide, 256 >> k high (0..8) */
-    uint32_t _pad[2];
+    uint32_t groups;                 /* 0 or 1: one launch sequence for all views; 2..8:
+                                        the views split in that many pixel-balanced groups,
+                                        each on a stream of the batch's own, so one group's
+                                        shading overlaps another's raster passes */
+    uint32_t _pad;
 } ocr_batch_tuning;
 
 int  ocr_batch_create(ocr_ctx* ctx, uint32_t max_views, ocr_batch** out);
@@ -326,8 +330,8 @@ int  ocr_batch_launch_graph(ocr_batch* b, void* stream);
 /* Synchronises `stream`; counters of the last render (OCR_ERR_UNSUPPORTED if
  * the views needed more than 2^31 - 1 raster samples together). */
 int  ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out);
-/* Times raster grid sizes x shading tile shapes on the current views and
- * keeps the fastest (reps graph launches each). */
+/* Times raster grid sizes x shading tile shapes x groups (1, 2, 4) on the
+ * current views and keeps the fastest (reps graph launches each). */
 int  ocr_batch_autotune(ocr_batch* b, void* stream, uint32_t reps, ocr_batch_tuning* best);
 
 /* ---- served form ----------------------------------------------------------

@@ -41,7 +41,17 @@ struct ocr_batch {
   CutTri* cuts = nullptr;
   uint32_t* counters = nullptr;
   uint32_t cap_cut = 0;
-  ocr_batch_tuning tune{8, 4, {0, 0}};
+  ocr_batch_tuning tune{8, 4, 1, 0};
+  // tune.groups > 1: sub-batches over pixel-balanced groups of the views,
+  // each rendered on a stream of its own between a fork and a join event
+  std::vector<ocr_batch*> subs;
+  std::vector<hipStream_t> sub_streams;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
+  // the views as last set, to regroup them when the tuning changes
+  std::vector<ocr_scene*> set_scenes;
+  std::vector<ocr_view> set_views;
+  std::vector<ocr_frame*> set_frames;
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -188,6 +198,71 @@ static hipError_t enqueue_batch(ocr_batch* b, hipStream_t st) {
   return e;
 }
 
+// Groups in use: the tuning's, but never more than there are views.
+static uint32_t group_count(const ocr_batch* b) {
+  return std::min<uint32_t>(std::max<uint32_t>(b->tune.groups, 1), std::max<uint32_t>(b->nviews, 1));
+}
+
+// Splits the views set last into tune.groups contiguous, pixel-balanced
+// groups and sets each on a sub-batch (created on first use) with the
+// batch's raster and shading tuning.
+static int regroup(ocr_batch* b) {
+  const uint32_t g = group_count(b);
+  if (g <= 1 || b->set_views.empty()) return OCR_OK;
+  while (b->subs.size() < g) {
+    ocr_batch* sb = nullptr;
+    int rc = ocr_batch_create(static_cast<ocr_ctx*>(b->ctx), b->max_views, &sb);
+    if (rc) return rc;
+    b->subs.push_back(sb);
+    hipStream_t q = nullptr;
+    hipEvent_t e = nullptr;
+    OCR_HIP_CHECK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+    b->sub_streams.push_back(q);
+    OCR_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->joins.push_back(e);
+  }
+  if (!b->fork) OCR_HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  const uint32_t n = (uint32_t)b->set_views.size();
+  uint64_t total = 0;
+  for (const ocr_view& v : b->set_views) total += (uint64_t)v.width * v.height;
+  ocr_batch_tuning st = b->tune;
+  st.groups = 1;
+  uint32_t begin = 0;
+  uint64_t acc = 0;
+  for (uint32_t k = 0; k < g; k++) {
+    // group k ends where the running pixel count reaches (k+1)/g of the
+    // total, keeping at least one view for each later group
+    uint32_t end = begin + 1;
+    acc += (uint64_t)b->set_views[begin].width * b->set_views[begin].height;
+    while (end < n - (g - 1 - k) && acc * g < total * (k + 1)) {
+      acc += (uint64_t)b->set_views[end].width * b->set_views[end].height;
+      end++;
+    }
+    if (k == g - 1) end = n;
+    ocr_batch* sb = b->subs[k];
+    int rc = ocr_batch_set_tuning(sb, &st);
+    if (!rc) rc = ocr_batch_set_views(sb, &b->set_scenes[begin], &b->set_views[begin], &b->set_frames[begin], end - begin);
+    if (rc) return rc;
+    begin = end;
+  }
+  return OCR_OK;
+}
+
+// Renders the groups: each sub-batch on its own stream after `st`'s work so
+// far, `st` after all of them.
+static int render_groups(ocr_batch* b, hipStream_t st, bool graph) {
+  OCR_HIP_CHECK(hipEventRecord(b->fork, st));
+  for (uint32_t k = 0; k < group_count(b); k++) {
+    OCR_HIP_CHECK(hipStreamWaitEvent(b->sub_streams[k], b->fork, 0));
+    const int rc = graph ? ocr_batch_launch_graph(b->subs[k], b->sub_streams[k])
+                         : ocr_batch_render(b->subs[k], b->sub_streams[k]);
+    if (rc) return rc;
+    OCR_HIP_CHECK(hipEventRecord(b->joins[k], b->sub_streams[k]));
+    OCR_HIP_CHECK(hipStreamWaitEvent(st, b->joins[k], 0));
+  }
+  return OCR_OK;
+}
+
 extern "C" {
 
 int ocr_batch_create(ocr_ctx* ctx, uint32_t max_views, ocr_batch** out) {
@@ -220,6 +295,10 @@ void ocr_batch_destroy(ocr_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->ctx->device);
   (void)hipDeviceSynchronize();
+  for (ocr_batch* sb : b->subs) ocr_batch_destroy(sb);
+  for (hipStream_t q : b->sub_streams) (void)hipStreamDestroy(q);
+  for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   drop_graph(b);
   if (b->d_views) (void)hipFree(b->d_views);
   if (b->d_scenes) (void)hipFree(b->d_scenes);
@@ -290,18 +369,28 @@ int ocr_batch_set_views(ocr_batch* b, ocr_scene* const* scenes, const ocr_view* 
   b->sum_tris = sum_tris;
   b->sum_pix = sum_pix;
   OCR_HIP_CHECK(hipMemcpy(b->d_scenes, ds.data(), sizeof(DevScene) * ds.size(), hipMemcpyHostToDevice));
+  b->set_scenes.assign(scenes, scenes + n);
+  b->set_views.assign(views, views + n);
+  b->set_frames.assign(frames, frames + n);
   int rc = upload_views(b);
   if (rc) return rc;
+  if (group_count(b) > 1) return regroup(b);   // the groups own the work lists
   return ensure_lists(b);
 }
 
 int ocr_batch_set_tuning(ocr_batch* b, const ocr_batch_tuning* t) {
-  if (!b || !t || t->raster_blocks_per_cu == 0 || t->raster_blocks_per_cu > 32 || t->shade_tile_w_log2 > 8)
+  if (!b || !t || t->raster_blocks_per_cu == 0 || t->raster_blocks_per_cu > 32 || t->shade_tile_w_log2 > 8 ||
+      t->groups > 8)
     return fail(OCR_ERR_INVALID, "ocr_batch_set_tuning: bad tuning");
   b->tune = *t;
+  b->tune.groups = std::max<uint32_t>(t->groups, 1);
+  b->tune._pad = 0;
   drop_graph(b);
-  if (b->nviews) return upload_views(b);
-  return OCR_OK;
+  if (!b->nviews) return OCR_OK;
+  int rc = upload_views(b);
+  if (!rc && group_count(b) > 1) rc = regroup(b);
+  if (!rc && group_count(b) == 1) rc = ensure_lists(b);
+  return rc;
 }
 
 int ocr_batch_get_tuning(ocr_batch* b, ocr_batch_tuning* t) {
@@ -325,9 +414,17 @@ int ocr_batch_set_profiling(ocr_batch* b, int enable) {
 int ocr_batch_render(ocr_batch* b, void* stream) {
   if (!b || b->nviews == 0) return fail(OCR_ERR_INVALID, "ocr_batch_render: no views");
   OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  if (group_count(b) > 1) {
+    const bool prof = b->profiling && b->n_profiled < kBatchRing;
+    if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled], st));
+    const int rg = render_groups(b, st, false);
+    if (rg) return rg;
+    if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled++ + 1], st));
+    return OCR_OK;
+  }
   int rc = ensure_lists(b);
   if (rc) return rc;
-  hipStream_t st = (hipStream_t)stream;
   std::vector<std::unique_lock<std::mutex>> locks;
   lock_scenes(b->scenes, locks);
   OCR_HIP_CHECK(bind_scenes(b, st));
@@ -342,6 +439,15 @@ int ocr_batch_render(ocr_batch* b, void* stream) {
 int ocr_batch_launch_graph(ocr_batch* b, void* stream) {
   if (!b || b->nviews == 0) return fail(OCR_ERR_INVALID, "ocr_batch_launch_graph: no views");
   OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
+  if (group_count(b) > 1) {
+    hipStream_t gst = (hipStream_t)stream;
+    const bool prof = b->profiling && b->n_profiled < kBatchRing;
+    if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled], gst));
+    const int rg = render_groups(b, gst, true);
+    if (rg) return rg;
+    if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled++ + 1], gst));
+    return OCR_OK;
+  }
   int rc = ensure_lists(b);
   if (rc) return rc;
   // the graph holds one shading kernel: capture again when a light upload
@@ -379,7 +485,19 @@ int ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out) {
   OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
   OCR_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
   uint32_t c[kCounterWords];
-  OCR_HIP_CHECK(hipMemcpy(c, b->counters, sizeof(c), hipMemcpyDeviceToHost));
+  memset(c, 0, sizeof(c));
+  if (group_count(b) > 1) {
+    // the groups' counters, summed (their streams joined `stream`)
+    for (uint32_t k = 0; k < group_count(b); k++) {
+      uint32_t g[kCounterWords];
+      OCR_HIP_CHECK(hipMemcpy(g, b->subs[k]->counters, sizeof(g), hipMemcpyDeviceToHost));
+      c[kPacked + 1] += g[kPacked + 1];
+      c[kFragCount] += g[kFragCount];
+      c[kOverflow] |= g[kOverflow];
+    }
+  } else {
+    OCR_HIP_CHECK(hipMemcpy(c, b->counters, sizeof(c), hipMemcpyDeviceToHost));
+  }
   memset(out, 0, sizeof(*out));
   out->n_cut = c[kPacked + 1];
   out->n_frag = c[kFragCount];
@@ -412,18 +530,27 @@ int ocr_batch_autotune(ocr_batch* b, void* stream, uint32_t reps, ocr_batch_tuni
   OCR_HIP_CHECK(hipEventCreate(&e1));
   const uint32_t blocks[] = {2, 4, 8, 16};
   const uint32_t tiles[] = {3, 4, 5, 6};     // tile widths 8, 16, 32, 64
+  const uint32_t groups[] = {1, 2, 4};
   ocr_batch_tuning best = b->tune;
   float best_ms = 1e30f;
   const bool prof = b->profiling;
   b->profiling = false;
   int status = OCR_OK;
-  for (uint32_t bl : blocks)
+  for (uint32_t gr : groups) {
+    if (gr > b->nviews) break;
+    for (uint32_t bl : blocks)
     for (uint32_t tl : tiles) {
-      ocr_batch_tuning t{bl, tl, {0, 0}};
+      if (status) break;
+      ocr_batch_tuning t{bl, tl, gr, 0};
       if ((status = ocr_batch_set_tuning(b, &t))) break;
       if ((status = ocr_batch_launch_graph(b, stream))) break;
-      hipError_t e;
-      {
+      hipError_t e = hipSuccess;
+      if (gr > 1) {
+        e = hipEventRecord(e0, st);
+        for (uint32_t r = 0; r < reps && e == hipSuccess && status == OCR_OK; r++)
+          status = ocr_batch_launch_graph(b, stream);
+        if (status) break;
+      } else {
         // the timed replays read the scenes like any launch
         std::vector<std::unique_lock<std::mutex>> locks;
         lock_scenes(b->scenes, locks);
@@ -446,6 +573,8 @@ int ocr_batch_autotune(ocr_batch* b, void* stream, uint32_t reps, ocr_batch_tuni
         best = t;
       }
     }
+    if (status) break;
+  }
   b->profiling = prof;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -43,7 +43,7 @@ class TexSrc(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("raster_blocks_per_cu", C.c_uint32), ("shade_tile_w_log2", C.c_uint32),
-                ("_pad", C.c_uint32 * 2)]
+                ("groups", C.c_uint32), ("_pad", C.c_uint32)]
 
 
 class FrameStats(C.Structure):
@@ -348,14 +348,15 @@ class Batch:
         self.frames = frames
         check(self.lib.ocr_batch_set_views(self.h, C.cast(sc, C.c_void_p), _p(vv), C.cast(fr, C.c_void_p), n))
 
-    def set_tuning(self, raster_blocks_per_cu, shade_tile_w_log2):
-        t = BatchTuning(raster_blocks_per_cu, shade_tile_w_log2)
+    def set_tuning(self, raster_blocks_per_cu, shade_tile_w_log2, groups=1):
+        t = BatchTuning(raster_blocks_per_cu, shade_tile_w_log2, groups)
         check(self.lib.ocr_batch_set_tuning(self.h, C.byref(t)))
 
     def tuning(self):
+        """(raster blocks per CU, shading tile width log2, view groups)."""
         t = BatchTuning()
         check(self.lib.ocr_batch_get_tuning(self.h, C.byref(t)))
-        return t.raster_blocks_per_cu, t.shade_tile_w_log2
+        return t.raster_blocks_per_cu, t.shade_tile_w_log2, max(t.groups, 1)
 
     def set_profiling(self, on=True):
         check(self.lib.ocr_batch_set_profiling(self.h, 1 if on else 0))
@@ -374,7 +375,7 @@ class Batch:
     def autotune(self, reps=5, stream=None):
         t = BatchTuning()
         check(self.lib.ocr_batch_autotune(self.h, stream, reps, C.byref(t)))
-        return t.raster_blocks_per_cu, t.shade_tile_w_log2
+        return t.raster_blocks_per_cu, t.shade_tile_w_log2, max(t.groups, 1)
 
     def render_checked(self, stream=None, graph=False):
         """Render, and once more if the shared lists overflowed (they grow)."""

@@ -60,18 +60,21 @@ def test_batch_graph_replay_and_autotune(gpu_ctx):
             got = f.read()
             for k in a:
                 assert np.array_equal(got[k], a[k]), k
-    blocks, tile = b.autotune(reps=2)
-    assert 1 <= blocks <= 32 and 0 <= tile <= 8
+    blocks, tile, groups = b.autotune(reps=2)
+    assert 1 <= blocks <= 32 and 0 <= tile <= 8 and groups in (1, 2, 4)
     b.render_checked(graph=True)
     for f, a in zip(frames, first):
         got = f.read()
         for k in a:
             assert np.array_equal(got[k], a[k]), k
-    for bl, tl in ((2, 3), (16, 6), (4, 5)):
-        b.set_tuning(bl, tl)
-        b.render_checked()
-        for f, a in zip(frames, first):
-            assert np.array_equal(f.read()["rgba"], a["rgba"])
+    for bl, tl, gr in ((2, 3, 1), (16, 6, 2), (4, 5, 4), (8, 4, 3)):
+        b.set_tuning(bl, tl, gr)
+        for graph in (False, True):
+            b.render_checked(graph=graph)
+            for f, a in zip(frames, first):
+                got = f.read()
+                for k in a:
+                    assert np.array_equal(got[k], a[k]), (bl, tl, gr, graph, k)
     b.close()
 
 
