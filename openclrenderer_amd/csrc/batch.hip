@@ -209,6 +209,15 @@ static uint32_t group_count(const ocr_batch* b) {
 static int regroup(ocr_batch* b) {
   const uint32_t g = group_count(b);
   if (g <= 1 || b->set_views.empty()) return OCR_OK;
+  if (b->cuts) {
+    // the groups hold the work lists now; the batch's own are sized for
+    // every view and would double the memory
+    OCR_HIP_CHECK(hipDeviceSynchronize());
+    drop_graph(b);
+    OCR_HIP_CHECK(hipFree(b->cuts));
+    b->cuts = nullptr;
+    b->cap_cut = 0;
+  }
   while (b->subs.size() < g) {
     ocr_batch* sb = nullptr;
     int rc = ocr_batch_create(static_cast<ocr_ctx*>(b->ctx), b->max_views, &sb);
