@@ -219,15 +219,21 @@ static int regroup(ocr_batch* b) {
     b->cap_cut = 0;
   }
   while (b->subs.size() < g) {
-    ocr_batch* sb = nullptr;
-    int rc = ocr_batch_create(static_cast<ocr_ctx*>(b->ctx), b->max_views, &sb);
-    if (rc) return rc;
-    b->subs.push_back(sb);
+    // a group's sub-batch, stream and join event exist together or not at all
     hipStream_t q = nullptr;
     hipEvent_t e = nullptr;
-    OCR_HIP_CHECK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+    hipError_t he = hipStreamCreateWithFlags(&q, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    ocr_batch* sb = nullptr;
+    int rc = he == hipSuccess ? ocr_batch_create(static_cast<ocr_ctx*>(b->ctx), b->max_views, &sb)
+                              : fail(OCR_ERR_HIP, std::string("batch groups: ") + hipGetErrorString(he));
+    if (rc) {
+      if (e) (void)hipEventDestroy(e);
+      if (q) (void)hipStreamDestroy(q);
+      return rc;
+    }
+    b->subs.push_back(sb);
     b->sub_streams.push_back(q);
-    OCR_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     b->joins.push_back(e);
   }
   if (!b->fork) OCR_HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
