@@ -378,3 +378,19 @@ def test_partial_updates_between_pipelined_renders(gpu_ctx, oracle):
     for key in ("depth", "tri_id", "rgba", "normals"):
         assert np.array_equal(last[key], outs[5][key]), key
     assert any(not np.array_equal(outs[k]["depth"], outs[k + 1]["depth"]) for k in range(5))
+
+
+def test_unshadowed_kernel_equals_general_kernel(gpu_ctx):
+    """Without shadow-casting lights a render shades with the kernel whose
+    occlusion code is compiled out; the texel-counting pass re-shades the
+    same frame with the general kernel.  Both write identical bits."""
+    sc, view = scenes.random_scene(4242, n_tris=3000, width=320, height=240)
+    assert not np.any(sc.lights["shadow"])
+    g = runtime.GpuScene(gpu_ctx, sc)
+    fr = runtime.Frame(gpu_ctx, 320, 240)
+    first = fr.render_read(g, view)
+    fr.count_texels(g, view)
+    again = fr.read()
+    for k in ("depth", "tri_id", "rgba", "normals"):
+        assert np.array_equal(first[k], again[k]), k
+    assert (first["depth"] != UINT_MAX).any()
