@@ -1,7 +1,7 @@
 """ctypes binding of the oracle (oracle/_build/libocr_oracle.so).
 
 Test infrastructure only: used by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker.
+cpu_baseline leg of bench.py, as the result the HIP path is compared with.
 """
 import ctypes as C
 import math

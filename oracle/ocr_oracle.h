@@ -3,8 +3,8 @@
  * deferred-shading path (cl2.cl prearrange/kernel1/kernel2/kernel3, the cube
  * shadow passes and the texture atlas/mip kernels).
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared
- * against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * TEST INFRASTRUCTURE ONLY.  This is the reference result the HIP path is
+ * compared against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg may load it; the product library never links or calls it.
  *
  * Arithmetic contract (the reference builds with -cl-fast-relaxed-math and

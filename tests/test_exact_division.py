@@ -26,15 +26,15 @@ ROOT = os.path.dirname(HERE)
 
 
 @pytest.fixture(scope="module")
-def checker(tmp_path_factory):
+def div_check(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("div") / "exact_division_check")
     subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, os.path.join(HERE, "exact_division_check.c"),
                     "-lm", "-ldl"], check=True)
     return exe
 
 
-def test_constant_divisors_exhaustive(checker):
-    procs = [subprocess.Popen([checker, "const", b], stdout=subprocess.PIPE, text=True)
+def test_constant_divisors_exhaustive(div_check):
+    procs = [subprocess.Popen([div_check, "const", b], stdout=subprocess.PIPE, text=True)
              for b in ("9", "125", "2.5", "255", "350000", "300", "500", "65535", "100")]
     outs = [(p.communicate(timeout=600)[0], p.returncode) for p in procs]
     for out, rc in outs:
@@ -42,8 +42,8 @@ def test_constant_divisors_exhaustive(checker):
         assert rc == 0 and " 0 mismatches" in out, out
 
 
-def test_runtime_divisors_random_pairs(checker):
-    r = subprocess.run([checker, "random", "200000000"], capture_output=True, text=True, timeout=600)
+def test_runtime_divisors_random_pairs(div_check):
+    r = subprocess.run([div_check, "random", "200000000"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0 and " 0 mismatches" in r.stdout, r.stdout
 
 
@@ -58,17 +58,17 @@ RUNTIME_SIGNIFICANDS = ("0x1.fffffep+0", "0x1.000002p+0", "0x1.555556p+0", "0x1.
                         "0x1.d8e47ap+0")
 
 
-def test_runtime_divisor_significands_exhaustive(checker):
-    procs = [subprocess.Popen([checker, "const", b], stdout=subprocess.PIPE, text=True)
+def test_runtime_divisor_significands_exhaustive(div_check):
+    procs = [subprocess.Popen([div_check, "const", b], stdout=subprocess.PIPE, text=True)
              for b in RUNTIME_SIGNIFICANDS]
     for p in procs:
         out = p.communicate(timeout=900)[0]
         assert p.returncode == 0 and " 0 mismatches" in out, out
 
 
-def test_tan_saturates_from_quarter_pi(checker, oracle):
+def test_tan_saturates_from_quarter_pi(div_check, oracle):
     from oracle.binding import LIB_PATH
-    r = subprocess.run([checker, "tan", LIB_PATH], capture_output=True, text=True, timeout=600)
+    r = subprocess.run([div_check, "tan", LIB_PATH], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout
 
 
