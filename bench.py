@@ -98,12 +98,12 @@ def profile_entry(kernel, workload):
 
 
 def shade_algorithmic_bytes(W, H, V, O, x_tex, x_shadow, L):
-    """Minimal field bytes of one shading launch: 20 B per pixel (depth 4, id
-    4, next-depth clear 4, RGBA8 4, normal 4), 132 B per distinct visible
-    triangle (112 B of vertex fields + 20 B fragment record), 52 B per
-    distinct visible object, 4 B per distinct atlas / cube-map texel and 56 B
-    per light."""
-    return 20 * W * H + 132 * V + 52 * O + 4 * x_tex + 4 * x_shadow + 56 * L
+    """Minimal field bytes of one shading launch: 16 B per pixel (depth 4,
+    triangle id 4, RGBA8 4, normal 4; the next render's clear is k_clear's),
+    132 B per distinct visible triangle (112 B of vertex fields + 20 B
+    fragment record), 52 B per distinct visible object, 4 B per distinct
+    atlas / cube-map texel and 56 B per light."""
+    return 16 * W * H + 132 * V + 52 * O + 4 * x_tex + 4 * x_shadow + 56 * L
 
 
 def cpu_baseline(sc, view, gpu_out, seconds, with_shadows=False):
