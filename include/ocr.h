@@ -241,7 +241,8 @@ int  ocr_frame_set_profiling(ocr_frame* f, int enable);
  * `stream` enqueued before render N was issued: enqueue what reads a
  * render's outputs (ocr_frame_device_buffers changes with every render) on
  * `stream` before the next ocr_render on the frame, and use one stream for
- * the renders of one frame. */
+ * the renders of one frame.  A frame serves either ocr_render or batches:
+ * a frame already set on a batch is refused (OCR_ERR_INVALID). */
 int  ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream);
 
 /* Replaces the CL/GL blit and its host read-back path
@@ -315,7 +316,11 @@ int  ocr_batch_create(ocr_ctx* ctx, uint32_t max_views, ocr_batch** out);
 void ocr_batch_destroy(ocr_batch* b);
 /* View i renders scenes[i] into frames[i] (distinct frames, frame size =
  * view size).  Uploads the view and scene tables and sizes the shared work
- * lists; enqueues no rendering. */
+ * lists; enqueues no rendering.  Frames rendered by ocr_render are refused:
+ * a batch frame has no cut list or counters of its own, so the passes that
+ * need them (ocr_frame_motion_blur, ocr_frame_count_texels,
+ * ocr_frame_get_stats) refuse it, and ocr_batch_get_stats reports the
+ * batch's status. */
 int  ocr_batch_set_views(ocr_batch* b, ocr_scene* const* scenes, const ocr_view* views,
                          ocr_frame* const* frames, uint32_t n);
 int  ocr_batch_set_tuning(ocr_batch* b, const ocr_batch_tuning* t);
@@ -376,13 +381,19 @@ typedef struct ocr_server_info {
                                    views needed more raster samples than one
                                    render holds                            */
     uint32_t n_pending, n_frames;
-} ocr_server_info;              /* 56 B */
+    uint64_t frame_bytes;       /* HBM of the pooled frames, idle and in use
+                                   (kept within hbm_budget)                */
+    uint64_t n_evicted;         /* idle frames freed to make room for a
+                                   frame of another size                   */
+} ocr_server_info;              /* 72 B */
 
 int  ocr_server_create(ocr_ctx* ctx, const ocr_server_config* cfg, ocr_server** out);
 /* Registers a scene (atlas uploaded) and renders its shadow maps once. */
 int  ocr_server_add_scene(ocr_server* s, ocr_scene* scene, uint32_t* slot);
 /* OCR_ERR_BUSY when the request's frame (20 B per pixel) does not fit the
- * HBM budget beside the admitted, unreleased requests. */
+ * HBM budget beside the admitted, unreleased requests.  Frames are pooled per
+ * size; idle frames of other sizes are freed when a new frame would take the
+ * pool past the budget, so the frames' HBM stays within it. */
 int  ocr_server_submit(ocr_server* s, uint32_t scene_slot, const ocr_view* view,
                        uint64_t* request_id);
 /* Up to `max` completions; waits up to wait_ms for the first one. */

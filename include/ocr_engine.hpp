@@ -20,8 +20,9 @@
 // What does not carry over: OpenGL presentation (the frame stays offscreen and
 // blit_to_screen reads it back, engine::get_screen_rgba8), SFML windows and
 // input, runtime OpenCL program options (the kernels are compiled in; the
-// -D constants of main.cpp:79-85 are fixed), and file loaders (set_file takes
-// a load function; see objects_container::set_load_func).
+// -D constants of main.cpp:79-85 are fixed).  set_file loads OBJ/MTL files
+// with PNG textures through ocr_obj_load (include/ocr_obj.h) unless
+// objects_container::set_load_func names another loader.
 #ifndef OCR_ENGINE_HPP_INCLUDED
 #define OCR_ENGINE_HPP_INCLUDED
 

@@ -155,6 +155,20 @@ static void lock_scenes(const std::vector<Scene*>& scenes, std::vector<std::uniq
   for (Scene* s : order) locks.emplace_back(s->ver_mu);
 }
 
+// With the scenes locked: every scene still has what the kernels read
+// through the scene table (an atlas or light upload that failed after the
+// views were set leaves a scene without one of them).
+static int check_scenes(const ocr_batch* b) {
+  for (const Scene* s : b->scenes) {
+    if (!s->atlas.pages) return fail(OCR_ERR_INVALID, "batch: a scene lost its atlas (failed ocr_atlas_upload)");
+    if (s->nlights && !s->lights) return fail(OCR_ERR_INVALID, "batch: a scene's lights are missing");
+    if (s->n_shadow && !s->shadow_maps) return fail(OCR_ERR_INVALID, "batch: a scene's shadow maps are missing");
+    if (s->nobj && s->max_tid >= std::max<uint32_t>(s->atlas.mip_start, 1))
+      return fail(OCR_ERR_INVALID, "batch: an object's texture id is outside its scene's atlas");
+  }
+  return OCR_OK;
+}
+
 // With the scenes locked: points d_scenes at the scenes' current buffers (a
 // copy on `st` when an update, atlas or light upload replaced one since the
 // last launch) and makes `st` follow the scenes' updates and shadow passes.
@@ -349,6 +363,8 @@ int ocr_batch_set_views(ocr_batch* b, ocr_scene* const* scenes, const ocr_view* 
       return fail(OCR_ERR_INVALID, "ocr_batch_set_views: an object's texture id is outside the atlas");
     for (uint32_t j = 0; j < i; j++)
       if (frames[j] == frames[i]) return fail(OCR_ERR_INVALID, "ocr_batch_set_views: a frame appears twice");
+    if (f->use == Frame::kRender)
+      return fail(OCR_ERR_INVALID, "ocr_batch_set_views: a frame was rendered by ocr_render (frames serve one or the other)");
     uint32_t si = 0;
     while (si < uniq.size() && uniq[si] != s) si++;
     if (si == uniq.size()) uniq.push_back(s);
@@ -372,6 +388,7 @@ int ocr_batch_set_views(ocr_batch* b, ocr_scene* const* scenes, const ocr_view* 
       gens[i] = uniq[i]->gen;
     }
   }
+  for (uint32_t i = 0; i < n; i++) static_cast<Frame*>(frames[i])->use = Frame::kBatch;
   drop_graph(b);
   b->scenes = uniq;
   b->bound_gen = gens;
@@ -442,6 +459,7 @@ int ocr_batch_render(ocr_batch* b, void* stream) {
   if (rc) return rc;
   std::vector<std::unique_lock<std::mutex>> locks;
   lock_scenes(b->scenes, locks);
+  if ((rc = check_scenes(b))) return rc;
   OCR_HIP_CHECK(bind_scenes(b, st));
   const bool prof = b->profiling && b->n_profiled < kBatchRing;
   if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled], st));
@@ -486,6 +504,7 @@ int ocr_batch_launch_graph(ocr_batch* b, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   std::vector<std::unique_lock<std::mutex>> locks;
   lock_scenes(b->scenes, locks);
+  if ((rc = check_scenes(b))) return rc;
   OCR_HIP_CHECK(bind_scenes(b, st));   // the graph reads d_scenes when it runs
   const bool prof = b->profiling && b->n_profiled < kBatchRing;
   if (prof) OCR_HIP_CHECK(hipEventRecord(b->ev[2 * b->n_profiled], st));
@@ -569,6 +588,7 @@ int ocr_batch_autotune(ocr_batch* b, void* stream, uint32_t reps, ocr_batch_tuni
         // the timed replays read the scenes like any launch
         std::vector<std::unique_lock<std::mutex>> locks;
         lock_scenes(b->scenes, locks);
+        if ((status = check_scenes(b))) break;
         e = bind_scenes(b, st);
         if (e == hipSuccess) e = hipEventRecord(e0, st);
         for (uint32_t r = 0; r < reps && e == hipSuccess; r++)

@@ -338,6 +338,13 @@ struct Frame {
   std::vector<hipEvent_t> ev;             // kRing x kEvents, created on first use
   int n_profiled = 0;
   bool rendered = false;
+  // A frame is written either by ocr_render or by batches, never both: a
+  // batch writes the output set that was current when its views were set,
+  // with the batch's own cut list and counters, while ocr_render alternates
+  // sets and decodes through the frame's lists.  The first of the two to
+  // take the frame fixes its use; the other refuses it.
+  enum Use : int { kUnused = 0, kRender = 1, kBatch = 2 };
+  int use = kUnused;
   void select(int b) {
     cur = b;
     depth = out[b].depth;

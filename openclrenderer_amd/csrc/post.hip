@@ -339,7 +339,7 @@ extern "C" int ocr_frame_motion_blur(ocr_frame* f, ocr_scene* s, const ocr_view*
                                      float camera_contribution, void* stream) {
   if (!f || !s || !view || !c_pos_old || !c_rot_old) return fail(OCR_ERR_INVALID, "ocr_frame_motion_blur: bad argument");
   if (f->ctx != s->ctx) return fail(OCR_ERR_INVALID, "ocr_frame_motion_blur: frame and scene of different contexts");
-  if (!f->rendered) return fail(OCR_ERR_INVALID, "ocr_frame_motion_blur: render the frame first");
+  if (!f->rendered) return fail(OCR_ERR_INVALID, "ocr_frame_motion_blur: render the frame with ocr_render first");
   if (view->width != f->width || view->height != f->height)
     return fail(OCR_ERR_INVALID, "ocr_frame_motion_blur: view size differs from the frame's");
   const float fov = view->fov_const;

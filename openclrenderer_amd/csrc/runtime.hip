@@ -520,16 +520,40 @@ int ocr_scene_read_atlas(ocr_scene* s, uint8_t* pages, uint32_t* nums, uint32_t*
 }
 
 // ---------------------------------------------------------------- lights
+// A light upload that failed after the old buffers were freed: the scene is
+// left with no lights and no light buffers (passes then refuse it until a
+// successful upload), and the error code is passed through.
+static int lights_failed(ocr_scene* s, int rc) {
+  dev_free(s->lights);
+  dev_free(s->light_lin);
+  for (int v = 0; v < 2; v++) dev_free(s->maps_v[v]);
+  s->shadow_maps = nullptr;
+  dev_free(s->static_maps);
+  s->nlights = s->n_shadow = s->n_static = 0;
+  s->host_lights.clear();
+  s->static_dirty = false;
+  return rc;
+}
+
 int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_t light_buffer_dim) {
   if (!s || (n && !lights)) return fail(OCR_ERR_INVALID, "ocr_lights_upload: bad argument");
   if (light_buffer_dim < 8 || (uint64_t)light_buffer_dim * light_buffer_dim > kMaxTargetPixels)
     return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light buffer dimension must be 8..4000");
+  // every light is checked before anything of the scene is replaced
+  for (uint32_t i = 0; i < n; i++)
+    if (lights[i].shadow > 1)
+      return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light.shadow must be 0 or 1");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   OCR_HIP_CHECK(hipDeviceSynchronize());   // renders still reading the lights end first
   {
     std::lock_guard<std::mutex> lk(s->ver_mu);
     s->gen++;   // the light and shadow-map buffers are replaced
   }
+  // From here on a failure leaves the scene with no lights (nlights = 0, no
+  // buffers), which renders and batches refuse or render unlit, never with
+  // counts that point past freed buffers.
+  s->nlights = s->n_shadow = s->n_static = 0;
+  s->host_lights.clear();
   dev_free(s->lights);
   dev_free(s->light_lin);
   for (int v = 0; v < 2; v++) {
@@ -540,29 +564,32 @@ int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_
   s->shadow_maps = nullptr;
   s->mver = 0;
   dev_free(s->static_maps);
-  for (uint32_t i = 0; i < n; i++)
-    if (lights[i].shadow > 1)
-      return fail(OCR_ERR_UNSUPPORTED, "ocr_lights_upload: light.shadow must be 0 or 1");
+  uint32_t n_shadow = 0, n_static = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    if (lights[i].shadow == 1) n_shadow++;
+    if (lights[i].shadow && lights[i].is_static) n_static++;
+  }
+  s->lbuf_dim = light_buffer_dim;
+  const size_t per = (size_t)6 * light_buffer_dim * light_buffer_dim;
+  int rc;
+  if ((rc = dev_alloc(&s->lights, n, "lights")) || (rc = dev_alloc(&s->light_lin, (size_t)4 * n, "light colours")) ||
+      (rc = dev_alloc(&s->maps_v[0], per * std::max<uint32_t>(n_shadow, 1), "shadow maps")) ||
+      (rc = dev_alloc(&s->static_maps, per * std::max<uint32_t>(n_static, 1), "static shadow maps")))
+    return lights_failed(s, rc);
+  s->shadow_maps = s->maps_v[0];
+  hipError_t e = hipSuccess;
+  if (n) e = hipMemcpy(s->lights, lights, sizeof(ocr_light) * n, hipMemcpyHostToDevice);
+  if (n && e == hipSuccess) e = launch_light_prep(s->lights, n, s->light_lin, nullptr);
+  if (e == hipSuccess) e = launch_fill_u32(s->shadow_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(n_shadow, 1), nullptr);
+  if (e == hipSuccess) e = launch_fill_u32(s->static_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(n_static, 1), nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess)
+    return lights_failed(s, fail(OCR_ERR_HIP, std::string("ocr_lights_upload: ") + hipGetErrorString(e)));
+  // the counts become visible only with every buffer in place
   s->host_lights.assign(lights, lights + n);
   s->nlights = n;
-  s->lbuf_dim = light_buffer_dim;
-  s->n_shadow = s->n_static = 0;
-  for (uint32_t i = 0; i < n; i++) {
-    if (lights[i].shadow == 1) s->n_shadow++;
-    if (lights[i].shadow && lights[i].is_static) s->n_static++;
-  }
-  int rc;
-  size_t per = (size_t)6 * light_buffer_dim * light_buffer_dim;
-  if ((rc = dev_alloc(&s->lights, n, "lights")) || (rc = dev_alloc(&s->light_lin, (size_t)4 * n, "light colours")) ||
-      (rc = dev_alloc(&s->maps_v[0], per * std::max<uint32_t>(s->n_shadow, 1), "shadow maps")) ||
-      (rc = dev_alloc(&s->static_maps, per * std::max<uint32_t>(s->n_static, 1), "static shadow maps")))
-    return rc;
-  s->shadow_maps = s->maps_v[0];
-  if (n) OCR_HIP_CHECK(hipMemcpy(s->lights, lights, sizeof(ocr_light) * n, hipMemcpyHostToDevice));
-  if (n) OCR_HIP_CHECK(launch_light_prep(s->lights, n, s->light_lin, nullptr));
-  OCR_HIP_CHECK(launch_fill_u32(s->shadow_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_shadow, 1), nullptr));
-  OCR_HIP_CHECK(launch_fill_u32(s->static_maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_static, 1), nullptr));
-  OCR_HIP_CHECK(hipDeviceSynchronize());
+  s->n_shadow = n_shadow;
+  s->n_static = n_static;
   s->static_dirty = true;
   return OCR_OK;
 }
@@ -767,6 +794,7 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   if (s->nobj && s->max_tid >= std::max<uint32_t>(s->atlas.mip_start, 1))
     return fail(OCR_ERR_INVALID, "ocr_render: an object's texture id is outside the atlas");
   if (!s->lights && s->nlights) return fail(OCR_ERR_INVALID, "ocr_render: lights missing");
+  if (f->use == Frame::kBatch) return fail(OCR_ERR_INVALID, "ocr_render: the frame is rendered by batches");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   // this render's buffer set: the one the previous render did not use
   const int prev = f->cur, b = (f->cur + 1) % Frame::kSets;
@@ -794,6 +822,7 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   std::lock_guard<std::mutex> lk(s->ver_mu);
   OCR_HIP_CHECK(s->wait_geometry(sb));
   f->entry_valid[b] = true;
+  f->use = Frame::kRender;
   f->select(b);
   const Frame::OutBuf& o = f->out[b];
   const PassArgs pa = single_pass(s, view, f);
@@ -824,6 +853,7 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
 
 int ocr_frame_get_stats(ocr_frame* f, void* stream, ocr_frame_stats* out) {
   if (!f || !out) return fail(OCR_ERR_INVALID, "ocr_frame_get_stats: null argument");
+  if (f->use == Frame::kBatch) return fail(OCR_ERR_INVALID, "ocr_frame_get_stats: the frame is rendered by batches (ocr_batch_get_stats)");
   OCR_HIP_CHECK(hipSetDevice(f->ctx->device));
   OCR_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
   uint32_t c[kCounterWords];
@@ -864,7 +894,8 @@ static uint64_t popcount_bits(const std::vector<uint32_t>& v) {
 
 int ocr_frame_count_texels(ocr_scene* s, const ocr_view* view, ocr_frame* f, uint64_t* atlas_texels,
                            uint64_t* shadow_texels, void* stream) {
-  if (!s || !view || !f || !f->rendered) return fail(OCR_ERR_INVALID, "ocr_frame_count_texels: render first");
+  if (!s || !view || !f || !f->rendered)
+    return fail(OCR_ERR_INVALID, "ocr_frame_count_texels: render the frame with ocr_render first");
   if (view->width != f->width || view->height != f->height)
     return fail(OCR_ERR_INVALID, "ocr_frame_count_texels: view size differs from the frame");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
@@ -910,6 +941,8 @@ int ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id, uint8_t* rgb
   hipStream_t st = (hipStream_t)stream;
   OCR_HIP_CHECK(hipStreamSynchronize(st));
   uint32_t c[kCounterWords];
+  // a frame rendered by batches has no counters of its own (the batch's
+  // status comes from ocr_batch_get_stats); its counters stay zero
   OCR_HIP_CHECK(hipMemcpy(c, f->out[f->cur].counters, sizeof(c), hipMemcpyDeviceToHost));
   if (c[kOverflow] & kOverflowSamples)
     return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_read: the render needs more than 2^31 - 1 raster samples");

@@ -44,6 +44,7 @@ uint64_t frame_bytes(uint32_t w, uint32_t h) { return (uint64_t)w * h * 20; }
 struct Request {
   uint64_t id = 0;
   uint32_t scene = 0;
+  Scene* sc = nullptr;             // resolved under the lock at submit time
   ocr_view view{};
   uint64_t t_submit = 0, t_start = 0, t_done = 0;
   int32_t status = OCR_OK;
@@ -81,6 +82,8 @@ struct ocr_server {
   std::map<size_t, std::vector<void*>> idle_host;
   std::vector<void*> all_host;
   uint64_t budget = 0, held = 0;
+  uint64_t pooled = 0;             // HBM of every frame in all_frames
+  uint64_t n_evicted = 0;
   uint64_t next_id = 1;
   uint64_t n_batches = 0, n_done = 0, n_refused = 0, n_splits = 0;
   bool stop = false;
@@ -130,10 +133,16 @@ size_t host_bytes_for(const ocr_server* s, uint32_t w, uint32_t h) {
   return b;
 }
 
-// Frames and pinned buffers come from pools that only grow; admission keeps
-// the frames' HBM within the budget.  Called without the lock held.
+// Frames come from a pool per size.  The pool's HBM stays within the budget:
+// a frame in use belongs to an admitted request, whose bytes admission
+// already counted in `held` (<= budget), so freeing idle frames of other sizes
+// always makes room for a new one.  Frees synchronise the device, so they
+// happen only when the size mix changes under a full budget.  Called by the
+// dispatcher thread without the lock held.
 int take_frame(ocr_server* s, uint32_t w, uint32_t h, ocr_frame** out) {
   const uint64_t key = (uint64_t)w << 32 | h;
+  const uint64_t need = frame_bytes(w, h);
+  std::vector<ocr_frame*> evict;
   {
     std::lock_guard<std::mutex> g(s->mu);
     auto it = s->idle_frames.find(key);
@@ -142,20 +151,35 @@ int take_frame(ocr_server* s, uint32_t w, uint32_t h, ocr_frame** out) {
       it->second.pop_back();
       return OCR_OK;
     }
+    for (auto jt = s->idle_frames.begin(); jt != s->idle_frames.end() && s->pooled + need > s->budget; ++jt) {
+      while (!jt->second.empty() && s->pooled + need > s->budget) {
+        ocr_frame* f = jt->second.back();
+        jt->second.pop_back();
+        s->pooled -= frame_bytes(f->width, f->height);
+        s->all_frames.erase(std::find(s->all_frames.begin(), s->all_frames.end(), f));
+        evict.push_back(f);
+        s->n_evicted++;
+      }
+    }
+    s->pooled += need;   // reserved for the frame allocated below
   }
+  for (ocr_frame* f : evict) ocr_frame_destroy(f);
   ocr_frame* f = new (std::nothrow) ocr_frame();
-  if (!f) return fail(OCR_ERR_OOM, "server: host allocation");
-  f->ctx = s->ctx;
-  f->width = w;
-  f->height = h;
-  // one output set: server frames are only rendered in batches
-  const int rc = frame_alloc_set(f, 0);
+  int rc = f ? OCR_OK : fail(OCR_ERR_OOM, "server: host allocation");
+  if (f) {
+    f->ctx = s->ctx;
+    f->width = w;
+    f->height = h;
+    // one output set: server frames are only rendered in batches
+    rc = frame_alloc_set(f, 0);
+  }
+  std::lock_guard<std::mutex> g(s->mu);
   if (rc) {
-    ocr_frame_destroy(f);
+    if (f) ocr_frame_destroy(f);
+    s->pooled -= need;
     return rc;
   }
   f->select(0);
-  std::lock_guard<std::mutex> g(s->mu);
   s->all_frames.push_back(f);
   *out = f;
   return OCR_OK;
@@ -185,6 +209,13 @@ int take_host(ocr_server* s, size_t bytes, void** out) {
   return OCR_OK;
 }
 
+// With the lock held: the request's frame goes back to the idle pool.
+void pool_frame(ocr_server* s, Request* r) {
+  if (!r->frame) return;
+  s->idle_frames[(uint64_t)r->view.width << 32 | r->view.height].push_back(r->frame);
+  r->frame = nullptr;
+}
+
 // Renders one batch on a slot; returns the HIP/ABI status (the requests are
 // completed with it either way).
 int launch_batch(ocr_server* s, Slot& slot, std::vector<Request*>& reqs) {
@@ -200,7 +231,7 @@ int launch_batch(ocr_server* s, Slot& slot, std::vector<Request*>& reqs) {
       if (rc) return rc;
       r->host_bytes = host_bytes_for(s, r->view.width, r->view.height);
     }
-    sc[i] = static_cast<ocr_scene*>(s->scenes[r->scene]);
+    sc[i] = static_cast<ocr_scene*>(r->sc);
     vw[i] = r->view;
     fr[i] = r->frame;
   }
@@ -251,7 +282,7 @@ void dispatcher_main(ocr_server* s) {
     const uint32_t limit = std::min(s->cfg.max_batch_views, s->pending.front()->max_views);
     while (!s->pending.empty() && reqs.size() < limit) {
       Request* r = s->pending.front();
-      const uint64_t t = s->scenes[r->scene]->ntri;
+      const uint64_t t = r->sc->ntri;
       if (!reqs.empty() && tris + t > s->cfg.max_batch_tris) break;
       reqs.push_back(r);
       tris += t;
@@ -268,7 +299,7 @@ void dispatcher_main(ocr_server* s) {
       for (Request* r : reqs) {
         r->status = rc;
         r->t_done = t;
-        if (r->frame) s->idle_frames[(uint64_t)r->view.width << 32 | r->view.height].push_back(r->frame);
+        if (r->frame) pool_frame(s, r);
         r->frame = nullptr;
         s->completed.push_back(r);
       }
@@ -305,7 +336,7 @@ void completer_main(ocr_server* s) {
       const uint32_t half = (uint32_t)std::max<size_t>(1, slot.reqs.size() / 2);
       for (size_t i = slot.reqs.size(); i-- > 0;) {
         Request* r = slot.reqs[i];
-        s->idle_frames[(uint64_t)r->view.width << 32 | r->view.height].push_back(r->frame);
+        pool_frame(s, r);
         r->frame = nullptr;
         r->max_views = std::min(r->max_views, half);
         s->pending.push_front(r);
@@ -319,7 +350,7 @@ void completer_main(ocr_server* s) {
     for (Request* r : slot.reqs) {
       r->t_done = t;
       r->status = rc;
-      s->idle_frames[(uint64_t)r->view.width << 32 | r->view.height].push_back(r->frame);
+      pool_frame(s, r);
       r->frame = nullptr;
       s->completed.push_back(r);
       s->n_done++;
@@ -407,6 +438,9 @@ int ocr_server_submit(ocr_server* s, uint32_t scene_slot, const ocr_view* view, 
     delete r;
     return fail(OCR_ERR_INVALID, "ocr_server_submit: unknown scene slot");
   }
+  // the dispatcher reads the scene through this pointer without the lock
+  // (ocr_server_add_scene may grow the table meanwhile)
+  r->sc = s->scenes[scene_slot];
   if (s->held + r->hbm > s->budget) {
     s->n_refused++;
     delete r;
@@ -498,6 +532,8 @@ int ocr_server_get_info(ocr_server* s, ocr_server_info* info) {
   info->n_splits = s->n_splits;
   info->n_pending = (uint32_t)s->pending.size();
   info->n_frames = (uint32_t)s->all_frames.size();
+  info->frame_bytes = s->pooled;
+  info->n_evicted = s->n_evicted;
   return OCR_OK;
 }
 

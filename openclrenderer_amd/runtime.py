@@ -68,7 +68,8 @@ class Completion(C.Structure):
 class ServerInfo(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("hbm_held", C.c_uint64), ("n_batches", C.c_uint64),
                 ("n_done", C.c_uint64), ("n_refused", C.c_uint64), ("n_splits", C.c_uint64),
-                ("n_pending", C.c_uint32), ("n_frames", C.c_uint32)]
+                ("n_pending", C.c_uint32), ("n_frames", C.c_uint32), ("frame_bytes", C.c_uint64),
+                ("n_evicted", C.c_uint64)]
 
 
 READ_RGBA, READ_DEPTH, READ_IDKEY = 1, 2, 4

@@ -71,7 +71,7 @@ def test_server_records_match_header(tmp_path):
             ctypes.sizeof(runtime.ServerInfo), runtime.Completion.host.offset, runtime.Completion.status.offset,
             runtime.ServerConfig.hbm_fraction.offset, ctypes.sizeof(runtime.FrameStats)]
     assert got == want
-    assert got[:3] == [24, 64, 56]
+    assert got[:3] == [24, 64, 72]
 
 
 def test_engine_library_exports_scene_ingest():

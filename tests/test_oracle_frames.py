@@ -227,7 +227,7 @@ def test_mip_filter_closed_form(oracle):
 
 def test_config1_regression_fixture(oracle):
     """The committed config-1 hashes pin the oracle against silent change (no
-    reference output exists to compare with; see tests/golden/README)."""
+    reference output exists to compare with; see tests/golden/README.md)."""
     import hashlib
     import json
     import os
