@@ -110,6 +110,13 @@ class Oracle:
         vp = C.c_void_p
         L.oracle_motion_blur.argtypes = [vp, vp, fp, vp, vp, vp, vp, u, vp, u, u, C.c_float, C.c_float, vp]
         L.oracle_sample_linear.argtypes = [vp, u, u, C.c_float, C.c_float, fp]
+        L.oracle_ssao.argtypes = [i, i, vp, u, u, f]
+        L.oracle_ssao.restype = f
+        L.oracle_shadow_term.argtypes = [fp, fp, fp, vp, u, u]
+        L.oracle_shadow_term.restype = f
+        L.oracle_texture_filter.argtypes = [f, f, f, f, i, u, vp, vp, vp, fp]
+        L.oracle_shadow_map_one.argtypes = [C.POINTER(OracleScene), u, i, vp, vp]
+        L.oracle_shadow_map_one.restype = i
 
     # --------------------------------------------------------------- helpers
     @staticmethod
@@ -176,6 +183,43 @@ class Oracle:
         out, po = self._f(np.zeros(4, np.float32))
         self.lib.oracle_sample_linear(img.ctypes.data, W, H, u, v, po)
         return out
+
+    def ssao(self, depth, x, y, fov):
+        """generate_ssao at pixel (x, y) of an H x W u32 depth buffer."""
+        d = np.ascontiguousarray(depth, dtype=np.uint32)
+        H, W = d.shape
+        return self.lib.oracle_ssao(int(x), int(y), d.ctypes.data, W, H, float(fov))
+
+    def shadow_term(self, gpos, normal, lpos, maps, shnum=0):
+        """kernel3's shadow term of one light (0 lit .. 1 shadowed) from cube
+        maps shaped (n, 6, L, L)."""
+        m = np.ascontiguousarray(maps, dtype=np.uint32)
+        g, pg = self._f(gpos)
+        n, pn = self._f(normal)
+        lp, pl = self._f(lpos)
+        return self.lib.oracle_shadow_term(pg, pn, pl, m.ctypes.data, shnum, m.shape[-1])
+
+    def texture_filter(self, atlas, vt, dx, dy, tid=0):
+        """texture_filter_diff on an atlas (pages, nums, sizes, mip_start)."""
+        pages, nums, sizes, mip_start = atlas
+        out, po = self._f(np.zeros(4, np.float32))
+        self.lib.oracle_texture_filter(float(vt[0]), float(vt[1]), float(dx), float(dy), tid, mip_start,
+                                       _ptr(nums), _ptr(sizes), _ptr(pages), po)
+        return out
+
+    def shadow_map_one(self, scene, light, only_static=False):
+        """One light's (6, L, L) cube map, rendered sequentially, and the
+        fragment records per face."""
+        L = scene.light_buffer_dim
+        m = np.zeros((6, L, L), np.uint32)
+        frags = np.zeros(6, np.uint64)
+        dummy_atlas = (np.zeros((1, 1, 1, 4), np.uint8), np.zeros(1, np.uint32),
+                       np.zeros(1, np.uint32), 0)
+        s = self._scene(scene, dummy_atlas, None)
+        rc = self.lib.oracle_shadow_map_one(C.byref(s), light, 1 if only_static else 0, _ptr(m), _ptr(frags))
+        if rc != 0:
+            raise ValueError("oracle_shadow_map_one: no such light")
+        return m, frags
 
     def face_sincos(self):
         arr = (SinCos * 6)()

@@ -30,6 +30,14 @@
 #define MIPL         4                  /* MIP_LEVELS, cl2.cl:5 */
 #define MIP_BIAS_F   1.1f               /* cl2.cl:5750-5752 */
 
+/* Planted mistakes for the mutation test (tests/test_oracle_pins.py builds
+ * copies with -DOCR_ORACLE_PLANT=k and checks that each fails a pin).  The
+ * oracle proper is built with 0: every PLANT(k) below is false. */
+#ifndef OCR_ORACLE_PLANT
+#define OCR_ORACLE_PLANT 0
+#endif
+#define PLANT(k) (OCR_ORACLE_PLANT == (k))
+
 typedef struct { float x, y, z; } f3;
 typedef struct { float x, y, z, w; } f4;
 
@@ -556,6 +564,7 @@ static f4 texture_filter_diff(float vtx, float vty, float dx, float dy, int tid2
     float mip_lower = floorf(wf);
     mip_lower = clampf_(mip_lower, 0.f, (float)MIPL);
     float fmd = wf - mip_lower;
+    if (PLANT(10)) fmd = 1.f - fmd;
     int tid_lower = mip_lower == 0 ? tid2
                   : (int)(mip_lower - 1 + (float)mip_start + (float)(tid2 * MIPL));
     int tid_higher = (int)(clampf_(mip_lower, 0.f, MIPL - 1.f) + (float)mip_start + (float)(tid2 * MIPL));
@@ -578,7 +587,7 @@ uint32_t oracle_wang_hash(uint32_t seed) {     /* cl2.cl:1919-1927 */
     seed = (seed ^ 61u) ^ (seed >> 16);
     seed *= 9u;
     seed = seed ^ (seed >> 4);
-    seed *= 0x27d4eb2du;
+    seed *= PLANT(4) ? 0x27d4eb2fu : 0x27d4eb2du;
     seed = seed ^ (seed >> 15);
     return seed;
 }
@@ -618,7 +627,7 @@ void oracle_gamma_inverse(const float in[4], float out[4]) {
 static inline int ret_cubeface(f3 point, f3 light) {
     f3 rel = sub3(point, light);
     f3 arel = V3(fabsf(rel.x), fabsf(rel.y), fabsf(rel.z));
-    if (arel.x >= arel.y && arel.x >= arel.z) return rel.x < 0 ? 4 : 5;
+    if (arel.x >= arel.y && arel.x >= arel.z) return (rel.x < 0) != PLANT(11) ? 4 : 5;
     if (arel.y > arel.x && arel.y >= arel.z) return rel.y < 0 ? 1 : 3;
     if (arel.z > arel.x && arel.z > arel.y) { if (rel.z < 0) return 2; }
     return 0;
@@ -678,7 +687,7 @@ static float generate_ssao(int sx, int sy, const uint32_t* depth_buffer, uint32_
         }
     }
     acc /= 125.f;       /* pow(samples*2.f + 1.f, 3.f) */
-    return 1.f - (1.f - acc) / SSAO_DIV_F;
+    return 1.f - (1.f - acc) / (PLANT(8) ? 2.0f : SSAO_DIV_F);
 }
 
 /* bilinear_interpolate, cl2.cl:2286-2302 */
@@ -696,7 +705,7 @@ static float generate_hard_occlusion(f3 lpos, f3 normal, f3 ptl, const uint32_t*
     ptl = fnorm3(ptl);
     f3 local_pos = rot(global_position, lpos, &face_sc[face]);
     float lf = (float)L;
-    float lfov = lf / 2.0f;                       /* LFOV_CONST, cl2.cl:11 */
+    float lfov = PLANT(12) ? lf : lf / 2.0f;      /* LFOV_CONST, cl2.cl:11 */
     f3 post = depth_project1(local_pos, lf, lf, lfov);
     float dpth = post.z;
     const uint32_t* map = &light_depth_buffer[((size_t)face + (size_t)shnum * 6) * L * L];
@@ -706,11 +715,12 @@ static float generate_hard_occlusion(f3 lpos, f3 normal, f3 ptl, const uint32_t*
     float acos_res = oracle_rational_acos(clampf_(dot3(normal, ptl), 0.05f, 0.95f));
     float bias = SHADOWBIAS_I * oracle_tan(acos_res);
     bias = clampf_(bias, 0.1f * SHADOWBIAS_I, (float)SHADOWBIAS_I);   /* pow(50, SHADOWEXP=1) */
+    if (PLANT(3)) bias = -bias;
     float shadow = 0.f;
     int cond[16];
     for (int y = -1; y <= 2; y++)
         for (int x = -1; x <= 2; x++) {
-            uint32_t raw = map[(size_t)(iy + y) * L + (size_t)(ix + x)];
+            uint32_t raw = map[(size_t)(iy + y + PLANT(9)) * L + (size_t)(ix + x)];
             float ldp1 = ((float)raw / MULINT_F) * DEPTH_FAR;
             cond[(y + 1) * 4 + x + 1] = dpth > ldp1 + bias ? 1 : 0;
         }
@@ -821,7 +831,7 @@ static void shade_pixel(const oracle_scene* sc, const ocr_view* v, const oracle_
         float vtyy = fmaf(vt1y, ly1, fmaf(vt2y, ly2, vt3y * ly3));
         float vdxx = fabsf(vtxx - vtx), vdxy = fabsf(vtxy - vty);
         float vdyx = fabsf(vtyx - vtx), vdyy = fabsf(vtyy - vty);
-        const float mip_bias = 1.f / MIP_BIAS_F;
+        const float mip_bias = PLANT(2) ? MIP_BIAS_F : 1.f / MIP_BIAS_F;
         vdiffx = (vdxx + vdyx) * mip_bias;
         vdiffy = (vdxy + vdyy) * mip_bias;
     }
@@ -940,8 +950,9 @@ void oracle_face_sincos(oracle_sincos out[6]) {
                              {0, M_PI_F / 2.0f, 0}};
     for (int f = 0; f < 6; f++)
         for (int a = 0; a < 3; a++) {
-            out[f].s[a] = (float)sin((double)ang[f][a]);
-            out[f].c[a] = (float)cos((double)ang[f][a]);
+            const int g = PLANT(1) ? (f == 1 ? 3 : f == 3 ? 1 : f) : f;
+            out[f].s[a] = (float)sin((double)ang[g][a]);
+            out[f].c[a] = (float)cos((double)ang[g][a]);
         }
 }
 
@@ -1028,9 +1039,10 @@ static void ksh_visit(void* c, int x, int y, uint32_t d) {
 }
 
 static void render_light_map(const oracle_scene* sc, f3 lpos, int only_static,
-                             const oracle_sincos face_sc[6], int nt, uint32_t* map) {
+                             const oracle_sincos face_sc[6], int nt, uint32_t* map, uint64_t* frags) {
     const uint32_t L = sc->lbuf_dim;
-    const float efov = (float)L / 2.0f, ew = (float)L, eh = (float)L;
+    const float efov = PLANT(5) ? (float)L : (float)L / 2.0f, ew = (float)L, eh = (float)L;
+    const int op_l = PLANT(6) ? 500 : OP_SIZE_L;
     #pragma omp parallel for schedule(dynamic, 1024) num_threads(nt)
     for (long id = 0; id < (long)sc->ntri; id++) {
         const ocr_triangle* T = &sc->tris[id];
@@ -1049,6 +1061,7 @@ static void render_light_map(const oracle_scene* sc, f3 lpos, int only_static,
             f3 r = add3(rot_quat(mul3(ld3(T->vertices[kk].pos), G->scale), q), wp);
             skip[ret_cubeface(r, lpos)] = 1;
         }
+        if (PLANT(7)) for (int kk = 0; kk < 6; kk++) skip[kk] = 1;
         int two_sided = (ff & OCR_FEATURE_TWO_SIDED) > 0;
         for (int kk = 0; kk < 6; kk++) {
             if (!skip[kk]) continue;
@@ -1070,10 +1083,14 @@ static void render_light_map(const oracle_scene* sc, f3 lpos, int only_static,
                 float mm[4];
                 calc_min_max(p, ew, eh, mm);
                 float area = (mm[1] - mm[0]) * (mm[3] - mm[2]);
-                float thread_num = ceilf(area / (float)OP_SIZE_L);
+                float thread_num = ceilf(area / (float)op_l);
                 kshctx k = {map, L, (uint32_t)kk, nt > 1};
+                if (frags) {
+                    #pragma omp atomic
+                    frags[kk] += (uint64_t)thread_num;
+                }
                 for (uint32_t a = 0; (float)a < thread_num; a++)
-                    walk_fragment(p, ew, eh, a, OP_SIZE_L, rconst, ksh_visit, &k);
+                    walk_fragment(p, ew, eh, a, op_l, rconst, ksh_visit, &k);
             }
         }
     }
@@ -1092,13 +1109,13 @@ int oracle_shadow_maps(const oracle_scene* sc, int nthreads, uint32_t* dyn_maps,
         if (l->shadow == 1) {
             uint32_t* m = &dyn_maps[per * nn];
             for (size_t j = 0; j < per; j++) m[j] = 0xFFFFFFFFu;
-            render_light_map(sc, lpos, 0, face_sc, nt, m);
+            render_light_map(sc, lpos, 0, face_sc, nt, m, NULL);
             nn++;
         }
         if (l->shadow && l->is_static) {
             uint32_t* m = &static_maps[per * kk];
             for (size_t j = 0; j < per; j++) m[j] = 0xFFFFFFFFu;
-            render_light_map(sc, lpos, 1, face_sc, nt, m);
+            render_light_map(sc, lpos, 1, face_sc, nt, m, NULL);
             kk++;
         }
     }
@@ -1301,6 +1318,39 @@ void oracle_get_barycentric(const float p[3], const float a[3], const float b[3]
     get_barycentric(ld3(p), ld3(a), ld3(b), ld3(c), u, v, w);
 }
 uint32_t oracle_encode_normal(const float n[3]) { return encode_normal(ld3(n)); }
+
+float oracle_ssao(int x, int y, const uint32_t* depth, uint32_t W, uint32_t H, float fov) {
+    return generate_ssao(x, y, depth, W, H, fov);
+}
+
+/* The shadow term of one light as kernel3 forms it (cl2.cl:6115-6165): the
+ * cube face of the point, then generate_hard_occlusion on map `shnum`. */
+float oracle_shadow_term(const float gpos[3], const float normal[3], const float lpos[3],
+                         const uint32_t* maps, uint32_t shnum, uint32_t L) {
+    oracle_sincos face_sc[6];
+    oracle_face_sincos(face_sc);
+    f3 g = ld3(gpos), lp = ld3(lpos);
+    int face = ret_cubeface(g, lp);
+    return generate_hard_occlusion(lp, ld3(normal), sub3(lp, g), maps, face, g, (int)shnum, L, face_sc);
+}
+
+void oracle_texture_filter(float vtx, float vty, float dx, float dy, int tid, uint32_t mip_start,
+                           const uint32_t* nums, const uint32_t* sizes, const uint8_t* atlas, float out[4]) {
+    f4 c = texture_filter_diff(vtx, vty, dx, dy, tid, mip_start, nums, sizes, atlas);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = c.w;
+}
+
+int oracle_shadow_map_one(const oracle_scene* sc, uint32_t light, int only_static, uint32_t* map,
+                          uint64_t frags[6]) {
+    if (light >= sc->nlights) return -1;
+    oracle_sincos face_sc[6];
+    oracle_face_sincos(face_sc);
+    const size_t per = (size_t)6 * sc->lbuf_dim * sc->lbuf_dim;
+    for (size_t j = 0; j < per; j++) map[j] = 0xFFFFFFFFu;
+    if (frags) for (int k = 0; k < 6; k++) frags[k] = 0;
+    render_light_map(sc, ld3(sc->lights[light].pos), only_static, face_sc, 1, map, frags);
+    return 0;
+}
 void oracle_texture_mod(const float vt[2], float out[2]) { out[0] = tmod1(vt[0]); out[1] = tmod1(vt[1]); }
 
 /* blend_screens_with_depth, cl2.cl:7842-7875: the pixel is left alone when

@@ -127,6 +127,22 @@ float oracle_pow4(float x);
 void  oracle_get_barycentric(const float p[3], const float a[3], const float b[3],
                              const float c[3], float* u, float* v, float* w);
 uint32_t oracle_encode_normal(const float n[3]);
+/* generate_ssao (cl2.cl:2194-2260) at pixel (x, y) of a depth buffer. */
+float oracle_ssao(int x, int y, const uint32_t* depth, uint32_t W, uint32_t H, float fov);
+/* The shadow term kernel3 forms for one light: ret_cubeface of the point,
+ * then generate_hard_occlusion (cl2.cl:2536-2701) on cube map `shnum` of
+ * `maps` (n x 6 x L x L u32).  0 = lit, 1 = fully shadowed. */
+float oracle_shadow_term(const float gpos[3], const float normal[3], const float lpos[3],
+                         const uint32_t* maps, uint32_t shnum, uint32_t L);
+/* texture_filter_diff (cl2.cl:1511-1573) on an atlas: RGBA in [0, 1]. */
+void  oracle_texture_filter(float vtx, float vty, float dx, float dy, int tid, uint32_t mip_start,
+                            const uint32_t* nums, const uint32_t* sizes, const uint8_t* atlas,
+                            float out[4]);
+/* One light's cube map (6 x L x L, from the dynamic geometry, or the static
+ * one with only_static), sequentially, with the fragment records
+ * prearrange_realtime_shadowing emits per face (may be NULL). */
+int   oracle_shadow_map_one(const oracle_scene* sc, uint32_t light, int only_static, uint32_t* map,
+                            uint64_t frags[6]);
 void  oracle_texture_mod(const float vt[2], float out[2]);
 
 /* blend_screens_with_depth (cl2.cl:7842-7875) with alpha_blend

@@ -1,0 +1,480 @@
+"""Pins for the oracle's cube shadow maps, shadow lookup, mip selection, SSAO
+and hash, each against something the geometry or the algebra fixes
+independently of the oracle's code:
+
+* shadow maps (prearrange_realtime_shadowing + kernel1_realtime_shadowing,
+  cl2.cl:4419-4636, :5129-5246): a square straight along each cube axis lands
+  at the centre of the face ret_cubeface names for that axis (cl2.cl:1745-1790),
+  with the size LFOV_CONST = LIGHTBUFFERDIM / 2 (cl2.cl:11) gives it, its
+  depth, and ceil(box area / 300) fragment records per piece (op_size_light,
+  cl2.cl:4249); a marker beside it lands where the face's rotation matrix
+  (cl2.cl:4487-4511, the matrix of the comment at cl2.cl:208-212) puts it; a
+  triangle whose vertices miss a face is not drawn into it (cl2.cl:4520-4539);
+  and every texel well inside triangles holds the minimum depth over the
+  triangles covering it, computed by brute force;
+* generate_hard_occlusion (cl2.cl:2536-2701): no blocker gives 0, a blocker
+  everywhere gives 1, the bias is SHADOWBIAS = 50 where n.l <= 0.05 and
+  50 tan(acos 0.95) = 16.4 where n.l = 1, and a straight shadow edge gives the
+  mean of the piecewise-linear edge profile at three texel offsets;
+* mip selection (get_vtdiff + texture_filter_diff, cl2.cl:5691-5760,
+  :1511-1573): on a camera-facing quad with a known UV gradient and an atlas
+  whose five levels have distinct constant alphas, the output alpha is the
+  blend of levels floor(w) and floor(w)+1 by frac(w), w = log2 of the texel
+  footprint divided by MIP_BIAS = 1.1;
+* SSAO (generate_ssao, cl2.cl:2194-2260): counts of passing depth tests on a
+  flat plane, a depth step seen from either side and a ramp;
+* wang_hash (cl2.cl:1919-1927): the inverse of the published hash (Thomas
+  Wang's integer hash as listed by N. Reed, "Quick and easy GPU random
+  numbers in D3D11", 2013: xor 61 and >>16, *9, ^>>4, *0x27d4eb2d, ^>>15)
+  undoes the oracle's, and its only zero is at 61.
+
+test_planted_mistakes builds copies of the oracle with one mistake each
+(OCR_ORACLE_PLANT in oracle/ocr_oracle.c) and checks that every copy fails at
+least one of these pins.
+"""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from openclrenderer_amd import abi, scenes
+from oracle.binding import Oracle
+
+FAR = 350000.0
+U32 = 2.0 ** 32
+PI_F = float(np.float32(3.1415927))
+# r_struct (cl2.cl:4487-4511) in binary32, and the axis ret_cubeface assigns
+# each face (cl2.cl:1745-1790).
+R_STRUCT = [(0, 0, 0), (PI_F / 2, 0, 0), (0, PI_F, 0), (3 * PI_F / 2, 0, 0), (0, 3 * PI_F / 2, 0),
+            (0, PI_F / 2, 0)]
+FACE_AXIS = {0: (0, 0, 1), 1: (0, -1, 0), 2: (0, 0, -1), 3: (0, 1, 0), 4: (-1, 0, 0), 5: (1, 0, 0)}
+
+
+def rot_matrix(ang):
+    """The rotation of the comment at cl2.cl:208-212 (1 = x, 2 = y, 3 = z)."""
+    s1, s2, s3 = (math.sin(float(np.float32(a))) for a in ang)
+    c1, c2, c3 = (math.cos(float(np.float32(a))) for a in ang)
+    return np.array([[c3 * c2, s3 * c2, -s2],
+                     [c3 * s1 * s2 - s3 * c1, s3 * s1 * s2 + c3 * c1, s1 * c2],
+                     [c3 * c1 * s2 + s3 * s1, s3 * c1 * s2 - c3 * s1, c1 * c2]])
+
+
+def qdepth(z):
+    """The quantised depth of distance z: UINT_MAX * z / depth_far."""
+    return z / FAR * U32
+
+
+def raw_depth(z):
+    return np.uint32(min(round(qdepth(z)), 0xFFFFFFFF))
+
+
+def _scene(tris_world, L=64, light=(0.0, 0.0, 0.0), flags=abi.FEATURE_TWO_SIDED):
+    tw = np.asarray(tris_world, np.float64).reshape(-1, 3, 3)
+    t = np.zeros(tw.shape[0], dtype=abi.TRIANGLE)
+    t["v"]["pos"][:, :, :3] = tw
+    t["v"]["normal"][:, :, 2] = 1.0
+    objs = scenes._desc((0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0), 1.0, flags=flags)
+    return scenes.Scene(t, objs, scenes.make_light(light, shadow=1), [], L, "pin")
+
+
+def _square(centre, u, v, h):
+    """Two triangles of the square centre +- h*u +- h*v."""
+    c, u, v = (np.asarray(a, np.float64) for a in (centre, u, v))
+    p = [c - h * u - h * v, c + h * u - h * v, c + h * u + h * v, c - h * u + h * v]
+    return [[p[0], p[1], p[2]], [p[0], p[2], p[3]]]
+
+
+def _perp(axis):
+    a = np.asarray(axis, np.float64)
+    u = np.array([0.0, 0.0, 1.0]) if abs(a[2]) < 0.5 else np.array([1.0, 0.0, 0.0])
+    u = u - a * (u @ a)
+    u /= np.linalg.norm(u)
+    return u, np.cross(a, u)
+
+
+def cross2(a, b):
+    return a[0] * b[1] - a[1] * b[0]
+
+
+def _covered(face_map):
+    return face_map != 0xFFFFFFFF
+
+
+# ------------------------------------------------------------ shadow maps
+def pin_face_orientation(o):
+    """A square straight along each face's axis lands at that face's centre
+    (and nowhere else) with its distance as depth; a marker offset along u
+    lands where the face's rotation puts it; the square's side on the face is
+    2h * LFOV / D texels (LFOV = L / 2); each of its two triangles emits
+    ceil(box area / 300) fragment records."""
+    L, D, h = 64, 1000.0, 600.0
+    for f, axis in FACE_AXIS.items():
+        a = np.asarray(axis, np.float64)
+        u, v = _perp(a)
+        tris = _square(D * a, u, v, h)
+        # a small marker square, offset by 0.3 D along u, 0.5 D nearer
+        mk = _square(0.5 * D * a + 0.15 * D * u, u, v, 0.02 * D)
+        m, frags = o.shadow_map_one(_scene(tris + mk, L), 0)
+        for g in range(6):
+            if g != f:
+                assert not _covered(m[g]).any(), (f, g, "content on a face no vertex falls in")
+        cov = _covered(m[f])
+        near = m[f] < raw_depth(0.75 * D)          # the marker, nearer than the square
+        big = cov & ~near
+        ys, xs = np.nonzero(cov)
+        assert cov.any(), (f, "the square is missing from its face")
+        # centre of the square: (L/2, L/2) on the face, up to the half-open box
+        assert abs(xs.mean() - L / 2) < 1.5 and abs(ys.mean() - L / 2) < 1.5, (f, xs.mean(), ys.mean())
+        side = 2 * h * (L / 2) / D
+        assert abs(cov.sum() - side * side) < 0.2 * side * side, (f, cov.sum(), side * side)
+        d = m[f][big]
+        assert np.all(np.abs(d.astype(np.float64) - qdepth(D)) <= 1e-5 * qdepth(D) + 2), f
+        # marker: local = R_f (p - light), projected with LFOV = L/2
+        loc = rot_matrix(R_STRUCT[f]) @ (0.5 * D * a + 0.15 * D * u)
+        px, py = loc[0] * (L / 2) / loc[2] + L / 2, loc[1] * (L / 2) / loc[2] + L / 2
+        ny, nx = np.nonzero(near)
+        assert near.any(), (f, "marker missing")
+        assert abs(nx.mean() - px) < 1.5 and abs(ny.mean() - py) < 1.5, (f, nx.mean(), ny.mean(), px, py)
+        # fragment records of the square's two triangles: the box of the
+        # rounded projected vertices (min - 1, clamped to the face)
+        want = 0
+        for tri in tris:
+            pr = np.array([rot_matrix(R_STRUCT[f]) @ np.asarray(p) for p in tri])
+            sx = np.floor(pr[:, 0] * (L / 2) / pr[:, 2] + L / 2 + 0.5)
+            sy = np.floor(pr[:, 1] * (L / 2) / pr[:, 2] + L / 2 + 0.5)
+            bx = min(max(sx.max(), 0), L - 1) - min(max(sx.min() - 1, 0), L - 1)
+            by = min(max(sy.max(), 0), L - 1) - min(max(sy.min() - 1, 0), L - 1)
+            want += math.ceil(bx * by / 300)
+        mk_frags = int(frags[f]) - want
+        assert 1 <= mk_frags <= 2, (f, int(frags[f]), want)
+
+
+def pin_vertex_face_rule(o):
+    """A triangle with vertices towards +x, +y and -x passes over the +z
+    direction, yet no vertex falls in face 0, so face 0 stays empty (the
+    reference draws a triangle only into faces its vertices fall in)."""
+    tri = [[1000.0, 0.0, 100.0], [0.0, 1000.0, 100.0], [-1000.0, 0.0, 100.0]]
+    # the +z axis point (0, 10, 100) is inside the triangle's plane figure
+    p = np.array([0.0, 10.0])
+    a, b, c = (np.array(t[:2]) for t in tri)
+    cr = [cross2(b - a, p - a), cross2(c - b, p - b), cross2(a - c, p - c)]
+    assert all(x > 0 for x in cr) or all(x < 0 for x in cr)
+    m, frags = o.shadow_map_one(_scene([tri], 64), 0)
+    assert not _covered(m[0]).any() and frags[0] == 0
+    assert not _covered(m[1]).any() and not _covered(m[2]).any()
+    assert _covered(m[3]).any() and _covered(m[4]).any() and _covered(m[5]).any()
+
+
+def pin_min_depth_brute_force(o):
+    """Random triangles facing the light inside the +z face at distinct
+    distances: every texel well inside some triangles holds the minimum of
+    their depths, every texel well outside all of them stays clear."""
+    rng = np.random.default_rng(77)
+    L = 96
+    tris, zs = [], []
+    for k in range(14):
+        z = 400.0 + 180.0 * k + rng.uniform(0, 50)
+        c = rng.uniform(-0.45, 0.45, 2) * z
+        pts = [np.array([*(c + rng.uniform(-0.4, 0.4, 2) * z), z]) for _ in range(3)]
+        tris.append(pts)
+        zs.append(z)
+    order = rng.permutation(len(tris))
+    tris = [tris[i] for i in order]
+    zs = [zs[i] for i in order]
+    m, _ = o.shadow_map_one(_scene(tris, L), 0)
+    face = m[0].astype(np.float64)
+    proj = [np.array([[p[0] * (L / 2) / p[2] + L / 2, p[1] * (L / 2) / p[2] + L / 2] for p in t]) for t in tris]
+
+    def signed_dist(P, x, y):
+        """Smallest signed distance of (x, y) to the triangle's edges
+        (positive inside)."""
+        area = cross2(P[1] - P[0], P[2] - P[0])
+        sgn = 1.0 if area > 0 else -1.0
+        out = np.inf
+        for i in range(3):
+            a, b = P[i], P[(i + 1) % 3]
+            e = b - a
+            out = min(out, sgn * cross2(e, np.array([x, y]) - a) / np.linalg.norm(e))
+        return out
+
+    checked_in = checked_out = 0
+    for y in range(2, L - 2):
+        for x in range(2, L - 2):
+            ds = [signed_dist(P, x, y) for P in proj]
+            if any(abs(d) <= 1.0 for d in ds):
+                continue   # within the rounding of some edge
+            inside = [zs[k] for k, d in enumerate(ds) if d > 1.0]
+            if inside:
+                want = qdepth(min(inside))
+                assert abs(face[y, x] - want) <= 1e-5 * want + 2, (x, y, face[y, x], want)
+                checked_in += 1
+            else:
+                assert m[0][y, x] == 0xFFFFFFFF, (x, y)
+                checked_out += 1
+    assert checked_in > 300 and checked_out > 500, (checked_in, checked_out)
+
+
+# ---------------------------------------------------------- shadow lookup
+def _face_point(f, px, py, D, L):
+    """A world point (light at the origin) that face f's projection puts at
+    (px, py) with depth D."""
+    loc = np.array([(px - L / 2) * D / (L / 2), (py - L / 2) * D / (L / 2), D])
+    return rot_matrix(R_STRUCT[f]).T @ loc
+
+
+def pin_hard_occlusion(o):
+    L, D = 64, 5000.0
+    maps = np.full((2, 6, L, L), 0xFFFFFFFF, np.uint32)
+    for f in (0, 3, 4):
+        g = _face_point(f, 30.3, 25.6, D, L)
+        ptl = -g / np.linalg.norm(g)
+        side = np.cross(ptl, [0.3, 0.5, 0.81])
+        side /= np.linalg.norm(side)          # n.l = 0 -> clamp 0.05 -> bias 50
+        # no blocker anywhere
+        assert o.shadow_term(g, side, (0, 0, 0), maps, 0) == 0.0, f
+        # a blocker in front on the whole face of map 1 only
+        m = maps.copy()
+        m[1, f] = raw_depth(D - 100)
+        assert o.shadow_term(g, side, (0, 0, 0), m, 1) == 1.0, f
+        assert o.shadow_term(g, side, (0, 0, 0), m, 0) == 0.0, f
+        # bias 50: a blocker 45 in front is within the bias, 55 is not
+        m[1, f] = raw_depth(D - 45)
+        assert o.shadow_term(g, side, (0, 0, 0), m, 1) == 0.0, f
+        m[1, f] = raw_depth(D - 55)
+        assert o.shadow_term(g, side, (0, 0, 0), m, 1) == 1.0, f
+        # n.l = 1: bias = 50 tan(acos(0.95)) = 16.4 (16.4 +- 1 for the
+        # rational acos): 10 in front is lit, 22 is shadowed
+        m[1, f] = raw_depth(D - 10)
+        assert o.shadow_term(g, ptl, (0, 0, 0), m, 1) == 0.0, f
+        m[1, f] = raw_depth(D - 22)
+        assert o.shadow_term(g, ptl, (0, 0, 0), m, 1) == 1.0, f
+        # straight shadow edges: the 0/1 field read bilinearly at the point
+        # and one texel either side, averaged: the mean of clamp(E - t, 0, 1)
+        # at t = p - 1, p, p + 1 for an edge at texel index E
+        for px, py in ((30.3, 25.6), (21.8, 40.45), (12.5, 12.25)):
+            g = _face_point(f, px, py, D, L)
+            for E in (int(px), int(px) + 1, int(px) + 2):
+                m = maps.copy()
+                m[0, f, :, :E] = raw_depth(D - 100)
+                want = sum(min(max(E - (px + k), 0.0), 1.0) for k in (-1, 0, 1)) / 3
+                got = o.shadow_term(g, side, (0, 0, 0), m, 0)
+                assert abs(got - want) < 2e-5, (f, px, E, got, want)
+            for E in (int(py), int(py) + 1, int(py) + 2):
+                m = maps.copy()
+                m[0, f, :E, :] = raw_depth(D - 100)
+                want = sum(min(max(E - (py + k), 0.0), 1.0) for k in (-1, 0, 1)) / 3
+                got = o.shadow_term(g, side, (0, 0, 0), m, 0)
+                assert abs(got - want) < 2e-5, (f, py, E, got, want)
+
+
+# ------------------------------------------------------------ mip selection
+ALPHAS = [0, 50, 100, 150, 200]
+
+
+def _level_atlas(o, size=64):
+    """An atlas of one texture whose level l is the constant (128, 128, 128,
+    ALPHAS[l])."""
+    tex = scenes.Texture(size, size, np.zeros((size, size, 4), np.uint8))
+    pages, nums, sizes, mip_start = o.atlas([tex])
+    for lvl in range(5):
+        d = 0 if lvl == 0 else mip_start + lvl - 1
+        slice_, which = int(nums[d]) >> 16, int(nums[d]) & 0xFFFF
+        s = int(sizes[slice_])
+        per_row = 2048 // s
+        ox, oy = (which % per_row) * s, (which // per_row) * s
+        pages[slice_, oy:oy + s, ox:ox + s] = (128, 128, 128, ALPHAS[lvl])
+    return pages, nums, sizes, mip_start
+
+
+def _alpha_of(w):
+    """Alpha of a footprint whose log2 is w: levels floor(w), floor(w) + 1
+    blended by frac(w); level 4 beyond."""
+    w = max(w, 0.0)
+    if w >= 4:
+        return ALPHAS[4]
+    m = int(math.floor(w))
+    return ALPHAS[m] + (ALPHAS[m + 1] - ALPHAS[m]) * (w - m)
+
+
+def pin_texture_filter(o):
+    """texture_filter_diff alone: footprint sqrt((dx*s)^2 + (dy*s)^2) for a
+    size-s texture."""
+    atlas = _level_atlas(o)
+    for w in np.linspace(-1.0, 5.5, 53):
+        if abs(w - round(w)) < 0.03:
+            continue   # log2_approx is within 0.0055 of log2
+        for dx_share in (1.0, 0.6):
+            foot = 2.0 ** w
+            dx = foot * dx_share / 64
+            dy = foot * math.sqrt(1 - dx_share ** 2) / 64
+            got = o.texture_filter(atlas, (0.3, 0.7), dx, dy)[3] * 255
+            assert abs(got - _alpha_of(w)) < 1.0, (w, dx_share, got, _alpha_of(w))
+
+
+def pin_mip_selection_frame(o):
+    """A camera-facing quad whose UV changes by k per pixel along x and y:
+    get_vtdiff gives k / 1.1 on both axes, so the footprint is
+    sqrt(2) * k * 64 / 1.1 texels per pixel."""
+    atlas = _level_atlas(o)
+    W, H = 64, 48
+    view = scenes.make_view(W, H, (0.0, 0.0, 0.0), hfov=60.0)
+    F = float(view["fov_const"][0])
+    Z = 3000.0
+    for w in (0.3, 1.22, 1.8, 2.7, 3.25, 4.6):
+        k = 1.1 * 2.0 ** w / (64 * math.sqrt(2))
+        su = Z / (F * k)               # world units per UV unit
+        X = 2 * Z
+        corners = [(-X, -X), (X, -X), (X, X), (-X, X)]
+        t = np.zeros(2, dtype=abi.TRIANGLE)
+        for ti, idx in enumerate(((0, 1, 2), (0, 2, 3))):
+            for vi, ci in enumerate(idx):
+                x, y = corners[ci]
+                t["v"]["pos"][ti, vi, :3] = (x, y, Z)
+                t["v"]["normal"][ti, vi, :3] = (0.0, 0.0, -1.0)
+                t["v"]["vt"][ti, vi] = (x / su + 0.25, y / su + 0.25)
+        objs = scenes._desc((0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0), 1.0, flags=abi.FEATURE_TWO_SIDED)
+        sc = scenes.Scene(t, objs, scenes.make_light((0.0, 0.0, -100.0)), [], 64, "mip")
+        out = o.render(sc, view, atlas=atlas)
+        alpha = out["rgba"][4:H - 4, 4:W - 4, 3].astype(np.float64)
+        assert np.all(np.abs(alpha - _alpha_of(w)) <= 1.5), (w, alpha.min(), alpha.max(), _alpha_of(w))
+
+
+# ------------------------------------------------------------------- SSAO
+def _occ(count):
+    acc = np.float32(np.float32(count) / np.float32(125))
+    return float(np.float32(1) - np.float32(np.float32(1) - acc) / np.float32(2.5))
+
+
+def pin_ssao(o):
+    """With fov / depth = 1/2 the sample offsets are exactly -2..2 pixels
+    (world_rad = (2 + u/2) / 2 for u in [0, 1)); a test passes when the
+    sample is more than z = -2..2 units deeper."""
+    W, H, D = 40, 24, 9000.0
+    fov = 0.5 * float(np.float32(raw_depth(D)) / np.float32(U32) * np.float32(FAR))
+    flat = np.full((H, W), raw_depth(D), np.uint32)
+    ys = range(4, H - 4)
+    for y in ys:
+        for x in range(4, W - 4):
+            assert abs(o.ssao(flat, x, y, fov) - _occ(50)) < 1e-6, (x, y)
+    S = W // 2
+    # near plane at columns < S, 100 units further from column S on: seen
+    # from the near side, each far column adds 3 passing tests per row
+    step = flat.copy()
+    step[:, S:] = raw_depth(D + 100)
+    for y in ys:
+        for d in (1, 2, 3, 4):
+            x = S - d
+            want = _occ(50 + 15 * max(0, 3 - d))
+            assert abs(o.ssao(step, x, y, fov) - want) < 1e-6, (x, y, d)
+    # far plane at columns >= S, 100 units nearer before it: seen from the
+    # far side, each near column removes 2 passing tests per row
+    step2 = flat.copy()
+    step2[:, :S] = raw_depth(D - 100)
+    for y in ys:
+        for e in (1, 2, 3, 4):
+            x = S + e - 1
+            want = _occ(50 - 10 * max(0, 3 - e))
+            assert abs(o.ssao(step2, x, y, fov) - want) < 1e-6, (x, y, e)
+    # a ramp of 0.4 units per column: offsets -2..2 see -0.8..0.8, which
+    # pass 2, 2, 2, 3, 3 of the thresholds -> 12 per row
+    ramp = np.array([[raw_depth(D + 0.4 * (x - W // 2)) for x in range(W)] for _ in range(H)], np.uint32)
+    for y in ys:
+        for x in range(6, W - 6):
+            assert abs(o.ssao(ramp, x, y, fov) - _occ(60)) < 1e-6, (x, y)
+
+
+# --------------------------------------------------------------- wang_hash
+M32 = 0xFFFFFFFF
+
+
+def _unxorshift_right(y, s):
+    x = y
+    for _ in range(32 // s + 1):
+        x = y ^ (x >> s)
+    return x & M32
+
+
+def wang_inverse(h):
+    """The inverse of the published hash, step by step backwards."""
+    h = _unxorshift_right(h, 15)
+    h = (h * pow(0x27D4EB2D, -1, 1 << 32)) & M32
+    h = _unxorshift_right(h, 4)
+    h = (h * pow(9, -1, 1 << 32)) & M32
+    return _unxorshift_right(h ^ 61, 16)
+
+
+def pin_wang_hash(o):
+    rng = np.random.default_rng(5)
+    for x in list(rng.integers(0, 1 << 32, 20000, dtype=np.uint64)) + [0, 1, 61, M32, 1 << 31]:
+        x = int(x)
+        assert wang_inverse(o.lib.oracle_wang_hash(x)) == x, x
+    assert o.lib.oracle_wang_hash(61) == 0
+    # values of the published listing
+    assert o.lib.oracle_wang_hash(0) == 3232319850
+    assert o.lib.oracle_wang_hash(1) == 663891101
+    assert o.lib.oracle_wang_hash(123456789) == 2974685137
+
+
+PINS = [pin_face_orientation, pin_vertex_face_rule, pin_min_depth_brute_force, pin_hard_occlusion,
+        pin_texture_filter, pin_mip_selection_frame, pin_ssao, pin_wang_hash]
+
+
+@pytest.mark.parametrize("pin", PINS, ids=[p.__name__ for p in PINS])
+def test_pin(oracle, pin):
+    pin(oracle)
+
+
+def test_wang_inverse_is_an_inverse():
+    """The test's own inverse, checked on the forward steps written out."""
+    def fwd(s):
+        s = ((s ^ 61) ^ (s >> 16)) & M32
+        s = (s * 9) & M32
+        s ^= s >> 4
+        s = (s * 0x27D4EB2D) & M32
+        return s ^ (s >> 15)
+    rng = np.random.default_rng(9)
+    for x in rng.integers(0, 1 << 32, 2000, dtype=np.uint64):
+        assert wang_inverse(fwd(int(x))) == int(x)
+
+
+# Each planted mistake and the pins expected to catch it.
+PLANTS = {
+    1: "cube faces 1 and 3 swapped",
+    2: "MIP_BIAS inverted",
+    3: "SHADOWBIAS sign flipped",
+    4: "wang_hash multiplier changed",
+    5: "shadow pass LFOV = L",
+    6: "op_size_light 500",
+    7: "triangles drawn into all six faces",
+    8: "SSAO divisor 2.0",
+    9: "shadow taps one row down",
+    10: "mip blend weight inverted",
+    11: "ret_cubeface +x and -x swapped",
+    12: "shadow lookup LFOV = L",
+}
+
+
+def test_planted_mistakes(tmp_path):
+    """Every planted mistake fails at least one pin above."""
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    procs = {}
+    for k in PLANTS:
+        out = str(tmp_path / f"oracle_plant{k}.so")
+        procs[k] = (out, subprocess.Popen(["make", "-s", "-C", here, "plant", f"K={k}", f"OUT={out}"],
+                                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT))
+    caught = {}
+    for k, (out, p) in procs.items():
+        assert p.wait() == 0, p.stdout.read().decode()
+        o = Oracle(out)
+        failed = []
+        for pin in PINS:
+            try:
+                pin(o)
+            except AssertionError:
+                failed.append(pin.__name__)
+        caught[k] = failed
+    missed = [PLANTS[k] for k, f in caught.items() if not f]
+    assert not missed, f"planted mistakes no pin caught: {missed}"
+    print({PLANTS[k]: f for k, f in caught.items()})
