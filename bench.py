@@ -7,7 +7,12 @@ step is one frame with the scene, atlas and lights already resident in HBM.
 
 Prints one JSON line: frames/s (value), Mpix/s, per-pass device times from
 HIP events on the render stream, the shading kernel's roofline against
-MI355X's 8 TB/s HBM, and the oracle timed on this host's cores.
+MI355X's 8 TB/s HBM and against the VALU issue rate measured on the device,
+the raster passes' depth tests per second against the measured atomicMin
+rate, the oracle timed on this host's cores, and two bounded legs of the
+metric's second half: the batched form (64 views over 8 scenes, batched vs
+sequential) and the served form (Poisson streams at 1000 and 2000 req/s,
+p50/p99 latency, batched vs one request at a time).
 
 Other workloads (one JSON line each):
   --config 1  800x600, 2,480 triangles, 1 light
@@ -79,13 +84,6 @@ class EventPair:
         return t.value
 
 
-# VALU issue model: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per
-# 4 cycles per SIMD.  Some instruction mixes issue faster than this (the
-# unshadowed shading kernel measures ~1.05 of it), so it is a lower bound of
-# the ceiling and fractions above 1 are possible.
-VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
-
-
 def profile_entry(kernel, workload):
     """The committed rocprofv3 --pmc summary of `kernel` on `workload`: HBM
     bytes per launch (FETCH_SIZE, WRITE_SIZE scaled by the copy-kernel
@@ -97,6 +95,28 @@ def profile_entry(kernel, workload):
         return {}
 
 
+def calibration():
+    """Measured ceilings on this GPU (calib.hip): the VALU issue rate of
+    independent binary32 and binary64 FMA chains at 8 wavefronts per SIMD,
+    and the u32 atomicMin rate on a 1080p-sized buffer, uniform random and
+    64-address clusters (the depth pass's neighbourhoods)."""
+    cal = C.CDLL(os.path.join(ROOT, "openclrenderer_amd", "libocr_calib.so"))
+    cal.ocr_calib_valu.argtypes = [C.c_int, C.c_uint32]
+    cal.ocr_calib_valu.restype = C.c_double
+    cal.ocr_calib_atomics.argtypes = [C.c_int, C.c_uint32, C.c_size_t, C.c_uint32, C.c_int]
+    cal.ocr_calib_atomics.restype = C.c_double
+    f32 = cal.ocr_calib_valu(0, 4096)
+    f64 = cal.ocr_calib_valu(1, 4096)
+    n = 1 << 26
+    words = 1920 * 1080
+    us_rand = cal.ocr_calib_atomics(0, words, n, 1, 5)
+    us_clus = cal.ocr_calib_atomics(0, words, n, 64, 5)
+    if min(f32, f64, us_rand, us_clus) <= 0:
+        raise RuntimeError("calibration kernels failed")
+    return {"valu_f32_wave_insts_per_s": f32, "valu_f64_wave_insts_per_s": f64,
+            "atomic_min_per_s_uniform": n / (us_rand * 1e-6), "atomic_min_per_s_clustered": n / (us_clus * 1e-6)}
+
+
 def shade_algorithmic_bytes(W, H, V, O, x_tex, x_shadow, L):
     """Minimal field bytes of one shading launch: 16 B per pixel (depth 4,
     triangle id 4, RGBA8 4, normal 4; the next render's clear is k_clear's),
@@ -106,38 +126,109 @@ def shade_algorithmic_bytes(W, H, V, O, x_tex, x_shadow, L):
     return 16 * W * H + 132 * V + 52 * O + 4 * x_tex + 4 * x_shadow + 56 * L
 
 
+def host_cpu():
+    """CPU model, logical CPUs the process may use, and physical cores."""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            k, _, v = ln.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model == "unknown":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None and core is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None and core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        logical = len(os.sched_getaffinity(0))
+    except AttributeError:
+        logical = os.cpu_count() or 1
+    return model, logical, len(cores) or logical
+
+
+def timing_oracle_path():
+    """The oracle's fastest build this host can run: the same source and
+    arithmetic contract at -O3 for x86-64-v4 (AVX-512) or -v3 (AVX2 + FMA),
+    else the test build."""
+    flags = set()
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("flags"):
+                flags = set(ln.split(":", 1)[1].split())
+                break
+    except OSError:
+        pass
+    base = os.path.join(ROOT, "oracle", "_build")
+    v3 = {"avx2", "fma", "bmi1", "bmi2", "movbe", "f16c", "lzcnt" if "lzcnt" in flags else "abm"}
+    v4 = v3 | {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+    for name, need in (("libocr_oracle_v4.so", v4), ("libocr_oracle_v3.so", v3)):
+        path = os.path.join(base, name)
+        if need <= flags and os.path.exists(path):
+            return path, name
+    return os.path.join(base, "libocr_oracle.so"), "libocr_oracle.so"
+
+
 def cpu_baseline(sc, view, gpu_out, seconds, with_shadows=False):
-    """The oracle on this host's cores over a bounded sample of the same
-    workload: whole frames (with the shadow pass when a step has one)."""
+    """The oracle on this host over the same workload: whole frames (with the
+    shadow pass when a step has one), on the host cores this GPU's share
+    allows (OMP_NUM_THREADS, at most the physical cores) after 2 warm-ups,
+    median of 5; then the same on one thread (1 warm-up, median of 3).
+    Stops early when `seconds` of CPU time are used up."""
     from oracle.binding import Oracle
-    cores = min(os.cpu_count() or 1, 16)
-    o = Oracle()
+    model, logical, physical = host_cpu()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or physical
+    cores = max(1, min(share, physical, logical))
+    path, build = timing_oracle_path()
+    o = Oracle(path)
     atlas = o.atlas(sc.textures)
     has_shadow = bool(np.any(sc.lights["shadow"]))
     maps = None if with_shadows or not has_shadow else o.shadow_maps(sc, cores)
-    times = []
-    ref = None
     t_end = time.perf_counter() + seconds
-    while True:
-        t0 = time.perf_counter()
-        if with_shadows:
-            maps = o.shadow_maps(sc, cores)
-        r = o.render(sc, view, atlas=atlas, maps=maps, nthreads=cores)
-        times.append(time.perf_counter() - t0)
-        ref = ref or r
-        if time.perf_counter() > t_end or len(times) >= 8:
-            break
+
+    shadow_samples = None
+
+    def frames(nthreads, warm, timed):
+        nonlocal maps, shadow_samples
+        times, first = [], None
+        for k in range(warm + timed):
+            t0 = time.perf_counter()
+            if with_shadows:
+                dyn, stat, shadow_samples = o.shadow_maps(sc, nthreads, with_samples=True)
+                maps = (dyn, stat)
+            r = o.render(sc, view, atlas=atlas, maps=maps, nthreads=nthreads)
+            if k >= warm:
+                times.append(time.perf_counter() - t0)
+            first = first or r
+            if times and time.perf_counter() > t_end:
+                break
+        return times, first
+
+    times, ref = frames(cores, 2, 5)
+    times1, _ = frames(1, 1, 3) if time.perf_counter() < t_end else ([], None)
     parity = None
     if gpu_out is not None:
         parity = bool(np.array_equal(gpu_out["depth"], ref["depth"]) and
                       np.array_equal(gpu_out["tri_id"], ref["tri_id"]) and
                       int(np.abs(gpu_out["rgba"].astype(int) - ref["rgba"].astype(int)).max()) <= 1)
     med = float(np.median(times))
+    med1 = float(np.median(times1)) if times1 else None
     what = "shadow pass + frame" if with_shadows else "frames"
     return {"value": 1.0 / med, "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": f"{len(times)} full {view['width'][0]}x{view['height'][0]} {what} of the same scene, "
-                      f"oracle on {cores} OpenMP threads, median {med * 1e3:.0f} ms/frame",
-            "ms_per_frame": med * 1e3}, parity
+            "sample": f"{view['width'][0]}x{view['height'][0]} {what} of the same scene: {len(times)} timed after "
+                      f"2 warm-ups on {cores} OpenMP threads (median {med * 1e3:.0f} ms), {len(times1)} on 1 "
+                      f"thread" + (f" (median {med1 * 1e3:.0f} ms)" if med1 else ""),
+            "ms_per_frame": med * 1e3, "single_thread_ms_per_frame": med1 * 1e3 if med1 else None,
+            "single_thread_value": 1.0 / med1 if med1 else None,
+            "host": {"cpu_model": model, "nproc": logical, "physical_cores": physical,
+                     "oracle_build": build}}, parity, dict(ref["stats"], shadow_samples=shadow_samples)
 
 
 def animated_descriptors(objs, n, seed=5):
@@ -310,12 +401,15 @@ def run_plain(args):
     prof = profile_entry(shade_kernel, workload)
     traffic = prof.get("bytes_per_launch")
     achieved_iso = B / (st_iso["ms_shade"] * 1e-3) / 1e9
+    cal = calibration()
     valu = None
     if "valu_insts_per_launch" in prof:
         n = float(prof["valu_insts_per_launch"])
-        valu = {"insts_per_launch": n, "peak_insts_per_s": VALU_PEAK_WAVE_INSTS,
-                "frac": n / (st["ms_shade"] * 1e-3) / VALU_PEAK_WAVE_INSTS,
-                "frac_isolated": n / (st_iso["ms_shade"] * 1e-3) / VALU_PEAK_WAVE_INSTS}
+        peak = cal["valu_f32_wave_insts_per_s"]
+        valu = {"insts_per_launch": n, "peak_insts_per_s": peak,
+                "peak_source": "measured: independent binary32 FMA chains, 8 wavefronts per SIMD (calib.hip)",
+                "frac": n / (st["ms_shade"] * 1e-3) / peak,
+                "frac_isolated": n / (st_iso["ms_shade"] * 1e-3) / peak}
 
     # the legs below time whole calls; per-pass timing events stay off
     fr.set_profiling(False)
@@ -331,7 +425,29 @@ def run_plain(args):
         if cal.ocr_calib_copy(256 << 20, 3) != 0:
             raise RuntimeError("calibration copy failed")
 
-    cpu, parity = (None, None) if args.no_cpu else cpu_baseline(sc, view, out, args.cpu_seconds, shadows)
+    cpu, parity, ref_stats = (None, None, None) if args.no_cpu else \
+        cpu_baseline(sc, view, out, args.cpu_seconds, shadows)
+    # depth tests per second of the raster passes (one per covered sample:
+    # the reference's atomic_min count; the oracle counts them) against the
+    # measured atomicMin rate
+    atomics = None
+    if ref_stats is not None:
+        S = int(ref_stats["n_samples"])
+        peak = cal["atomic_min_per_s_clustered"]
+        atomics = {"covered_samples": S, "peak_atomic_min_per_s": peak,
+                   "peak_source": "measured: u32 atomicMin over a 1080p-sized buffer in 64-address clusters "
+                                  f"(uniform random: {cal['atomic_min_per_s_uniform']:.3g}/s)"}
+        for name, key in (("depth", "ms_depth"), ("id", "ms_id")):
+            rate = S / (st_iso[key] * 1e-3)
+            atomics[name] = {"ms_isolated": st_iso[key], "tests_per_s": rate, "frac": rate / peak}
+        if shadows and ms_shadow > 0 and ref_stats.get("shadow_samples"):
+            rate = ref_stats["shadow_samples"] / (ms_shadow * 1e-3)
+            atomics["shadow"] = {"covered_samples": int(ref_stats["shadow_samples"]), "ms": ms_shadow,
+                                 "tests_per_s": rate, "frac": rate / peak}
+    served = batched = None
+    if extras:
+        batched = batched_leg(ctx, steps=10, warmup=2, autotune_reps=2)
+        served = served_leg(ctx, n_req=400, rates=(1000, 2000), modes=("8x4", "1x1"))
     passes = {k: st[k] for k in ("ms_clear", "ms_prearrange", "ms_depth", "ms_id", "ms_shade", "ms_total")}
     if shadows:
         passes["ms_shadow"] = ms_shadow
@@ -349,10 +465,15 @@ def run_plain(args):
         "roofline": {"bound": "hbm", "kernel": shade_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": B, "valu_issue": valu,
+                     "limiter": "valu" if valu and valu["frac"] > achieved / HBM_PEAK_GBS else "hbm",
                      "isolated": {"ms": st_iso["ms_shade"], "achieved": achieved_iso,
                                   "frac": achieved_iso / HBM_PEAK_GBS}},
+        "atomics": atomics,
+        "calibration": cal,
         "post_blend_with_depth": post,
         "animated": animated,
+        "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
     })
@@ -370,10 +491,11 @@ def config4_workload():
     return scs, views
 
 
-def run_batched(args):
+def batched_leg(ctx, steps, warmup, autotune_reps=3):
+    """Config 4: the 64 views rendered as one autotuned, graph-replayed batch
+    per step against the same views as 64 plain render sequences."""
     from openclrenderer_amd import runtime
     scs, views = config4_workload()
-    ctx = runtime.Context(0)
     gs = [runtime.GpuScene(ctx, s) for s in scs]
     frames = [runtime.Frame(ctx, int(v["width"][0]), int(v["height"][0])) for _, v in views]
     owners = [gs[i] for i, _ in views]
@@ -381,23 +503,23 @@ def run_batched(args):
     b = runtime.Batch(ctx, len(views))
     b.set_views(owners, vlist, frames)
     b.render_checked()
-    tuning = b.autotune(reps=3)
+    tuning = b.autotune(reps=autotune_reps)
     b.render_checked(graph=True)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         b.launch_graph()
     hip_sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         b.launch_graph()
     hip_sync()
-    t_batch = (time.perf_counter() - t0) / args.steps
+    t_batch = (time.perf_counter() - t0) / steps
     b.stats()
     # the same 64 views rendered one plain launch sequence at a time
     plain = [runtime.Frame(ctx, f.width, f.height) for f in frames]
     for f, g, v in zip(plain, owners, vlist):
         f.render_read(g, v)
     hip_sync()
-    seq_steps = max(1, args.steps // 4)
+    seq_steps = max(1, steps // 2)
     t0 = time.perf_counter()
     for _ in range(seq_steps):
         for f, g, v in zip(plain, owners, vlist):
@@ -406,15 +528,32 @@ def run_batched(args):
     t_seq = (time.perf_counter() - t0) / seq_steps
     same = all(np.array_equal(a.read()["rgba"], p.read()["rgba"]) for a, p in zip(frames, plain))
     npix = sum(f.width * f.height for f in frames)
+    out = {"views_per_s": len(views) / t_batch, "ms_per_batch": t_batch * 1e3, "mpix_per_s": npix / t_batch / 1e6,
+           "sequential_views_per_s": len(views) / t_seq, "sequential_ms": t_seq * 1e3,
+           "speedup_vs_sequential": t_seq / t_batch,
+           "tuning": {"raster_blocks_per_cu": tuning[0], "shade_tile_w_log2": tuning[1], "groups": tuning[2]},
+           "batched_equals_sequential": bool(same), "views": len(views), "scenes": len(scs),
+           "triangles": [s.ntri for s in scs],
+           "workload": "64 views (320x240 .. 2560x1440) over 8 scenes of 50k-2M tris, 4 lights each"}
+    b.close()
+    for f in frames + plain:
+        f.close()
+    for g in gs:
+        g.close()
+    return out
+
+
+def run_batched(args):
+    from openclrenderer_amd import runtime
+    ctx = runtime.Context(0)
+    r = batched_leg(ctx, args.steps, args.warmup)
     line = base_line(args, "views/s, 64 mixed-size views over 8 resident scenes (batched vs sequential plain)",
-                     len(views) / t_batch, "views/s", t_batch * 1e3,
-                     {"workload": "64 views (320x240 .. 2560x1440) over 8 scenes of 50k-2M tris, 4 lights each",
-                      "views": len(views), "scenes": len(scs), "triangles": [s.ntri for s in scs],
-                      "path": "batched (hipGraph, autotuned)"})
-    line.update({"mpix_per_s": npix / t_batch / 1e6, "sequential_views_per_s": len(views) / t_seq,
-                 "sequential_ms_per_step": t_seq * 1e3, "speedup_vs_sequential": t_seq / t_batch,
-                 "tuning": {"raster_blocks_per_cu": tuning[0], "shade_tile_w_log2": tuning[1], "groups": tuning[2]},
-                 "batched_equals_sequential": bool(same)})
+                     r["views_per_s"], "views/s", r["ms_per_batch"],
+                     {"workload": r["workload"], "views": r["views"], "scenes": r["scenes"],
+                      "triangles": r["triangles"], "path": "batched (hipGraph, autotuned)"})
+    line.update({k: r[k] for k in ("mpix_per_s", "sequential_views_per_s", "speedup_vs_sequential", "tuning",
+                                   "batched_equals_sequential")})
+    line["sequential_ms_per_step"] = r["sequential_ms"]
     print(json.dumps(line), flush=True)
 
 
@@ -430,15 +569,15 @@ def config5_workload(seed=None):
     return scs, rng
 
 
-def run_served(args):
+def served_leg(ctx, n_req, rates, modes):
+    """Config 5: Poisson arrivals of n_req requests per rate against 12
+    resident scenes through a server in each mode ("VxF": at most V views per
+    batch, F batches in flight; "1x1" renders one request at a time)."""
     from openclrenderer_amd import runtime, scenes
     scs, rng = config5_workload()
-    ctx = runtime.Context(0)
     gs = [runtime.GpuScene(ctx, s) for s in scs]
     res = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
     cum = np.cumsum([0.4, 0.3, 0.2, 0.1])
-    rates = [int(r) for r in args.rates.split(",")]
-    n_req = args.requests
 
     def schedule(rate):
         t, out = 0.0, []
@@ -453,11 +592,7 @@ def run_served(args):
     def drive(srv, slots, sched):
         lat, qwait, batch = [], [], []
         t_base = srv.now_ns()
-        pending = 0
-        refused = 0
-        failed = 0
-        i = 0
-        done = 0
+        refused = failed = i = done = 0
         while done < len(sched):
             now = (srv.now_ns() - t_base) * 1e-9
             while i < len(sched) and sched[i][0] <= now:
@@ -465,7 +600,6 @@ def run_served(args):
                 if rid is None:
                     refused += 1
                     break                      # retried after completions are released
-                pending += 1
                 i += 1
             for c in srv.poll(max_n=512, wait_ms=0):
                 lat.append((c.t_done_ns - c.t_submit_ns) * 1e-6)
@@ -485,36 +619,51 @@ def run_served(args):
                 "queue_p50_ms": float(np.percentile(qwait, 50)), "mean_batch": float(np.mean(batch)),
                 "refused_submits": refused, "failed": failed}
 
-    modes = {}
-    for m in args.modes.split(","):
-        v, f = (int(x) for x in m.split("x"))
-        modes["one_at_a_time" if (v, f) == (1, 1) else f"batched_{v}x{f}"] = dict(max_batch_views=v, max_inflight=f)
     results = {}
-    for name, kw in modes.items():
-        with runtime.Server(ctx, max_batch_tris=24_000_000, hbm_fraction=0.25, read_mask=runtime.READ_RGBA,
-                            **kw) as srv:
+    for m in modes:
+        v, f = (int(x) for x in m.split("x"))
+        name = "one_at_a_time" if (v, f) == (1, 1) else f"batched_{v}x{f}"
+        with runtime.Server(ctx, max_batch_views=v, max_inflight=f, max_batch_tris=24_000_000, hbm_fraction=0.25,
+                            read_mask=runtime.READ_RGBA) as srv:
             slots = [srv.add_scene(g) for g in gs]
             warm = schedule(2000)[:64]
-            drive(srv, slots, [(0.0, k, v) for _, k, v in warm])
+            drive(srv, slots, [(0.0, k, vw) for _, k, vw in warm])
             results[name] = {}
             for rate in rates:
                 r = drive(srv, slots, schedule(rate))
                 r["splits_so_far"] = srv.info()["n_splits"]
                 results[name][str(rate)] = r
                 print(json.dumps({"progress": name, "rate": rate, **r}), file=sys.stderr, flush=True)
+    for g in gs:
+        g.close()
     top = str(max(rates))
     batched = [k for k in results if k != "one_at_a_time"]
     best_mode = min(batched, key=lambda k: results[k][top]["p99_ms"]) if batched else "one_at_a_time"
-    best = results[best_mode][top]
+    out = {"workload": "12 resident scenes (50k-4M tris, 1-16 shadowed lights), resolutions "
+                       "640x480/1280x720/1920x1080/3840x2160 at .4/.3/.2/.1, Poisson arrivals",
+           "requests_per_rate": n_req, "rates": list(rates), "scenes_tris": [s.ntri for s in scs],
+           "modes": results, "best_mode": best_mode, "best": results[best_mode][top],
+           "p50_ms_at_top_rate": results[best_mode][top]["p50_ms"],
+           "p99_ms_at_top_rate": results[best_mode][top]["p99_ms"]}
+    if "one_at_a_time" in results:
+        alone = results["one_at_a_time"][top]
+        out["one_at_a_time_p99_ms_at_top_rate"] = alone["p99_ms"]
+        out["throughput_vs_one_at_a_time"] = results[best_mode][top]["completed_per_s"] / alone["completed_per_s"]
+    return out
+
+
+def run_served(args):
+    from openclrenderer_amd import runtime
+    ctx = runtime.Context(0)
+    rates = [int(r) for r in args.rates.split(",")]
+    r = served_leg(ctx, args.requests, rates, args.modes.split(","))
+    best = r["best"]
     line = base_line(args, "served requests/s with p50/p99 latency (Poisson arrivals, mixed resolutions)",
                      best["completed_per_s"], "requests/s", 1e3 / best["completed_per_s"],
-                     {"workload": "12 resident scenes (50k-4M tris, 1-16 shadowed lights), resolutions "
-                                  "640x480/1280x720/1920x1080/3840x2160 at .4/.3/.2/.1",
-                      "requests_per_rate": n_req, "rates": rates, "scenes_tris": [s.ntri for s in scs],
-                      "path": "served (continuous batching)"})
-    line.update({"served": results, "best_mode": best_mode, "p99_ms_at_top_rate": best["p99_ms"],
-                 "one_at_a_time_p99_ms_at_top_rate":
-                     results["one_at_a_time"][top]["p99_ms"] if "one_at_a_time" in results else None})
+                     {"workload": r["workload"], "requests_per_rate": r["requests_per_rate"], "rates": rates,
+                      "scenes_tris": r["scenes_tris"], "path": "served (continuous batching)"})
+    line.update({"served": r["modes"], "best_mode": r["best_mode"], "p99_ms_at_top_rate": r["p99_ms_at_top_rate"],
+                 "one_at_a_time_p99_ms_at_top_rate": r.get("one_at_a_time_p99_ms_at_top_rate")})
     print(json.dumps(line), flush=True)
 
 

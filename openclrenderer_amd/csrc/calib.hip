@@ -120,10 +120,90 @@ extern "C" double ocr_calib_atomics(int mode, uint32_t words, size_t n, uint32_t
   return e == hipSuccess ? (double)ms * 1e3 / iters : -3.0;
 }
 
+// VALU issue ceiling: every work item runs `iters` rounds of 8 independent
+// FMA chains (mode 0: binary32 v_fma_f32, mode 1: binary64 v_fma_f64), so
+// each round issues 8 VALU instructions per wavefront with no dependency
+// stall; the grid holds 8 wavefronts per SIMD.  The sums are stored so the
+// chains stay live.
+template <typename T>
+__global__ void __launch_bounds__(256) k_calib_valu(T* __restrict__ out, uint32_t iters, T a, T b) {
+#pragma clang fp contract(off)
+  T x[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) x[k] = (T)(threadIdx.x + k);
+  for (uint32_t i = 0; i < iters; i++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if constexpr (sizeof(T) == 4) x[k] = __builtin_fmaf(x[k], a, b);
+      else x[k] = __builtin_fma(x[k], a, b);
+    }
+  }
+  T s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) s += x[k];
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// Returns the measured VALU rate in wave-instructions per second (the FMAs
+// only; loop control is scalar), or a negative value on failure.
+extern "C" double ocr_calib_valu(int mode, uint32_t iters) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1.0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1.0;
+  const uint32_t blocks = (uint32_t)cus * 4u * 8u / 4u;   // 8 wavefronts per SIMD, 4 per block
+  void* buf = nullptr;
+  if (hipMalloc(&buf, (size_t)blocks * 256 * 8) != hipSuccess) return -1.0;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  for (int w = 0; w < 3; w++) {
+    if (w == 2) (void)hipEventRecord(e0, nullptr);
+    if (mode == 0)
+      hipLaunchKernelGGL(k_calib_valu<float>, dim3(blocks), dim3(256), 0, nullptr, (float*)buf, iters, 0.999f, 0.001f);
+    else
+      hipLaunchKernelGGL(k_calib_valu<double>, dim3(blocks), dim3(256), 0, nullptr, (double*)buf, iters, 0.999, 0.001);
+  }
+  (void)hipEventRecord(e1, nullptr);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(buf);
+  if (e != hipSuccess || ms <= 0.f) return -3.0;
+  const double wave_insts = (double)blocks * 4.0 * (double)iters * 8.0;
+  return wave_insts / ((double)ms * 1e-3);
+}
+
+// Evaluates one of the device's arithmetic-contract functions over n inputs
+// (fn 0: exp_f), for comparison with the oracle's restatement.
+__global__ void __launch_bounds__(256) k_calib_eval(int fn, const float* __restrict__ x, float* __restrict__ y,
+                                                    uint32_t n) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] = fn == 0 ? ocr::exp_f(x[i]) : x[i];
+}
+
+extern "C" int ocr_calib_eval(int fn, const float* x, float* y, uint32_t n) {
+  float *dx = nullptr, *dy = nullptr;
+  if (hipMalloc(&dx, (size_t)n * 4) != hipSuccess) return -1;
+  if (hipMalloc(&dy, (size_t)n * 4) != hipSuccess) { (void)hipFree(dx); return -1; }
+  hipError_t e = hipMemcpy(dx, x, (size_t)n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_calib_eval, dim3((n + 255) / 256), dim3(256), 0, nullptr, fn, dx, dy, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(y, dy, (size_t)n * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return e == hipSuccess ? 0 : -3;
+}
+
 // Exhaustive checks of the short reciprocal sequences against IEEE
 // arithmetic, over every binary32 bit pattern in [lo, hi] (NaN inputs must
 // give NaN): mode 0 ocr::rcp_rn(b) vs 1.0f / b, mode 1 ocr::inv_sqrt_cr(d) vs
-// 1.0f / sqrtf(d).  One count per work item, added once, so a bad helper
+// 1.0f / sqrtf(d), mode 2 ocr::sqrt_cr(x) vs sqrtf(x).  One count per work item, added once, so a bad helper
 // cannot turn the check into billions of same-address atomics.
 __global__ void __launch_bounds__(256) k_calib_rcp_check(int mode, uint32_t lo, uint32_t hi,
                                                          unsigned long long* __restrict__ bad,
@@ -135,8 +215,8 @@ __global__ void __launch_bounds__(256) k_calib_rcp_check(int mode, uint32_t lo, 
   uint32_t first = 0xFFFFFFFFu;
   for (uint64_t u = (uint64_t)lo + i; u <= (uint64_t)hi; u += stride) {
     const float b = __uint_as_float((uint32_t)u);
-    const float fast = mode == 0 ? ocr::rcp_rn(b) : ocr::inv_sqrt_cr(b);
-    const float ref = mode == 0 ? 1.0f / b : 1.0f / sqrtf(b);
+    const float fast = mode == 0 ? ocr::rcp_rn(b) : mode == 1 ? ocr::inv_sqrt_cr(b) : ocr::sqrt_cr(b);
+    const float ref = mode == 0 ? 1.0f / b : mode == 1 ? 1.0f / sqrtf(b) : sqrtf(b);
     const bool same = __float_as_uint(fast) == __float_as_uint(ref) || (ref != ref && fast != fast);
     if (!same) {
       n++;
@@ -147,6 +227,64 @@ __global__ void __launch_bounds__(256) k_calib_rcp_check(int mode, uint32_t lo, 
     atomicAdd(bad, n);
     atomicMin(first_bad, first);
   }
+}
+
+// Candidate square-root sequences, compared bit for bit with the correctly
+// rounded sqrtf over every binary32 pattern in [lo, hi]; mismatches are
+// counted per input exponent (256 buckets).
+//   mode 0: the hardware estimate, then the residual test on its two
+//           neighbours (no scaling, no special-value handling);
+//   mode 1: the hardware estimate corrected once by the residual times half
+//           the hardware reciprocal square root.
+__device__ __forceinline__ float sqrt_two_sided(float x) {
+  float s = __builtin_amdgcn_sqrtf(x);
+  const float sd = __int_as_float(__float_as_int(s) - 1);
+  const float su = __int_as_float(__float_as_int(s) + 1);
+  const float rd = fmaf(-sd, s, x);
+  const float ru = fmaf(-su, s, x);
+  s = rd <= 0.f ? sd : s;
+  s = ru > 0.f ? su : s;
+  return s;
+}
+__device__ __forceinline__ float sqrt_rsq_step(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float h = 0.5f * __builtin_amdgcn_rsqf(x);
+  const float r = fmaf(-s, s, x);
+  return fmaf(r, h, s);
+}
+
+__global__ void __launch_bounds__(256) k_calib_sqrt_check(int mode, uint32_t lo, uint32_t hi,
+                                                          unsigned long long* __restrict__ bad_by_exp) {
+#pragma clang fp contract(off)
+  __shared__ unsigned int local[256];
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) local[k] = 0;
+  __syncthreads();
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t u = (uint64_t)lo + i; u <= (uint64_t)hi; u += stride) {
+    const float x = __uint_as_float((uint32_t)u);
+    const float fast = mode == 0 ? sqrt_two_sided(x) : sqrt_rsq_step(x);
+    const float ref = sqrtf(x);
+    const bool same = __float_as_uint(fast) == __float_as_uint(ref) || (ref != ref && fast != fast);
+    if (!same) atomicAdd(&local[((uint32_t)u >> 23) & 255], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < 256; k += blockDim.x)
+    if (local[k]) atomicAdd(&bad_by_exp[k], (unsigned long long)local[k]);
+}
+
+extern "C" int ocr_calib_sqrt_check(int mode, uint32_t lo, uint32_t hi, unsigned long long* bad_by_exp) {
+  unsigned long long* d_bad = nullptr;
+  if (hipMalloc(&d_bad, 256 * sizeof(*d_bad)) != hipSuccess) return -1;
+  hipError_t e = hipMemset(d_bad, 0, 256 * sizeof(*d_bad));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_calib_sqrt_check, dim3(8192), dim3(256), 0, nullptr, mode, lo, hi, d_bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(bad_by_exp, d_bad, 256 * sizeof(*d_bad), hipMemcpyDeviceToHost);
+  (void)hipFree(d_bad);
+  return e == hipSuccess ? 0 : -3;
 }
 
 extern "C" int ocr_calib_rcp_check(int mode, uint32_t lo, uint32_t hi, unsigned long long* mismatches,

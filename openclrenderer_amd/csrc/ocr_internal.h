@@ -33,7 +33,13 @@ struct TriAttr {
 };
 static_assert(sizeof(TriAttr) == 72, "TriAttr layout");
 
-// The obj_g_descriptor fields the path reads (52 bytes), padded to 64.
+// The obj_g_descriptor fields the path reads (52 bytes), and what kernel3
+// derives from them identically for every pixel of the object, formed once
+// per descriptor upload with the same operations (pack_obj):
+//   qn  = q * (1 / sqrt(q.q))                 rot_quat's normalised quaternion
+//   qb  = n(c / (c.c)), c = conj(q)           back_rot_quat's, normalised by rot_quat
+//   rough2 = r * r, r = clamp(1 - specular, 0.001, 10), and RN(1 / rough2)
+// (cl2.cl:350-370, :6232-6240).  96 bytes.
 struct alignas(16) ObjDesc {
   float wp[3];
   float scale;
@@ -43,9 +49,13 @@ struct alignas(16) ObjDesc {
   float spec_mult;
   float diffuse;
   int32_t flags;
-  uint32_t _pad[3];
+  float rough2;
+  float inv_rough2;
+  uint32_t _pad;
+  float qn[4];
+  float qb[4];
 };
-static_assert(sizeof(ObjDesc) == 64, "ObjDesc layout");
+static_assert(sizeof(ObjDesc) == 96, "ObjDesc layout");
 
 // A clipped, culled, projected triangle: screen x, y and camera z of its
 // three vertices, its 1/det constant, its sort key (2*tri + sub for a camera
@@ -89,7 +99,7 @@ struct ViewParams {
   float fov;
   uint32_t linear;
   float cpos[3];
-  float _pad0;
+  float ifov;               // RN(1 / fov); views keep fov in [2^-60, 2^60]
   SinCos cam;
   float clear[4];
 };

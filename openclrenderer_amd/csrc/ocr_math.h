@@ -11,8 +11,8 @@
 //     divide-sqrt);
 //   * fast_normalize(v) = v * (1/sqrt(dot(v,v))), dot summed left to right;
 //   * pow(x,2) = x*x; pow(x,4) and powr(x,5) are built in binary64 from the
-//     exact square and rounded once; exp and tan are evaluated in binary64
-//     and rounded once;
+//     exact square and rounded once; tan is evaluated in binary64 and
+//     rounded once; exp by a fixed binary32 sequence (exp_f);
 //   * float -> u32 conversions saturate, NaN -> 0.
 #pragma once
 #pragma clang fp contract(off)
@@ -100,13 +100,54 @@ constexpr float kInv65535 = 1.0f / 65535.0f;
 // normal, which keeps every quotient the path forms in the normal range.
 __device__ __forceinline__ bool rcp_safe(float d) { return fabsf(d) >= 0x1p-60f && fabsf(d) <= 0x1p60f; }
 
-// 1 / sqrt(d), both correctly rounded, for d >= 0: a nonzero d has a normal
-// square root (>= 2^-75), where rcp_rn applies; at +0 and +inf the hardware
-// estimate is the exact +inf / +0.  Equal to 1.0f / sqrtf(d) for every
-// binary32 d >= 0 and NaN: checked on the device for each (calib.hip).
-__device__ __forceinline__ float inv_sqrt_cr(float d) { return rcp_cr(sqrtf(d)); }
+// The correctly rounded square root.  For x in [2^-100, FLT_MAX] the
+// hardware estimate corrected once, s + (x - s*s) * rsq(x) / 2 (five
+// instructions), is the correctly rounded root: checked on the device for
+// every binary32 in that range (ocr_calib_rcp_check, calib.hip), which also
+// finds it wrong below (residuals there underflow).  Other inputs (zero,
+// tiny, infinite, negative, NaN) take the full sequence in a branch that a
+// wavefront skips when none of its lanes needs it.
+__device__ __forceinline__ bool sqrt_fast_ok(float x) {
+  // x in [2^-100, FLT_MAX]: the bit pattern of a non-negative finite x is
+  // its order, negative x and NaN are above every finite positive pattern
+  return (uint32_t)__float_as_uint(x) - 0x0D800000u <= 0x7F7FFFFFu - 0x0D800000u;
+}
+__device__ __forceinline__ float sqrt_core(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float h = 0.5f * __builtin_amdgcn_rsqf(x);
+  return fmaf(fmaf(-s, s, x), h, s);
+}
+__device__ __forceinline__ float sqrt_cr(float x) {
+  float s = sqrt_core(x);
+  if (!sqrt_fast_ok(x)) s = sqrtf(x);
+  return s;
+}
+// 1 / sqrt(d), both correctly rounded (the reference's fast_normalize and
+// fast_length, under the arithmetic contract).  On the fast path the root
+// lies in [2^-50, 2^64], where rcp_rn is the correctly rounded reciprocal;
+// other inputs divide.  Equal to 1.0f / sqrtf(d) for every binary32 d >= 0
+// and NaN: checked on the device for each (calib.hip).
+__device__ __forceinline__ float inv_sqrt_cr(float d) {
+  float r = rcp_rn(sqrt_core(d));
+  if (!sqrt_fast_ok(d)) r = 1.0f / sqrtf(d);
+  return r;
+}
+// sqrt_cr(d) and inv_sqrt_cr(d) together, from one root.
+__device__ __forceinline__ void length_and_inverse(float d, float& len, float& inv) {
+  len = sqrt_core(d);
+  inv = rcp_rn(len);
+  if (!sqrt_fast_ok(d)) {
+    len = sqrtf(d);
+    inv = 1.0f / len;
+  }
+}
+// A dividend whose quotients by divisors in [1, 2^60] the short division
+// forms correctly rounded: +0, or a normal magnitude in [2^-60, 2^60].
+__device__ __forceinline__ bool bright_ok(float b) {
+  return __float_as_uint(b) == 0u || (fabsf(b) >= 0x1p-60f && fabsf(b) <= 0x1p60f);
+}
 __device__ __forceinline__ f3 fast_normalize(f3 v) { return v * inv_sqrt_cr(dot(v, v)); }
-__device__ __forceinline__ float fast_length(f3 v) { return sqrtf(dot(v, v)); }
+__device__ __forceinline__ float fast_length(f3 v) { return sqrt_cr(dot(v, v)); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(fminf(a, b), c); }
 __device__ __forceinline__ float max3f(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
@@ -126,35 +167,6 @@ __device__ __forceinline__ uint32_t sat_u32(float f) {
 __device__ __forceinline__ uint32_t depth_quant(float fd) {
   const uint32_t d = sat_u32(kMulint * rcp_rn(fd));
   return (fd >= 0.f && fd < 0x1p-126f && !signbit(fd)) ? 0xFFFFFFFFu : d;
-}
-
-// exp in binary64: Cody-Waite reduction by ln2 + degree-13 Taylor polynomial.
-__device__ __forceinline__ double exp_d(double x) {
-  if (x != x) return x;
-  if (x > 100.0) return __longlong_as_double(0x7FF0000000000000ll);
-  if (x < -120.0) return 0.0;
-  const double kInvLn2 = 1.44269504088896338700e+00;
-  const double kLn2Hi = 6.93147180369123816490e-01;
-  const double kLn2Lo = 1.90821492927058770002e-10;
-  double k = floor(fma(x, kInvLn2, 0.5));
-  double r = fma(-k, kLn2Lo, fma(-k, kLn2Hi, x));
-  double p = 1.0 / 6227020800.0;
-  p = fma(p, r, 1.0 / 479001600.0);
-  p = fma(p, r, 1.0 / 39916800.0);
-  p = fma(p, r, 1.0 / 3628800.0);
-  p = fma(p, r, 1.0 / 362880.0);
-  p = fma(p, r, 1.0 / 40320.0);
-  p = fma(p, r, 1.0 / 5040.0);
-  p = fma(p, r, 1.0 / 720.0);
-  p = fma(p, r, 1.0 / 120.0);
-  p = fma(p, r, 1.0 / 24.0);
-  p = fma(p, r, 1.0 / 6.0);
-  p = fma(p, r, 0.5);
-  p = fma(p, r, 1.0);
-  p = fma(p, r, 1.0);
-  long long ki = (long long)k;
-  double scale = __longlong_as_double((ki + 1023) << 52);
-  return p * scale;
 }
 
 // tan in binary64: reduction by pi/2 + Taylor sin (deg 19) / cos (deg 18).
@@ -189,7 +201,26 @@ __device__ __forceinline__ double tan_d(double x) {
   return (ki & 1) ? -c / s : s / c;
 }
 
-__device__ __forceinline__ float exp_f(float x) { return (float)exp_d((double)x); }
+// exp in binary32: k = rint(x log2 e), r = x - k ln2 (two-constant
+// Cody-Waite: k * ln2_hi is exact for |k| < 2^11), e^r = 1 + r p(r) with a
+// degree-6 polynomial fitted to e^r on [-ln2/2, ln2/2] (relative error
+// 3.4e-9), scaled by ldexp.  Within one ulp of the correctly rounded exp
+// over [-104, 88.75] (oracle pins); the same operations as the oracle's.
+__device__ __forceinline__ float exp_f(float x) {
+  if (!(x >= -104.0f)) return x != x ? x : 0.0f;
+  if (x > 88.75f) return __builtin_inff();
+  const float k = rintf(x * 1.44269502f);
+  float r = fmaf(-k, 0.693145752f, x);
+  r = fmaf(-k, 1.42860677e-06f, r);
+  float p = 0.001399628003127873f;
+  p = fmaf(p, r, 0.008374124765396118f);
+  p = fmaf(p, r, 0.04166550189256668f);
+  p = fmaf(p, r, 0.16666433215141296f);
+  p = fmaf(p, r, 0.5000000596046448f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  return ldexpf(p, (int)k);
+}
 __device__ __forceinline__ float tan_f(float x) { return (float)tan_d((double)x); }
 __device__ __forceinline__ float powr5(float x) { double d = x; double d2 = d * d; return (float)((d2 * d2) * d); }
 __device__ __forceinline__ float pow4(float x) { double d = x; double d2 = d * d; return (float)(d2 * d2); }
@@ -231,6 +262,15 @@ __device__ __forceinline__ f3 rot_quat(f3 p, f4 q) {
   float qw = q.w * inv;
   f3 t = cross(qv, p) * 2.f;
   return (p + t * qw) + cross(qv, t);
+}
+
+// rot_quat's rotation by a quaternion it has already normalised (qv, qw as
+// rot_quat forms them): the same operations, without forming the
+// normalisation again.
+__device__ __forceinline__ f3 rot_quat_n(f3 p, f4 qn) {
+  const f3 qv = mk3(qn.x, qn.y, qn.z);
+  f3 t = cross(qv, p) * 2.f;
+  return (p + t * qn.w) + cross(qv, t);
 }
 
 // back_rot_quat, cl2.cl:359-370.
@@ -361,6 +401,22 @@ __device__ __forceinline__ float rational_acos(float x) {
   const float d = 0.295624144969963174f;
   return kPi / 2.f + (a * x + b * x * x * x) / (1.f + c * x * x + d * pow4(x));
 }
+// The same for x in [0, 1] (kernel3's n.h), with the quotient from the
+// correctly rounded reciprocal of the denominator.  There the denominator
+// lies in [0.0110, 1]; the numerator x (a + b x^2) is +0 at x = 0 (the
+// quotient's +0 then equals IEEE's), has magnitude >= 0.017 x elsewhere, so
+// for x >= 2^-30 it is >= 2^-36 and div_rcp is the correctly rounded
+// quotient, and for 0 < x < 2^-30 both quotients are below 2^-29, which
+// pi/2 absorbs (half an ulp of pi/2 is 2^-24): every x gives the bits of
+// rational_acos(x).
+__device__ __forceinline__ float rational_acos_unit(float x) {
+  const float a = -0.939115566365855f;
+  const float b = 0.9217841528914573f;
+  const float c = -1.2845906244690837f;
+  const float d = 0.295624144969963174f;
+  const float den = 1.f + c * x * x + d * pow4(x);
+  return kPi / 2.f + div_rcp(a * x + b * x * x * x, den, rcp_rn(den));
+}
 
 // log2_approx, cl2.cl:1498-1505.
 __device__ __forceinline__ float log2_approx(float val) {
@@ -378,8 +434,28 @@ __device__ __forceinline__ float gamma_fwd(float c) {
   return 0.012522878f * c + 0.682171111f * c * c + 0.305306011f * c * c * c;
 }
 __device__ __forceinline__ float gamma_inv(float c) {
-  float s1 = sqrtf(c), s2 = sqrtf(s1), s3 = sqrtf(s2);
+  float s1 = sqrt_cr(c), s2 = sqrt_cr(s1), s3 = sqrt_cr(s2);
   return 0.585122381f * s1 + 0.783140355f * s2 - 0.368262736f * s3;
+}
+// gamma_inv of three channels.  When every channel of every lane lies in
+// [2^-100, FLT_MAX] so do the nested roots (>= 2^-50), and the short root
+// needs no per-root check.
+__device__ __forceinline__ float gamma_inv_core(float c) {
+  float s1 = sqrt_core(c), s2 = sqrt_core(s1), s3 = sqrt_core(s2);
+  return 0.585122381f * s1 + 0.783140355f * s2 - 0.368262736f * s3;
+}
+__device__ __forceinline__ void gamma_inv3(float& x, float& y, float& z) {
+  const bool ok = x >= 0x1p-100f && x <= 0x1.fffffep127f && y >= 0x1p-100f && y <= 0x1.fffffep127f &&
+                  z >= 0x1p-100f && z <= 0x1.fffffep127f;
+  if (__all(ok)) {
+    x = gamma_inv_core(x);
+    y = gamma_inv_core(y);
+    z = gamma_inv_core(z);
+  } else {
+    x = gamma_inv(x);
+    y = gamma_inv(y);
+    z = gamma_inv(z);
+  }
 }
 
 // ret_cubeface, cl2.cl:1745-1790.
@@ -390,6 +466,48 @@ __device__ __forceinline__ int cubeface(f3 point, f3 light) {
   if (ay > ax && ay >= az) return rel.y < 0 ? 1 : 3;
   if (az > ax && az > ay) { if (rel.z < 0) return 2; }
   return 0;
+}
+
+// get_barycentric (cl2.cl:5372-5384) split into the terms of the triangle,
+// formed once per pixel, and the evaluation at a point.  v and w are the two
+// numerators over denom: when denom lies in [2^-60, 2^60] and each numerator
+// is +0 or of magnitude in [2^-60, 2^60] for every lane of the wavefront, the
+// quotients by the correctly rounded reciprocal of denom are the correctly
+// rounded ones; otherwise the wavefront divides.
+struct BaryPre {
+  f3 a, v0, v1;
+  float d00, d01, d11, denom, rden;
+  bool fast;
+};
+__device__ __forceinline__ BaryPre bary_pre(f3 a, f3 b, f3 c) {
+  BaryPre B;
+  B.a = a;
+  B.v0 = b - a;
+  B.v1 = c - a;
+  B.d00 = dot(B.v0, B.v0);
+  B.d01 = dot(B.v0, B.v1);
+  B.d11 = dot(B.v1, B.v1);
+  B.denom = B.d00 * B.d11 - B.d01 * B.d01;
+  B.fast = rcp_safe(B.denom);
+  B.rden = rcp_rn(B.denom);
+  return B;
+}
+__device__ __forceinline__ bool quot_ok(float n) {
+  return __float_as_uint(n) == 0u || (fabsf(n) >= 0x1p-60f && fabsf(n) <= 0x1p60f);
+}
+__device__ __forceinline__ void bary_eval(const BaryPre& B, f3 p, float& u, float& v, float& w) {
+  const f3 v2 = p - B.a;
+  const float d20 = dot(v2, B.v0), d21 = dot(v2, B.v1);
+  const float nv = B.d11 * d20 - B.d01 * d21;
+  const float nw = B.d00 * d21 - B.d01 * d20;
+  if (__all(B.fast && quot_ok(nv) && quot_ok(nw))) {
+    v = div_rcp(nv, B.denom, B.rden);
+    w = div_rcp(nw, B.denom, B.rden);
+  } else {
+    v = nv / B.denom;
+    w = nw / B.denom;
+  }
+  u = 1.0f - v - w;
 }
 
 // get_barycentric, cl2.cl:5372-5384.
@@ -414,7 +532,7 @@ __device__ __forceinline__ uint32_t to_short(float r) {
 __device__ __forceinline__ uint32_t encode_normal(f3 v) {
   if (v.x * v.x + v.y * v.y < 0.0001f) v.x = 0.01f;
   float inv = inv_sqrt_cr(v.x * v.x + v.y * v.y);
-  float sq = sqrtf(fmaxf(v.z * 0.5f + 0.5f, 0.f));
+  float sq = sqrt_cr(fmaxf(v.z * 0.5f + 0.5f, 0.f));
   return to_short((v.x * inv) * sq) | (to_short((v.y * inv) * sq) << 16);
 }
 

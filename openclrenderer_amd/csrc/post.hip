@@ -69,7 +69,7 @@ __device__ __forceinline__ f3 decode_normal_u16(uint32_t packed) {
   const float vy = div_rcp((float)(packed >> 16), 65535.f, kInv65535) * 2.f - 1.f;
   const float z = (vx * vx + vy * vy) * 2.f - 1.f;
   const float r = inv_sqrt_cr(vx * vx + vy * vy);
-  const float k = sqrtf(fmaxf(1.f - z * z, 0.f));
+  const float k = sqrt_cr(fmaxf(1.f - z * z, 0.f));
   return mk3((vx * r) * k, (vy * r) * k, z);
 }
 

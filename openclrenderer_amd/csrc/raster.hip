@@ -226,8 +226,6 @@ __device__ __forceinline__ void store_cut(CutTri* dst, const f3 p[3], float rcon
   *dst = c;
 }
 
-__device__ __forceinline__ f4 ld_quat(const ObjDesc& g) { return f4{g.q[0], g.q[1], g.q[2], g.q[3]}; }
-
 // full_rotate_quat's object-to-world step (cl2.cl:700-729): the same for the
 // camera and every light, so it is applied once per object update into the
 // scene's world-space copy of its triangles.
@@ -242,10 +240,11 @@ __global__ void __launch_bounds__(256) k_world(const TriPos* __restrict__ tris, 
   if (T.oid < nobj) {
     const ObjDesc G = objs[T.oid];
     const f3 wp = mk3(G.wp[0], G.wp[1], G.wp[2]);
-    const f4 q = ld_quat(G);
+    // rot_quat's normalised quaternion, formed once per descriptor
+    const f4 qn = f4{G.qn[0], G.qn[1], G.qn[2], G.qn[3]};
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-      const f3 r = rot_quat(mk3(T.p[3 * i], T.p[3 * i + 1], T.p[3 * i + 2]) * G.scale, q) + wp;
+      const f3 r = rot_quat_n(mk3(T.p[3 * i], T.p[3 * i + 1], T.p[3 * i + 2]) * G.scale, qn) + wp;
       W.p[3 * i] = r.x;
       W.p[3 * i + 1] = r.y;
       W.p[3 * i + 2] = r.z;

@@ -76,6 +76,7 @@ static ViewParams make_view_params(const ocr_view* v) {
   p.width = v->width;
   p.height = v->height;
   p.fov = v->fov_const;
+  p.ifov = 1.0f / v->fov_const;
   p.linear = v->use_linear_rendering;
   for (int i = 0; i < 3; i++) p.cpos[i] = v->c_pos[i];
   sincos_of(v->c_rot, p.cam);
@@ -196,6 +197,16 @@ int ocr_query_free_hbm(ocr_ctx* ctx, size_t* free_bytes, size_t* total_bytes) {
 }
 
 // ----------------------------------------------------------------- scene
+// The quaternion rot_quat rotates by (cl2.cl:350-357): q scaled by the
+// reciprocal of its correctly rounded length, products and sums left to
+// right in binary32 (host IEEE arithmetic, no contraction), exactly what the
+// kernels' rot_quat forms from q.
+static void normalised_quat(const float q[4], float out[4]) {
+  const float d = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  const float inv = 1.0f / sqrtf(d);
+  for (int i = 0; i < 4; i++) out[i] = q[i] * inv;
+}
+
 static void pack_obj(const ocr_obj_desc& s, ObjDesc& d) {
   memset(&d, 0, sizeof(d));
   for (int i = 0; i < 3; i++) d.wp[i] = s.world_pos[i];
@@ -206,6 +217,15 @@ static void pack_obj(const ocr_obj_desc& s, ObjDesc& d) {
   d.spec_mult = s.spec_mult;
   d.diffuse = s.diffuse;
   d.flags = s.feature_flag;
+  // per-object terms of kernel3 (cl2.cl:350-370, :6232-6240), formed once
+  normalised_quat(d.q, d.qn);
+  const float c[4] = {-d.q[0], -d.q[1], -d.q[2], d.q[3]};
+  const float len_sq = c[0] * c[0] + c[1] * c[1] + c[2] * c[2] + c[3] * c[3];
+  const float cb[4] = {c[0] / len_sq, c[1] / len_sq, c[2] / len_sq, c[3] / len_sq};
+  normalised_quat(cb, d.qb);
+  const float rough = fminf(fmaxf(1.f - d.specular, 0.001f), 10.f);
+  d.rough2 = rough * rough;            // in [1e-6, 100]
+  d.inv_rough2 = 1.0f / d.rough2;
 }
 
 int ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri, const ocr_obj_desc* objs,

@@ -100,7 +100,11 @@ __device__ __forceinline__ f4 read_tex_pre(const ShadeArgs& a, float cx, float c
   const float imax_tex_size = 1.f / 2048;
   float ihnum = width * imax_tex_size;
   float tnumy = floorf((float)which * ihnum);
-  float tnumx = (float)which - tnumy / ihnum;
+  // tnumy / ihnum: atlas tile sizes are powers of two 2^e (16 .. 2048, set
+  // by the page plan), so ihnum = 2^(e-11) and the quotient is the exact
+  // product by 2^(11-e), whose bits are (265 << 23) minus those of 2^e
+  const float inv_ihnum = __int_as_float((265 << 23) - __float_as_int(width));
+  float tnumx = (float)which - tnumy * inv_ihnum;
   cx = clampf(cx, 0.001f, width - 0.001f);
   cy = clampf(cy, 0.001f, width - 0.001f);
   int ix = (int)fmaf(tnumx, width, cx);
@@ -145,22 +149,50 @@ __device__ __forceinline__ f4 texture_filter_diff(const ShadeArgs& a, float vtx,
   int tsize = (int)ldg(a.sizes, (uint32_t)(nv >> 16));
   float vmx = tmod(vtx), vmy = tmod(vty);
   float tx = dx * (float)tsize, ty = dy * (float)tsize;
-  float worst = sqrtf(tx * tx + ty * ty);
+  float worst = sqrt_cr(tx * tx + ty * ty);
   float wf = fmaxf(log2_approx(worst), 0.f);
   float mip_lower = clampf(floorf(wf), 0.f, (float)kMipLevels);
   float fmd = wf - mip_lower;
   int tid_lower = mip_lower == 0 ? tid2 : (int)(mip_lower - 1 + (float)a.mip_start + (float)(tid2 * kMipLevels));
   int tid_higher = (int)(clampf(mip_lower, 0.f, kMipLevels - 1.f) + (float)a.mip_start + (float)(tid2 * kMipLevels));
-  int lnv = (int)ldg(a.nums, (uint32_t)tid_lower), hnv = (int)ldg(a.nums, (uint32_t)tid_higher);
-  float sl = (float)ldg(a.sizes, (uint32_t)(lnv >> 16)), sh = (float)ldg(a.sizes, (uint32_t)(hnv >> 16));
+  int lnv = (int)ldg(a.nums, (uint32_t)tid_lower);
+  float sl = (float)ldg(a.sizes, (uint32_t)(lnv >> 16));
   f4 c1 = bilinear_pre<kCount>(a, vmx * sl, vmy * sl, lnv & 0xFFFF, lnv >> 16, sl);
-  f4 c2 = bilinear_pre<kCount>(a, vmx * sh, vmy * sh, hnv & 0xFFFF, hnv >> 16, sh);
   const float i255 = 1.f / 255.f;
+  // From mip level 4 on both fetches read the last level: c2 = c1, the
+  // blend c1 + 0 * fmd is c1 (fmd is finite), and a wavefront whose lanes
+  // are all there skips the second fetch
+  if (__all(tid_lower == tid_higher)) return f4{c1.x * i255, c1.y * i255, c1.z * i255, c1.w * i255};
+  int hnv = (int)ldg(a.nums, (uint32_t)tid_higher);
+  float sh = (float)ldg(a.sizes, (uint32_t)(hnv >> 16));
+  f4 c2 = bilinear_pre<kCount>(a, vmx * sh, vmy * sh, hnv & 0xFFFF, hnv >> 16, sh);
   return f4{(c1.x + (c2.x - c1.x) * fmd) * i255, (c1.y + (c2.y - c1.y) * fmd) * i255,
             (c1.z + (c2.z - c1.z) * fmd) * i255, (c1.w + (c2.w - c1.w) * fmd) * i255};
 }
 
-// generate_ssao, cl2.cl:2194-2260.
+// generate_ssao, cl2.cl:2194-2260.  The 5x5 taps sit at offsets
+// round(k * world_rad), k = -2..2 on each axis (clamped to the frame).  Where
+// round(world_rad) is 0 the offsets are (-m, 0, 0, 0, m), m = round(2 *
+// world_rad), so the 25 taps read at most 9 distinct depths, each counted
+// 1, 3 or 1 times per axis; where m is 0 as well they all read one depth.
+// A wavefront whose every lane is in such a case counts the distinct taps
+// with their multiplicities (the same sum of the same per-tap counts); the
+// others run the 25 taps.  Far geometry (world_rad = (2 .. 2.5) * fov /
+// depth below 0.5) takes the short forms.
+struct SsaoTaps {
+  float thr[5];   // depth + z, z = -2..2, as the reference adds them
+  const ShadeArgs* a;
+  // passing depth tests of the tap at (wx, wy): thr[] is non-decreasing
+  // (rounding is monotone), so the count comes from the middle threshold and
+  // the pair on its side
+  __device__ __forceinline__ uint32_t at(float wx, float wy) const {
+    const float d2 = ((float)ldg(a->depth, (uint32_t)(int)wy * a->v.width + (uint32_t)(int)wx) / kMulint) * kDepthFar;
+    const bool hi = d2 > thr[2];
+    const float ta = hi ? thr[3] : thr[0], tb = hi ? thr[4] : thr[1];
+    return (hi ? 3u : 0u) + (d2 > ta ? 1u : 0u) + (d2 > tb ? 1u : 0u);
+  }
+};
+
 __device__ __forceinline__ float ssao(const ShadeArgs& a, int sx, int sy) {
   const uint32_t W = a.v.width, H = a.v.height;
   uint32_t seed2 = xorshift(wang_hash((uint32_t)sx + W * H * (uint32_t)sy));
@@ -168,29 +200,44 @@ __device__ __forceinline__ float ssao(const ShadeArgs& a, int sx, int sy) {
   float depth = ((float)ldg(a.depth, (uint32_t)sy * W + (uint32_t)sx) / kMulint) * kDepthFar;
   float rad = kSsaoRad;
   rad += foffset / 2.f;
-  float world_rad = rad * a.v.fov / depth;
+  // (rad * fov) / depth.  Depths of drawn pixels lie in [20, 350000]; with
+  // rad * fov in [2^-59, 2^62] the quotient by the correctly rounded
+  // reciprocal is then the correctly rounded one.  A degenerate two-sided
+  // triangle can leave depth 0 (its plane is NaN): a wavefront holding one
+  // divides.
+  float world_rad;
+  if (__all(depth >= 1.f)) world_rad = div_rcp(rad * a.v.fov, depth, rcp_rn(depth));
+  else world_rad = rad * a.v.fov / depth;
   float fsx = (float)sx, fsy = (float)sy;
-  // thresholds depth + z, z = -2..2, as the reference adds them
-  float thr[5];
+  SsaoTaps T;
+  T.a = &a;
 #pragma unroll
-  for (int z = 0; z < 5; z++) thr[z] = depth + (float)(z - 2);
+  for (int z = 0; z < 5; z++) T.thr[z] = depth + (float)(z - 2);
+  const float xmax = (float)W - 2.f, ymax = (float)H - 2.f;
+  const float m1 = roundf(world_rad), m2 = roundf(2.f * world_rad);
   // the reference sums 1.0f per passing test; the count (<= 125) is exact
   // in binary32, so it is kept as an integer and converted once
   uint32_t count = 0;
+  if (__all(m2 == 0.f)) {
+    count = 25u * T.at(clampf(fsx, 1.f, xmax), clampf(fsy, 1.f, ymax));
+  } else if (__all(m1 == 0.f)) {
+    const float xs[3] = {clampf(fsx - m2, 1.f, xmax), clampf(fsx, 1.f, xmax), clampf(fsx + m2, 1.f, xmax)};
+    const float ys[3] = {clampf(fsy - m2, 1.f, ymax), clampf(fsy, 1.f, ymax), clampf(fsy + m2, 1.f, ymax)};
+    const uint32_t mult[3] = {1u, 3u, 1u};
 #pragma unroll
-  for (int y = -2; y <= 2; y++) {
-    float oy = roundf((float)y * world_rad);
-    float wy = clampf(fsy + oy, 1.f, (float)H - 2.f);
+    for (int j = 0; j < 3; j++)
 #pragma unroll
-    for (int x = -2; x <= 2; x++) {
-      float ox = roundf((float)x * world_rad);
-      float wx = clampf(fsx + ox, 1.f, (float)W - 2.f);
-      float d2 = ((float)ldg(a.depth, (uint32_t)(int)wy * W + (uint32_t)(int)wx) / kMulint) * kDepthFar;
-      // thr[] is non-decreasing (rounding is monotone), so the number of
-      // thresholds below d2 comes from the middle one and the pair on its side
-      const bool hi = d2 > thr[2];
-      const float ta = hi ? thr[3] : thr[0], tb = hi ? thr[4] : thr[1];
-      count += (hi ? 3u : 0u) + (d2 > ta ? 1u : 0u) + (d2 > tb ? 1u : 0u);
+      for (int i = 0; i < 3; i++) count += mult[j] * mult[i] * T.at(xs[i], ys[j]);
+  } else {
+#pragma unroll
+    for (int y = -2; y <= 2; y++) {
+      float oy = roundf((float)y * world_rad);
+      float wy = clampf(fsy + oy, 1.f, ymax);
+#pragma unroll
+      for (int x = -2; x <= 2; x++) {
+        float ox = roundf((float)x * world_rad);
+        count += T.at(clampf(fsx + ox, 1.f, xmax), wy);
+      }
     }
   }
   float acc = (float)count;
@@ -309,6 +356,7 @@ __device__ __forceinline__ void uniform(ShadeArgs& a) {
   a.v.width = rfl(a.v.width);
   a.v.height = rfl(a.v.height);
   a.v.fov = rfl(a.v.fov);
+  a.v.ifov = rfl(a.v.ifov);
   a.v.linear = rfl(a.v.linear);
 #pragma unroll
   for (int i = 0; i < 3; i++) {
@@ -403,6 +451,8 @@ __device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
     a.normals[pix] = 0u;
     return;
   }
+  // SSAO reads only the depth buffer: formed first, while little else is live
+  const float occ_ssao = ssao(a, x, y);
   const float fov = a.v.fov;
   const CutTri ct = ldg(a.cuts, (uint32_t)key);
   const uint32_t tri = (uint32_t)(key >> 32) - 1u;
@@ -415,25 +465,30 @@ __device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
   const f3 p2 = mk3(T.p[3], T.p[4], T.p[5]) * G.scale;
   const f3 p3 = mk3(T.p[6], T.p[7], T.p[8]) * G.scale;
   const f3 wpos = mk3(G.wp[0], G.wp[1], G.wp[2]);
-  const f4 q = f4{G.q[0], G.q[1], G.q[2], G.q[3]};
+  // the object's quaternion as rot_quat normalises it, and its inverse as
+  // back_rot_quat forms and rot_quat then normalises it (formed once per
+  // descriptor upload with the same operations)
+  const f4 qn = f4{G.qn[0], G.qn[1], G.qn[2], G.qn[3]};
+  const f4 qb = f4{G.qb[0], G.qb[1], G.qb[2], G.qb[3]};
 
   const float actual_depth = ((float)ft / kMulint) * kDepthFar;
-  // quotients by the view's fov from its correctly rounded reciprocal
-  const bool fov_fast = rcp_safe(fov);
-  const float ifov = fov_fast ? 1.f / fov : 0.f;
-  auto by_fov = [&](float n) { return fov_fast ? div_rcp(n, fov, ifov) : n / fov; };
+  // quotients by the view's fov from its correctly rounded reciprocal (the
+  // view's fov is checked to lie in [2^-60, 2^60] on the host)
+  const float ifov = a.v.ifov;
+  auto by_fov = [&](float n) { return div_rcp(n, fov, ifov); };
   f3 local_position = mk3(by_fov(((float)x - (float)W / 2.0f) * actual_depth),
                           by_fov(((float)y - (float)H / 2.0f) * actual_depth), actual_depth);
   f3 global_position = back_rot(local_position, a.v.cam) + camera_pos;
-  f3 object_local = back_rot_quat(global_position - wpos, q);
+  f3 object_local = rot_quat_n(global_position - wpos, qb);
   float l1, l2, l3;
-  barycentric(object_local, p1, p2, p3, l1, l2, l3);
+  const BaryPre BP = bary_pre(p1, p2, p3);
+  bary_eval(BP, object_local, l1, l2, l3);
   const float vtx = fmaf(TA.vt[0], l1, fmaf(TA.vt[2], l2, TA.vt[4] * l3));
   const float vty = fmaf(TA.vt[1], l1, fmaf(TA.vt[3], l2, TA.vt[5] * l3));
   f3 normal = mk3(fmaf(TA.n[0], l1, fmaf(TA.n[3], l2, TA.n[6] * l3)),
                   fmaf(TA.n[1], l1, fmaf(TA.n[4], l2, TA.n[7] * l3)),
                   fmaf(TA.n[2], l1, fmaf(TA.n[5], l2, TA.n[8] * l3)));
-  normal = rot_quat(normal, q);
+  normal = rot_quat_n(normal, qn);
 
   // get_vtdiff, cl2.cl:5691-5760.
   float vdx, vdy;
@@ -443,19 +498,25 @@ __device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
     const float DA = ct.da, DB = ct.db, DC = ct.dc;
     float dmx = fmaf(DA, fx + 1, fmaf(DB, fy, DC));
     float dmy = fmaf(DA, fx, fmaf(DB, fy + 1, DC));
-    // (n / fov) / d for both components, and 1 / d, which is the correctly
-    // rounded reciprocal the other two quotients are formed from
-    auto over = [](float n0, float n1, float d) {
-      const float y = 1.f / d;
-      return rcp_safe(d) ? mk3(div_rcp(n0, d, y), div_rcp(n1, d, y), y) : mk3(n0 / d, n1 / d, y);
-    };
-    f3 lmx = over(by_fov(fx + 1 - (float)W / 2.f), by_fov(fy - (float)H / 2.f), dmx);
-    f3 lmy = over(by_fov(fx - (float)W / 2.f), by_fov(fy + 1 - (float)H / 2.f), dmy);
-    f3 gmx = back_rot_quat((back_rot(lmx, a.v.cam) + camera_pos) - wpos, q);
-    f3 gmy = back_rot_quat((back_rot(lmy, a.v.cam) + camera_pos) - wpos, q);
+    // (n / fov) / d for both components, and 1 / d.  The numerators are
+    // +0 or pixel offsets of at least 1/2 over fov, so where d lies in
+    // [2^-60, 2^60] the quotients by the correctly rounded reciprocal are
+    // the correctly rounded ones; other lanes (a plane extrapolated through
+    // 0) divide, in a branch skipped when no lane needs it.
+    const float n0x = by_fov(fx + 1 - (float)W / 2.f), n1x = by_fov(fy - (float)H / 2.f);
+    const float n0y = by_fov(fx - (float)W / 2.f), n1y = by_fov(fy + 1 - (float)H / 2.f);
+    const float yx = rcp_rn(dmx), yy = rcp_rn(dmy);
+    f3 lmx = mk3(div_rcp(n0x, dmx, yx), div_rcp(n1x, dmx, yx), yx);
+    f3 lmy = mk3(div_rcp(n0y, dmy, yy), div_rcp(n1y, dmy, yy), yy);
+    if (!(rcp_safe(dmx) && rcp_safe(dmy))) {
+      lmx = mk3(n0x / dmx, n1x / dmx, 1.f / dmx);
+      lmy = mk3(n0y / dmy, n1y / dmy, 1.f / dmy);
+    }
+    f3 gmx = rot_quat_n((back_rot(lmx, a.v.cam) + camera_pos) - wpos, qb);
+    f3 gmy = rot_quat_n((back_rot(lmy, a.v.cam) + camera_pos) - wpos, qb);
     float lx1, lx2, lx3, ly1, ly2, ly3;
-    barycentric(gmx, p1, p2, p3, lx1, lx2, lx3);
-    barycentric(gmy, p1, p2, p3, ly1, ly2, ly3);
+    bary_eval(BP, gmx, lx1, lx2, lx3);
+    bary_eval(BP, gmy, ly1, ly2, ly3);
     float vtxx = fmaf(TA.vt[0], lx1, fmaf(TA.vt[2], lx2, TA.vt[4] * lx3));
     float vtxy = fmaf(TA.vt[1], lx1, fmaf(TA.vt[3], lx2, TA.vt[5] * lx3));
     float vtyx = fmaf(TA.vt[0], ly1, fmaf(TA.vt[2], ly2, TA.vt[4] * ly3));
@@ -489,7 +550,6 @@ __device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
   const bool is_front = front_facing(mk3(ct.x[0], ct.y[0], ct.z[0]), mk3(ct.x[1], ct.y[1], ct.z[1]),
                                      mk3(ct.x[2], ct.y[2], ct.z[2]));
   if (!is_front && two_sided) normal = -normal;
-  const float occ_ssao = ssao(a, x, y);
   normal = fast_normalize(normal);
   // rseed / 100: each component is 0 or of magnitude >= 2^-25, where the
   // short division by 100 is exact
@@ -507,26 +567,38 @@ __device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
     // the static and the dynamic map of a light share the face, position
     // and bias; both use the normalised point-to-light vector shading uses
     const bool stat = kShadows && l.shadow && l.is_static;
+    // fast_length(ptl) and fast_normalize(ptl) form the same dot product and
+    // correctly rounded root: formed once
+    float distance, inv_len;
+    length_and_inverse(dot(ptl, ptl), distance, inv_len);
     f3 ptl_n;
     ShadowSetup ss;
     if (stat) {
-      ptl_n = fast_normalize(ptl);
+      ptl_n = ptl * inv_len;
       ss = shadow_setup(a, lpos, normal, ptl_n, global_position);
       occlusion = 1.f - shadow_taps<kCount>(a, ss, a.static_maps, a.static_touch, static_num);
       static_num++;
     }
-    float distance = fast_length(ptl);
     // 1/radius from the light upload (0 when the radius is out of range)
     const float inv_r = rfl(ldg(a.light_lin, 4 * i + 3));
     float t = (inv_r != 0.f ? div_rcp(distance, l.radius, inv_r) : distance / l.radius) + 1;
-    float illumination = l.brightness / (t * t);
+    // brightness / t^2: with the brightness +0 or of magnitude in
+    // [2^-60, 2^60] and t^2 in [1, 2^60] the quotient lies in [2^-120, 2^60],
+    // where the one from the correctly rounded reciprocal is correctly
+    // rounded; a wavefront with a lane outside divides
+    const float t2 = t * t;
+    float illumination;
+    if (bright_ok(l.brightness) && __all(t2 >= 1.f && t2 <= 0x1p60f))
+      illumination = div_rcp(l.brightness, t2, rcp_rn(t2));
+    else
+      illumination = l.brightness / t2;
     const float cutoff = 0.1f;
     illumination -= cutoff;
     illumination *= 1.f / (1.f - cutoff);
     if (illumination <= 0) continue;
     f3 light_col = mk3(l.col[0], l.col[1], l.col[2]);
     if (linear) light_col = mk3(ldg(a.light_lin, 4 * i), ldg(a.light_lin, 4 * i + 1), ldg(a.light_lin, 4 * i + 2));
-    if (!stat) ptl_n = fast_normalize(ptl);
+    if (!stat) ptl_n = ptl * inv_len;
     if (kShadows && l.shadow && recv_dyn) {
       if (!stat) ss = shadow_setup(a, lpos, normal, ptl_n, global_position);
       float dyn = 1.f - shadow_taps<kCount>(a, ss, a.shadow_maps, a.shadow_touch, shnum);
@@ -548,10 +620,14 @@ __device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
     float ndl = fmaxf(0.f, dot(N, ptl));
     const float F0 = 0.4f;
     float fresnel = F0 + (1 - F0) * powr5(1.f - vdh);
-    float rough = clampf(1.f - G.specular, 0.001f, 10.f);
     const float gauss_constant = 0.8346f;
-    float alpha = rational_acos(ndh);
-    float microfacet = gauss_constant * exp_f(-alpha * alpha / (rough * rough));
+    float alpha = rational_acos_unit(ndh);
+    // -alpha^2 / rough^2, rough = clamp(1 - specular, 0.001, 10), with rough^2
+    // in [1e-6, 100] and its correctly rounded reciprocal from the object's
+    // descriptor: for |alpha^2| in [2^-100, 2.5] the quotient is the correctly
+    // rounded one; below, both quotients are zero or below 2^-80, whose exp
+    // is 1 either way
+    float microfacet = gauss_constant * exp_f(div_rcp(-alpha * alpha, G.rough2, G.inv_rough2));
     float sv = 2 * ndh / vdh;
     float c1 = sv * ndv;
     float c2 = sv * ndl;
@@ -568,7 +644,7 @@ __device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
   f3 fc = mk3(fmaf(colclamp.x, diffuse_sum.x, specular_sum.x * (1.f - rsc)),
               fmaf(colclamp.y, diffuse_sum.y, specular_sum.y * (1.f - rsc)),
               fmaf(colclamp.z, diffuse_sum.z, specular_sum.z * (1.f - rsc)));
-  if (linear) fc = mk3(gamma_inv(fc.x), gamma_inv(fc.y), gamma_inv(fc.z));
+  if (linear) gamma_inv3(fc.x, fc.y, fc.z);
   a.rgba[pix] = quant8(fc.x) | (quant8(fc.y) << 8) | (quant8(fc.z) << 16) | (quant8(col.w) << 24);
   a.normals[pix] = encode_normal(normal);
 }

@@ -69,6 +69,8 @@ class Oracle:
         L.oracle_render.argtypes = [C.POINTER(OracleScene), C.c_void_p, C.POINTER(SinCos), C.c_int,
                                     C.POINTER(OracleOut)]
         L.oracle_shadow_maps.argtypes = [C.POINTER(OracleScene), C.c_int, C.c_void_p, C.c_void_p]
+        L.oracle_shadow_maps_ex.argtypes = [C.POINTER(OracleScene), C.c_int, C.c_void_p, C.c_void_p,
+                                            C.POINTER(C.c_uint64)]
         L.oracle_atlas_plan.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
                                         C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]
         L.oracle_atlas_fill.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
@@ -282,7 +284,9 @@ class Oracle:
         n = self.lib.oracle_cut_list(C.byref(s), _ptr(v), C.byref(cam), _ptr(cuts), _ptr(keys), cap)
         return cuts[:n], keys[:n]
 
-    def shadow_maps(self, scene, nthreads=1):
+    def shadow_maps(self, scene, nthreads=1, with_samples=False):
+        """Dynamic and static cube maps; with_samples also returns the
+        covered samples (depth tests) of all the maps."""
         L = scene.light_buffer_dim
         nsh = int(np.sum(scene.lights["shadow"] == 1))
         nst = int(np.sum((scene.lights["shadow"] != 0) & (scene.lights["is_static"] != 0)))
@@ -291,8 +295,9 @@ class Oracle:
         dummy_atlas = (np.zeros((1, 1, 1, 4), np.uint8), np.zeros(1, np.uint32),
                        np.zeros(1, np.uint32), 0)
         s = self._scene(scene, dummy_atlas, None)
-        self.lib.oracle_shadow_maps(C.byref(s), nthreads, _ptr(dyn), _ptr(stat))
-        return dyn, stat
+        samples = C.c_uint64(0)
+        self.lib.oracle_shadow_maps_ex(C.byref(s), nthreads, _ptr(dyn), _ptr(stat), C.byref(samples))
+        return (dyn, stat, samples.value) if with_samples else (dyn, stat)
 
     def render(self, scene, view, atlas=None, maps=None, nthreads=1, shade=True):
         if atlas is None:

@@ -80,33 +80,8 @@ static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 /* ------------------------------------------------------ binary64 helpers */
 /* exp in binary64: Cody-Waite reduction by ln2, degree-13 Taylor polynomial
  * (|r| <= ln2/2, truncation error < 2e-17), scaled by 2^k. */
-static double ocrx_exp(double x) {
-    if (x != x) return x;
-    if (x > 100.0) return INFINITY;
-    if (x < -120.0) return 0.0;
-    const double INV_LN2 = 1.44269504088896338700e+00;
-    const double LN2_HI = 6.93147180369123816490e-01;
-    const double LN2_LO = 1.90821492927058770002e-10;
-    double k = floor(fma(x, INV_LN2, 0.5));
-    double r = fma(-k, LN2_LO, fma(-k, LN2_HI, x));
-    double p = 1.0 / 6227020800.0;                 /* 1/13! */
-    p = fma(p, r, 1.0 / 479001600.0);
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
-    p = fma(p, r, 0.5);
-    p = fma(p, r, 1.0);
-    p = fma(p, r, 1.0);
-    int64_t ki = (int64_t)k;
-    uint64_t bits = (uint64_t)(ki + 1023) << 52;
-    double scale; memcpy(&scale, &bits, 8);
-    return p * scale;
+void oracle_exp_n(const float* x, float* y, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) y[i] = oracle_exp(x[i]);
 }
 
 /* tan in binary64: reduction by pi/2 (two-constant Cody-Waite, exact for the
@@ -142,7 +117,27 @@ static double ocrx_tan(double x) {
     return (ki & 1) ? -c / s : s / c;
 }
 
-float oracle_exp(float x) { return (float)ocrx_exp((double)x); }
+/* exp in binary32 (the arithmetic contract's exp; the reference's exp under
+ * -cl-fast-relaxed-math has implementation-defined accuracy): k =
+ * rint(x log2 e), r = x - k ln2 in two Cody-Waite steps (k * ln2_hi exact for
+ * |k| < 2^11), e^r = 1 + r p(r), p of degree 5 fitted to (e^r - 1) / r on
+ * [-ln2/2, ln2/2] (relative error of e^r 3.4e-9), then ldexp.  Within one
+ * ulp of the correctly rounded exp on [-104, 88.75]. */
+float oracle_exp(float x) {
+    if (!(x >= -104.0f)) return x != x ? x : 0.0f;
+    if (x > 88.75f) return INFINITY;
+    const float k = rintf(x * 1.44269502f);
+    float r = fmaf(-k, 0.693145752f, x);
+    r = fmaf(-k, 1.42860677e-06f, r);
+    float p = 0.001399628003127873f;
+    p = fmaf(p, r, 0.008374124765396118f);
+    p = fmaf(p, r, 0.04166550189256668f);
+    p = fmaf(p, r, PLANT(13) ? 0.17f : 0.16666433215141296f);
+    p = fmaf(p, r, 0.5000000596046448f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    return ldexpf(p, (int)k);
+}
 float oracle_tan(float x) { return (float)ocrx_tan((double)x); }
 /* native_powr(x, 5.f) (cl2.cl:6236) and pow(x, 4.f) (cl2.cl:2480) */
 float oracle_powr5(float x) { double d = x; double d2 = d * d; return (float)((d2 * d2) * d); }
@@ -1026,9 +1021,10 @@ int oracle_render(const oracle_scene* sc, const ocr_view* v, const oracle_sincos
 /* --------------------------------------------------------- shadows */
 /* prearrange_realtime_shadowing (cl2.cl:4419-4636) + kernel1_realtime_shadowing
  * (cl2.cl:5129-5246) for one light into one 6-face map. */
-typedef struct { uint32_t* map; uint32_t L; uint32_t face; int atomic; } kshctx;
+typedef struct { uint32_t* map; uint32_t L; uint32_t face; int atomic; uint64_t samples; } kshctx;
 static void ksh_visit(void* c, int x, int y, uint32_t d) {
     kshctx* k = (kshctx*)c;
+    k->samples++;
     uint32_t* ft = &k->map[(size_t)y * k->L + (size_t)x + (size_t)k->face * k->L * k->L];
     if (k->atomic) {
         uint32_t old = __atomic_load_n(ft, __ATOMIC_RELAXED);
@@ -1039,7 +1035,8 @@ static void ksh_visit(void* c, int x, int y, uint32_t d) {
 }
 
 static void render_light_map(const oracle_scene* sc, f3 lpos, int only_static,
-                             const oracle_sincos face_sc[6], int nt, uint32_t* map, uint64_t* frags) {
+                             const oracle_sincos face_sc[6], int nt, uint32_t* map, uint64_t* frags,
+                             uint64_t* samples) {
     const uint32_t L = sc->lbuf_dim;
     const float efov = PLANT(5) ? (float)L : (float)L / 2.0f, ew = (float)L, eh = (float)L;
     const int op_l = PLANT(6) ? 500 : OP_SIZE_L;
@@ -1084,13 +1081,17 @@ static void render_light_map(const oracle_scene* sc, f3 lpos, int only_static,
                 calc_min_max(p, ew, eh, mm);
                 float area = (mm[1] - mm[0]) * (mm[3] - mm[2]);
                 float thread_num = ceilf(area / (float)op_l);
-                kshctx k = {map, L, (uint32_t)kk, nt > 1};
+                kshctx k = {map, L, (uint32_t)kk, nt > 1, 0};
                 if (frags) {
                     #pragma omp atomic
                     frags[kk] += (uint64_t)thread_num;
                 }
                 for (uint32_t a = 0; (float)a < thread_num; a++)
                     walk_fragment(p, ew, eh, a, op_l, rconst, ksh_visit, &k);
+                if (samples) {
+                    #pragma omp atomic
+                    *samples += k.samples;
+                }
             }
         }
     }
@@ -1098,6 +1099,12 @@ static void render_light_map(const oracle_scene* sc, f3 lpos, int only_static,
 
 int oracle_shadow_maps(const oracle_scene* sc, int nthreads, uint32_t* dyn_maps,
                        uint32_t* static_maps) {
+    return oracle_shadow_maps_ex(sc, nthreads, dyn_maps, static_maps, NULL);
+}
+
+int oracle_shadow_maps_ex(const oracle_scene* sc, int nthreads, uint32_t* dyn_maps,
+                          uint32_t* static_maps, uint64_t* samples) {
+    if (samples) *samples = 0;
     oracle_sincos face_sc[6];
     oracle_face_sincos(face_sc);
     const size_t per = (size_t)6 * sc->lbuf_dim * sc->lbuf_dim;
@@ -1109,13 +1116,13 @@ int oracle_shadow_maps(const oracle_scene* sc, int nthreads, uint32_t* dyn_maps,
         if (l->shadow == 1) {
             uint32_t* m = &dyn_maps[per * nn];
             for (size_t j = 0; j < per; j++) m[j] = 0xFFFFFFFFu;
-            render_light_map(sc, lpos, 0, face_sc, nt, m, NULL);
+            render_light_map(sc, lpos, 0, face_sc, nt, m, NULL, samples);
             nn++;
         }
         if (l->shadow && l->is_static) {
             uint32_t* m = &static_maps[per * kk];
             for (size_t j = 0; j < per; j++) m[j] = 0xFFFFFFFFu;
-            render_light_map(sc, lpos, 1, face_sc, nt, m, NULL);
+            render_light_map(sc, lpos, 1, face_sc, nt, m, NULL, samples);
             kk++;
         }
     }
@@ -1348,7 +1355,7 @@ int oracle_shadow_map_one(const oracle_scene* sc, uint32_t light, int only_stati
     const size_t per = (size_t)6 * sc->lbuf_dim * sc->lbuf_dim;
     for (size_t j = 0; j < per; j++) map[j] = 0xFFFFFFFFu;
     if (frags) for (int k = 0; k < 6; k++) frags[k] = 0;
-    render_light_map(sc, ld3(sc->lights[light].pos), only_static, face_sc, 1, map, frags);
+    render_light_map(sc, ld3(sc->lights[light].pos), only_static, face_sc, 1, map, frags, NULL);
     return 0;
 }
 void oracle_texture_mod(const float vt[2], float out[2]) { out[0] = tmod1(vt[0]); out[1] = tmod1(vt[1]); }

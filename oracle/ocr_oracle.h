@@ -19,8 +19,8 @@
  *     rounded once (the camera's are supplied by the caller);
  *   - pow(x,2) = x*x; pow(x,4), powr(x,5) are formed in binary64 by exact
  *     squaring and rounded once to binary32;
- *   - exp and tan are evaluated in binary64 (ocrx_exp / ocrx_tan below)
- *     and rounded once to binary32;
+ *   - tan is evaluated in binary64 (ocrx_tan) and rounded once to binary32;
+ *     exp by a fixed binary32 sequence (oracle_exp);
  *   - float -> u32 conversions of depths and texels saturate (NaN -> 0);
  *   - the id buffer tie rule is the sequential one: fragments are produced
  *     in triangle order and the last fragment that passes the depth window
@@ -83,6 +83,9 @@ int oracle_render(const oracle_scene* sc, const ocr_view* view,
  * sin/cos come from oracle_face_sincos. */
 int oracle_shadow_maps(const oracle_scene* sc, int nthreads, uint32_t* dyn_maps,
                        uint32_t* static_maps);
+/* The same, also returning the covered samples (depth tests) of all maps. */
+int oracle_shadow_maps_ex(const oracle_scene* sc, int nthreads, uint32_t* dyn_maps,
+                          uint32_t* static_maps, uint64_t* samples);
 /* sin/cos of the six cube-face rotations r_struct (cl2.cl:4487-4511). */
 void oracle_face_sincos(oracle_sincos out[6]);
 
@@ -121,6 +124,7 @@ void  oracle_gamma_transform(const float in[4], float out[4]);
 void  oracle_gamma_inverse(const float in[4], float out[4]);
 int   oracle_ret_cubeface(const float p[3], const float l[3]);
 float oracle_exp(float x);
+void  oracle_exp_n(const float* x, float* y, uint32_t n);
 float oracle_tan(float x);
 float oracle_powr5(float x);
 float oracle_pow4(float x);

@@ -417,7 +417,18 @@ def pin_wang_hash(o):
     assert o.lib.oracle_wang_hash(123456789) == 2974685137
 
 
-PINS = [pin_face_orientation, pin_vertex_face_rule, pin_min_depth_brute_force, pin_hard_occlusion,
+def pin_exp(o):
+    """exp within one ulp of the correctly rounded exp over [-104, 88.7]."""
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(-104, 88.7, 6000), rng.uniform(-0.4, 0.4, 2000)]).astype(np.float32)
+    got = np.array([o.lib.oracle_exp(float(x)) for x in xs], np.float32)
+    want = np.exp(xs.astype(np.float64)).astype(np.float32)
+    ulps = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    normal = xs > -87.3     # subnormal results are rounded twice
+    assert ulps.max() <= 1 and np.mean(ulps[normal]) < 0.15, (ulps.max(), np.mean(ulps[normal]))
+
+
+PINS = [pin_exp, pin_face_orientation, pin_vertex_face_rule, pin_min_depth_brute_force, pin_hard_occlusion,
         pin_texture_filter, pin_mip_selection_frame, pin_ssao, pin_wang_hash]
 
 
@@ -453,6 +464,7 @@ PLANTS = {
     10: "mip blend weight inverted",
     11: "ret_cubeface +x and -x swapped",
     12: "shadow lookup LFOV = L",
+    13: "exp polynomial coefficient of r^3 off by 2%",
 }
 
 
