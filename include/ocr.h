@@ -244,6 +244,21 @@ int  ocr_frame_set_profiling(ocr_frame* f, int enable);
  * the renders of one frame.  A frame serves either ocr_render or batches:
  * a frame already set on a batch is refused (OCR_ERR_INVALID). */
 int  ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream);
+/* No counterpart in the reference: ocr_render for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Renders `view` of `s` on `stream` into
+ * the caller's device buffers of view->width * view->height elements: depth
+ * (u32), id_key (u64: ((2*tri + piece + 1) << 32) | record, 0 where nothing
+ * is drawn; ocr_frame_read's triangle id is (key >> 32) - 1 >> 1), rgba8
+ * (u32) and normals (u32, two u16).  The work lists live in a 64 MiB device
+ * scratch array the library holds per GPU (524,286 cut-triangle records, two
+ * per scene triangle), so the scene may have at most 262,111 triangles
+ * (OCR_ERR_UNSUPPORTED beyond); the scene's lights must already be uploaded.  Calls on one GPU share the
+ * scratch: calls that overlap on one GPU race (order them on one stream).
+ * A view needing more than 2^31 - 1 raster samples is not drawn (outputs
+ * hold the clear); ocr_render reports that case. */
+int  ocr_render_noalloc(ocr_scene* s, const ocr_view* view, uint32_t* depth, unsigned long long* id_key,
+                        uint32_t* rgba8, uint32_t* normals, void* stream);
 
 /* Replaces the CL/GL blit and its host read-back path
  * (cl_gl_interop_texture.hpp:189-252, :312-331).  Synchronises `stream`,

@@ -194,6 +194,9 @@ struct Ctx {
   int device = 0;
   hipDeviceProp_t prop{};
   SinCos face_sc[6];
+  // device address of the no-allocation render's scratch on this device
+  // (hipGetSymbolAddress at ocr_create)
+  unsigned char* noalloc_scratch = nullptr;
 };
 
 struct Atlas {

@@ -22,6 +22,17 @@ namespace ocr {
 
 static thread_local std::string g_last_error;
 
+// Scratch of ocr_render_noalloc: the counters and the cut list (CutTri +
+// RasterTri records) of one render, in a file-scope device array that every
+// GPU the library runs on holds until the process ends.  Host code reaches
+// it only through the address hipGetSymbolAddress returns for the device
+// (Ctx::noalloc_scratch); calls on one GPU share it and must not overlap.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr size_t kNoAllocCounterBytes = 256;
+constexpr uint32_t kNoAllocCapCut =
+    (uint32_t)((kNoAllocScratchBytes - kNoAllocCounterBytes) / (sizeof(CutTri) + sizeof(RasterTri)));
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 void set_error(const std::string& msg) { g_last_error = msg; }
 int fail(int code, const std::string& msg) {
   set_error(msg);
@@ -180,6 +191,11 @@ int ocr_create(int device, ocr_ctx** out) {
   }
   face_table(c->face_sc);
   g_cu_count = c->prop.multiProcessorCount;
+  e = hipGetSymbolAddress((void**)&c->noalloc_scratch, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    delete c;
+    return fail(OCR_ERR_HIP, std::string("ocr_create: scratch address: ") + hipGetErrorString(e));
+  }
   *out = c;
   return OCR_OK;
 }
@@ -868,6 +884,61 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   OCR_HIP_CHECK(hipStreamWaitEvent(st, f->done[b], 0));
   if (prof) f->n_profiled++;
   f->rendered = true;
+  return OCR_OK;
+}
+
+int ocr_render_noalloc(ocr_scene* s, const ocr_view* view, uint32_t* depth, unsigned long long* id_key,
+                       uint32_t* rgba8, uint32_t* normals, void* stream) {
+  if (!s || !view || !depth || !id_key || !rgba8 || !normals)
+    return fail(OCR_ERR_INVALID, "ocr_render_noalloc: null argument");
+  if (view->width < 4 || view->height < 4 || (uint64_t)view->width * view->height > kMaxTargetPixels ||
+      view->width > 32768 || view->height > 32768)
+    return fail(OCR_ERR_INVALID, "ocr_render_noalloc: view size outside 4x4 .. 16,000,000 pixels");
+  if (!(view->fov_const >= 0x1p-60f && view->fov_const <= 0x1p60f))
+    return fail(OCR_ERR_INVALID, "ocr_render_noalloc: fov_const must be in [2^-60, 2^60]");
+  if (!s->atlas.pages) return fail(OCR_ERR_INVALID, "ocr_render_noalloc: scene has no atlas (ocr_atlas_upload)");
+  if (s->nobj && s->max_tid >= std::max<uint32_t>(s->atlas.mip_start, 1))
+    return fail(OCR_ERR_INVALID, "ocr_render_noalloc: an object's texture id is outside the atlas");
+  // ocr_render gives a scene without lights empty light buffers, which
+  // allocates: here the caller uploads them first
+  if (!s->lights) return fail(OCR_ERR_INVALID, "ocr_render_noalloc: upload the scene's lights first (ocr_lights_upload)");
+  // at most two cut pieces per triangle: the list cannot overflow
+  if ((uint64_t)s->ntri * 2 + 64 > kNoAllocCapCut)
+    return fail(OCR_ERR_UNSUPPORTED, "ocr_render_noalloc: scene larger than the 64 MiB scratch holds (at most " +
+                                         std::to_string((kNoAllocCapCut - 64) / 2) + " triangles)");
+  if (!s->ctx->noalloc_scratch) return fail(OCR_ERR_INVALID, "ocr_render_noalloc: context has no scratch");
+  OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t* counters = reinterpret_cast<uint32_t*>(s->ctx->noalloc_scratch);
+  CutTri* cuts = reinterpret_cast<CutTri*>(s->ctx->noalloc_scratch + kNoAllocCounterBytes);
+  PassArgs pa;
+  memset(&pa, 0, sizeof(pa));
+  pa.view0.v = make_view_params(view);
+  pa.view0.depth = depth;
+  pa.view0.idkey = id_key;
+  pa.view0.rgba = rgba8;
+  pa.view0.normals = normals;
+  pa.nviews = 1;
+  pa.tri_slots = round256(s->ntri);
+  pa.pix_slots = round256((uint64_t)view->width * view->height);
+  pa.tile_w_log2 = 4;
+  pa.tiles = view_tiles(view->width, view->height, pa.tile_w_log2);
+  // reads the current geometry version, registered as its reader below
+  std::lock_guard<std::mutex> lk(s->ver_mu);
+  pa.scene0 = dev_scene(s);
+  pa.unshadowed = s->n_shadow == 0 ? 1u : 0u;
+  pa.lists = make_lists(cuts, counters, kNoAllocCapCut);
+  PassArgs pc = pa;
+  pc.lists.counters = nullptr;   // zeroed by their own kernel below
+  OCR_HIP_CHECK(s->wait_geometry(st));
+  OCR_HIP_CHECK(launch_zero_counters(counters, st));
+  OCR_HIP_CHECK(launch_prearrange(pa, st));
+  OCR_HIP_CHECK(launch_clear(pc, st));
+  OCR_HIP_CHECK(launch_depth(pa, kRasterBlocksPerCu, st));
+  OCR_HIP_CHECK(launch_id(pa, kRasterBlocksPerCu, st));
+  OCR_HIP_CHECK(s->wait_shadow_maps(st));
+  OCR_HIP_CHECK(launch_shade(pa, s->ctx->face_sc, st));
+  OCR_HIP_CHECK(s->note_read(st, true));
   return OCR_OK;
 }
 

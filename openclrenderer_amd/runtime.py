@@ -20,7 +20,7 @@ EXPORTS = [
     "ocr_scene_create", "ocr_scene_update_objs", "ocr_scene_update_objs_async", "ocr_atlas_upload", "ocr_lights_upload",
     "ocr_shadow_pass", "ocr_scene_destroy", "ocr_scene_atlas_info", "ocr_scene_read_atlas",
     "ocr_scene_read_shadow_maps", "ocr_scene_read_motion", "ocr_frame_create", "ocr_frame_destroy",
-    "ocr_frame_set_profiling", "ocr_render", "ocr_frame_read", "ocr_frame_device_buffers",
+    "ocr_frame_set_profiling", "ocr_render", "ocr_render_noalloc", "ocr_frame_read", "ocr_frame_device_buffers",
     "ocr_frame_blend_with_depth", "ocr_frame_pseudo_aa", "ocr_frame_motion_blur",
     "ocr_frame_get_stats", "ocr_frame_count_texels", "ocr_frame_reserve",
     "ocr_batch_create", "ocr_batch_destroy", "ocr_batch_set_views", "ocr_batch_set_tuning",
@@ -110,6 +110,7 @@ def load_library(path=LIB_PATH):
     L.ocr_frame_set_profiling.argtypes = [vp, i32]
     L.ocr_frame_reserve.argtypes = [vp, u32]
     L.ocr_render.argtypes = [vp, vp, vp, vp]
+    L.ocr_render_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.ocr_frame_read.argtypes = [vp, vp, vp, vp, vp, vp]
     L.ocr_frame_device_buffers.argtypes = [vp, pp, pp, pp, pp]
     L.ocr_frame_blend_with_depth.argtypes = [vp, vp, vp]
