@@ -117,6 +117,12 @@ def calibration():
             "atomic_min_per_s_uniform": n / (us_rand * 1e-6), "atomic_min_per_s_clustered": n / (us_clus * 1e-6)}
 
 
+def shadow_group_size(ntri):
+    """Lights per shadow prearrange launch (the runtime renders up to 16
+    lights' maps in one launch, whatever the triangle count)."""
+    return 16
+
+
 def shade_algorithmic_bytes(W, H, V, O, x_tex, x_shadow, L):
     """Minimal field bytes of one shading launch: 16 B per pixel (depth 4,
     triangle id 4, RGBA8 4, normal 4; the next render's clear is k_clear's),
@@ -193,16 +199,17 @@ def cpu_baseline(sc, view, gpu_out, seconds, with_shadows=False):
     maps = None if with_shadows or not has_shadow else o.shadow_maps(sc, cores)
     t_end = time.perf_counter() + seconds
 
-    shadow_samples = None
+    shadow_samples = covered = None
 
     def frames(nthreads, warm, timed):
-        nonlocal maps, shadow_samples
+        nonlocal maps, shadow_samples, covered
         times, first = [], None
         for k in range(warm + timed):
             t0 = time.perf_counter()
             if with_shadows:
                 dyn, stat, shadow_samples = o.shadow_maps(sc, nthreads, with_samples=True)
                 maps = (dyn, stat)
+                covered = int(np.count_nonzero(dyn != 0xFFFFFFFF))
             r = o.render(sc, view, atlas=atlas, maps=maps, nthreads=nthreads)
             if k >= warm:
                 times.append(time.perf_counter() - t0)
@@ -228,7 +235,8 @@ def cpu_baseline(sc, view, gpu_out, seconds, with_shadows=False):
             "ms_per_frame": med * 1e3, "single_thread_ms_per_frame": med1 * 1e3 if med1 else None,
             "single_thread_value": 1.0 / med1 if med1 else None,
             "host": {"cpu_model": model, "nproc": logical, "physical_cores": physical,
-                     "oracle_build": build}}, parity, dict(ref["stats"], shadow_samples=shadow_samples)
+                     "oracle_build": build}}, parity, dict(ref["stats"], shadow_samples=shadow_samples,
+                                                        shadow_texels_covered=covered)
 
 
 def animated_descriptors(objs, n, seed=5):
@@ -386,6 +394,15 @@ def run_plain(args):
         fr.render(g, view)
         hip_sync()
     st_iso = fr.stats()
+    ms_shadow_iso = 0.0
+    if shadows:
+        evi = [EventPair() for _ in range(max(5, min(args.steps, 20)))]
+        for e in evi:
+            hip_sync()
+            e.start()
+            g.shadow_pass()
+            e.stop()
+        ms_shadow_iso = float(np.mean([e.ms() for e in evi]))
 
     # algorithmic bytes of the shading launch
     tri = out["tri_id"]
@@ -449,8 +466,21 @@ def run_plain(args):
         batched = batched_leg(ctx, steps=10, warmup=2, autotune_reps=2)
         served = served_leg(ctx, n_req=400, rates=(1000, 2000), modes=("8x4", "1x1"))
     passes = {k: st[k] for k in ("ms_clear", "ms_prearrange", "ms_depth", "ms_id", "ms_shade", "ms_total")}
+    shadow = None
     if shadows:
         passes["ms_shadow"] = ms_shadow
+        Ls = int(sc.light_buffer_dim)
+        nl = int(np.sum(sc.lights["shadow"] == 1))
+        # algorithmic bytes of one pass: the triangle positions and object
+        # ids read once per group of lights, every map texel cleared once,
+        # one 4-byte write per texel that some triangle covers
+        groups = -(-nl // shadow_group_size(sc.ntri))
+        covered = ref_stats.get("shadow_texels_covered") if ref_stats else None
+        Bs = groups * sc.ntri * 40 + nl * 6 * Ls * Ls * 4 + (covered or 0) * 4
+        shadow = {"ms_isolated": ms_shadow_iso, "ms_in_stream": ms_shadow, "lights": nl, "groups": groups,
+                  "map_dim": Ls, "covered_texels": covered, "algorithmic_bytes": Bs,
+                  "achieved_gbs": Bs / (ms_shadow_iso * 1e-3) / 1e9,
+                  "frac_hbm": Bs / (ms_shadow_iso * 1e-3) / 1e9 / HBM_PEAK_GBS}
     passes_iso = {k: st_iso[k] for k in ("ms_clear", "ms_prearrange", "ms_depth", "ms_id", "ms_shade", "ms_total")}
     line = base_line(args, METRIC, fps, "frames/s", ms_per_step,
                      {"workload": workload, "width": W, "height": H, "triangles": sc.ntri,
@@ -469,6 +499,7 @@ def run_plain(args):
                      "isolated": {"ms": st_iso["ms_shade"], "achieved": achieved_iso,
                                   "frac": achieved_iso / HBM_PEAK_GBS}},
         "atomics": atomics,
+        "shadow_pass": shadow,
         "calibration": cal,
         "post_blend_with_depth": post,
         "animated": animated,
@@ -482,20 +513,11 @@ def run_plain(args):
     print(json.dumps(line), flush=True)
 
 
-def config4_workload():
-    from openclrenderer_amd import scenes
-    sizes = scenes.config4_scene_sizes()
-    scs = [scenes.small_scene(scenes.SEED_BASE + 40 + i, max(20, min(400, n // 5000)), n, n_lights=4,
-                              ntex=1, tex_size=1024, name=f"cfg4-{i}") for i, n in enumerate(sizes)]
-    views = scenes.config4_views()
-    return scs, views
-
-
 def batched_leg(ctx, steps, warmup, autotune_reps=3):
     """Config 4: the 64 views rendered as one autotuned, graph-replayed batch
     per step against the same views as 64 plain render sequences."""
-    from openclrenderer_amd import runtime
-    scs, views = config4_workload()
+    from openclrenderer_amd import runtime, scenes
+    scs, views = scenes.config4_workload()
     gs = [runtime.GpuScene(ctx, s) for s in scs]
     frames = [runtime.Frame(ctx, int(v["width"][0]), int(v["height"][0])) for _, v in views]
     owners = [gs[i] for i, _ in views]
@@ -557,37 +579,16 @@ def run_batched(args):
     print(json.dumps(line), flush=True)
 
 
-def config5_workload(seed=None):
-    from openclrenderer_amd import scenes
-    rng = scenes.SplitMix64(scenes.SEED_BASE + 5 if seed is None else seed)
-    scs = []
-    for i in range(12):
-        n = int(round(math.exp(math.log(50_000) + (math.log(4_000_000) - math.log(50_000)) * rng.uniform())))
-        nl = 1 + int(rng.uniform() * 16)
-        scs.append(scenes.small_scene(scenes.SEED_BASE + 50 + i, max(20, min(400, n // 5000)), n,
-                                      n_lights=nl, shadow_lights=nl, ntex=1, tex_size=1024, name=f"cfg5-{i}"))
-    return scs, rng
-
-
 def served_leg(ctx, n_req, rates, modes):
     """Config 5: Poisson arrivals of n_req requests per rate against 12
     resident scenes through a server in each mode ("VxF": at most V views per
     batch, F batches in flight; "1x1" renders one request at a time)."""
     from openclrenderer_amd import runtime, scenes
-    scs, rng = config5_workload()
+    scs, rng = scenes.config5_workload()
     gs = [runtime.GpuScene(ctx, s) for s in scs]
-    res = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
-    cum = np.cumsum([0.4, 0.3, 0.2, 0.1])
 
     def schedule(rate):
-        t, out = 0.0, []
-        for _ in range(n_req):
-            t += -math.log(max(rng.uniform(), 1e-12)) / rate
-            u = rng.uniform()
-            w, h = res[int(np.searchsorted(cum, u, side="right"))] if u < 1.0 else res[-1]
-            k = int(rng.uniform() * len(scs))
-            out.append((t, k, scenes.orbit_view(rng, w, h, 3 + 17 * rng.uniform())))
-        return out
+        return scenes.config5_requests(rng, n_req, rate, len(scs))
 
     def drive(srv, slots, sched):
         lat, qwait, batch = [], [], []

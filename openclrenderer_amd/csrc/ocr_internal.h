@@ -177,6 +177,7 @@ __host__ __device__ inline uint32_t view_tiles(uint32_t w, uint32_t h, uint32_t 
 enum Counter : int {
   kPacked = 0,
   kOverflow = 2,
+  kValidEnd = 16,          // shadow passes: u64, end of the list's valid prefix, byte 64
   kFragCount = 32,         // reference fragments (statistics), byte 128
   kCounterWords = 64,
 };
@@ -189,6 +190,18 @@ constexpr uint32_t kOverflowCuts = 1u, kOverflowSamples = 2u;
 constexpr uint32_t kMaxSamples = 0x7FFFFFFFu;
 constexpr uint32_t kMaxTriSamples = (1u << 24) - 1;
 constexpr uint64_t kMaxTargetPixels = 16000000ull;
+
+// Shadow-pass lists pack their counter as (records << 40 | samples): a
+// wavefront allocates at most 2^25 samples at once (a lane's two pieces
+// hold fewer than 2^25) and only if the counter it loads first is below
+// the limits, and at most 8192 wavefronts are resident, so the sample part
+// stays below 2^30 + 2^13 * 2^25 < 2^40 and the record part below
+// kShadowMaxRecords + 2^13 * 128 < 2^24.
+constexpr int kShadowSampleBits = 40;
+constexpr unsigned long long kShadowSampleMask = (1ull << kShadowSampleBits) - 1;
+constexpr uint32_t kShadowWaveSamples = 1u << 25;
+constexpr uint32_t kShadowMaxSamples = 1u << 30;
+constexpr uint32_t kShadowMaxRecords = 14u << 20;
 
 struct Ctx {
   int device = 0;
@@ -226,8 +239,8 @@ struct Scene {
   uint32_t* shadow_maps = nullptr;   // n_shadow x 6 x dim^2
   uint32_t* static_maps = nullptr;   // n_static x 6 x dim^2
   bool static_dirty = false;
-  // cut list of the shadow passes
-  CutTri* sh_cuts = nullptr;
+  // work list of the shadow passes (raster setups only)
+  RasterTri* sh_rast = nullptr;
   uint32_t* sh_counters = nullptr;
   uint32_t sh_cap_cut = 0;
   // recorded on the stream of the last shadow pass after its last launch;
@@ -396,8 +409,10 @@ DevScene dev_scene(const Scene* sc);
 int batch_reserve(ocr_batch* b, uint64_t sum_tris);
 DevView dev_view(const ocr_view* v, uint32_t scene, Frame* f);
 int device_cu_count();
-hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, uint32_t* map,
-                               hipStream_t s);
+// The cube maps of up to 16 lights in one prearrange and one raster launch:
+// light i at lpos[i] writes the 6 faces of map slots[i] of `maps`.
+hipError_t launch_shadow_group(Scene* sc, const float (*lpos)[3], const uint32_t* slots, uint32_t n,
+                               int only_static, uint32_t* maps, hipStream_t s);
 hipError_t launch_atlas_upload(const Atlas& a, uint32_t tid, const uint8_t* src, uint32_t w,
                                uint32_t h, hipStream_t s);
 hipError_t launch_mip(const Atlas& a, uint32_t src_id, uint32_t dst_id, uint32_t gw, uint32_t gh,

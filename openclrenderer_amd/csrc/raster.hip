@@ -21,7 +21,10 @@
 // LDS and sweeps the chunk with all lanes.  No per-fragment records exist.
 #pragma clang fp contract(off)
 
+#include <stdlib.h>
 #include <string.h>
+
+#include <algorithm>
 
 #include "ocr_internal.h"
 
@@ -303,7 +306,8 @@ __device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds<kWaves
   for (int i = 0; i < 2; i++) {
     if (!ns[i]) continue;
     const uint32_t key = key0 + key_step * (uint32_t)i;
-    store_cut<kCamera>(&L.cuts[c], tp[i], rc[i], key, oid, view);
+    // shadow records are read only through their raster setup
+    if constexpr (kCamera) store_cut<kCamera>(&L.cuts[c], tp[i], rc[i], key, oid, view);
     rt[i].key = key;
     rt[i].sample_begin = smp;
     L.rast[c] = rt[i];
@@ -458,25 +462,151 @@ struct TriLds {
   float ek[7][kWave];
   uint32_t key[kWave];
   uint32_t begin[kWave + 1];
-  uint32_t start[kWave];   // start[p] == base: a staged triangle begins at sample base + p
+  uint32_t start[2 * kWave];   // start[p] == base: a staged triangle begins at sample base + p
   uint32_t pitch[kBatch ? kWave : 1];
-  uint32_t* dst[kBatch || kMode == RasterMode::Shadow ? kWave : 1];
+  uint32_t* dst[kBatch ? kWave : 1];
   unsigned long long* ids[kBatch && kMode == RasterMode::Id ? kWave : 1];
 };
+
+// Sample `j` (index within its triangle, an exact binary32 integer) of a
+// set-up triangle: its column x, row y and whether it lies inside
+// (point_in_tri).  Sample j is sample o = j % (op + 1) of fragment
+// a = j / (op + 1), at walk index q = op*a + o; the quotient is a float
+// estimate corrected by one step each way.
+template <int kOp>
+__device__ __forceinline__ bool sample_xy(float slf, float min_x, float min_y, float w, float iw, float e0, float e1,
+                                          float e2, float e3, float e4, float e5, float e6, float& x, float& y) {
+  float faf = floorf(slf * (1.0f / (float)(kOp + 1)));
+  float of = fmaf(-faf, (float)(kOp + 1), slf);
+  faf = of < 0.f ? faf - 1.f : (of >= (float)(kOp + 1) ? faf + 1.f : faf);
+  of = fmaf(-faf, (float)(kOp + 1), slf);
+  const float paf = faf * (float)kOp;
+  const float qf = paf + of;
+  y = row_of(qf, iw, min_y);
+  // cursor column: (r % w) + min_x + (q - r), r = max(pa, row start)
+  const float rs = run_start(y, w, iw, min_y);
+  const float r = paf > rs ? paf : rs;
+  const float m = floorf(r * iw);
+  float e = fmaf(-m, w, r);
+  e = e < 0.f ? e + w : e;
+  e = e >= w ? e - w : e;
+  const float xo = e + (qf - r);
+  x = min_x + xo;
+  if (xo >= w) return false;   // x >= max_x
+  const float s_ = e0 + e1 * x + e2 * y;
+  const float t_ = e3 + e4 * x + e5 * y;
+  return s_ > -0.0001f && t_ > -0.0001f && (s_ + t_) < e6;
+}
+
+// Every sample of [s, stage_end) of the staged triangles with all lanes of
+// the wavefront, two samples per lane per step (called wave-uniformly).
+// my_begin is the first sample of the lane's staged triangle (stage_end or
+// more for lanes without one); staged triangles hold at least one sample
+// each, in lane order, and L.start holds no value in [s, stage_end) on
+// entry.  A single view's target (width, depth, id keys) comes in view_*;
+// shadow maps come in view_depth.
+template <RasterMode kMode, int kOp, bool kBatch>
+__device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t view_width, uint32_t* view_depth,
+                                              unsigned long long* view_ids, uint32_t shadow_dim, uint32_t my_begin,
+                                              uint32_t s, uint32_t stage_end, uint32_t c0) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (uint32_t base = s; base < stage_end; base += 2 * kWave) {
+    // Lane p takes samples j = base + p + 64h (h = 0, 1).  Its staged
+    // triangle is one less than the number of staged triangles beginning at
+    // or before j: those beginning at or before base + 64h (a prefix of the
+    // lanes, one ballot) plus those beginning in (base + 64h, j] (flags
+    // written by the triangles' lanes into start[], counted below p with
+    // mbcnt).  Offset 64 is left unflagged: a triangle beginning there is
+    // in the second half's prefix.
+    const unsigned long long before0 = __ballot(my_begin <= base);
+    const unsigned long long before1 = __ballot(my_begin <= base + kWave);
+    const uint32_t off = my_begin - base;
+    if (my_begin > base && off < 2u * kWave && off != (uint32_t)kWave) L.start[off] = base;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t d[2], pix[2];
+    uint32_t* dst[2];
+    int k[2];
+    bool live[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const bool starts_here = L.start[h * kWave + lane] == base;
+      const unsigned long long starts = __ballot(starts_here);
+      const uint32_t j = base + (uint32_t)(h * kWave + lane);
+      k[h] = __popcll(h ? before1 : before0) - 1 +
+             (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
+             (starts_here ? 1 : 0);
+      live[h] = false;
+      d[h] = pix[h] = 0u;
+      dst[h] = nullptr;
+      if (j < stage_end) {
+        const int kk = k[h];
+        float x, y;
+        live[h] = sample_xy<kOp>((float)(j - L.begin[kk]), L.min_x[kk], L.min_y[kk], L.w[kk], L.iw[kk],
+                                 L.ek[0][kk], L.ek[1][kk], L.ek[2][kk], L.ek[3][kk], L.ek[4][kk], L.ek[5][kk],
+                                 L.ek[6][kk], x, y);
+        if (live[h]) {
+          d[h] = depth_quant(fmaf(L.pa[kk], x, fmaf(L.pb[kk], y, L.pc[kk])));
+          uint32_t pitch;
+          if constexpr (kMode == RasterMode::Shadow) {
+            // view_depth: the maps, 6 faces of shadow_dim^2 per light
+            pitch = shadow_dim;
+            dst[h] = view_depth + (size_t)L.key[kk] * shadow_dim * shadow_dim;
+          } else if constexpr (kBatch) {
+            pitch = L.pitch[kk];
+            dst[h] = L.dst[kk];
+          } else {
+            pitch = view_width;
+            dst[h] = view_depth;
+          }
+          pix[h] = (uint32_t)(int)y * pitch + (uint32_t)(int)x;
+        }
+      }
+    }
+    // both samples' loads before either test: one memory latency per step
+    uint32_t cur[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) cur[h] = live[h] ? dst[h][pix[h]] : 0u;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      if (!live[h]) continue;
+      if constexpr (kMode == RasterMode::Id) {
+        unsigned long long* ids = kBatch ? L.ids[kBatch ? k[h] : 0] : view_ids;
+        if (d[h] > cur[h] - kBufError && d[h] < cur[h] + kBufError)
+          atomicMax(&ids[pix[h]], ((unsigned long long)(L.key[k[h]] + 1u) << 32) | (c0 + (uint32_t)k[h]));
+      } else {
+        // The L2 performs atomics at ~26.6 G cache-line operations/s; a
+        // plain load first drops the samples that cannot win (the stored
+        // depth only decreases, so d >= any value a load returns loses).
+        if (d[h] < cur[h]) atomicMin(&dst[h][pix[h]], d[h]);
+      }
+    }
+  }
+}
 
 template <RasterMode kMode, int kOp, bool kBatch>
 __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
   __shared__ TriLds<kMode, kBatch> lds_all[256 / kWave];
   const Lists& Ls = a.lists;
   if (Ls.counters[kOverflow]) return;
-  const unsigned long long packed = *reinterpret_cast<const unsigned long long*>(Ls.counters + kPacked);
-  const uint32_t ncut = (uint32_t)(packed >> 32);
-  const uint32_t total = (uint32_t)packed;
+  unsigned long long packed = *reinterpret_cast<const unsigned long long*>(Ls.counters + kPacked);
+  uint32_t ncut, total;
+  if constexpr (kMode == RasterMode::Shadow) {
+    // (records << 40 | samples), up to the valid prefix
+    packed = min(packed, *reinterpret_cast<const unsigned long long*>(Ls.counters + kValidEnd));
+    ncut = (uint32_t)(packed >> kShadowSampleBits);
+    total = (uint32_t)(packed & kShadowSampleMask);
+  } else {
+    ncut = (uint32_t)(packed >> 32);
+    total = (uint32_t)packed;
+  }
   const int lane = threadIdx.x & (kWave - 1);
   TriLds<kMode, kBatch>& L = lds_all[threadIdx.x / kWave];
   const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  L.start[lane] = 0xFFFFFFFFu;   // never a sample index (total < 2^31)
+  L.start[lane] = L.start[kWave + lane] = 0xFFFFFFFFu;   // never a sample index (total < 2^31)
   for (uint32_t s0 = wave0 * kChunk; s0 < total; s0 += nwaves * kChunk) {
     const uint32_t s1 = min(s0 + kChunk, total);
     // ---- the chunk's first triangle: last record with sample_begin <= s0
@@ -511,9 +641,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
         for (int e = 0; e < 7; e++) L.ek[e][lane] = t.ek[e];
         L.key[lane] = t.key;
         L.begin[lane] = t.sample_begin;
-        if constexpr (kMode == RasterMode::Shadow) {
-          L.dst[lane] = sh.map + (size_t)t.key * sh.L * sh.L;
-        } else if constexpr (kBatch) {
+        if constexpr (kBatch) {
           const DevView& V = a.views[Ls.cuts[ci].view];
           L.dst[lane] = V.depth;
           L.pitch[lane] = V.v.width;
@@ -534,77 +662,8 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const uint32_t stage_end = min(s1, L.begin[kWave]);
       // ---- every sample of [s, stage_end) with all lanes
-      for (uint32_t base = s; base < stage_end; base += kWave) {
-        // Lane p takes sample j = base + p.  Its staged triangle k is one less
-        // than the number of staged triangles beginning at or before j: those
-        // beginning at or before `base` (a prefix of the lanes, one ballot)
-        // plus those beginning in (base, j] (flags written by the triangles'
-        // lanes, counted below p with mbcnt).
-        const unsigned long long before = __ballot(my_begin <= base);
-        const uint32_t off = my_begin - base;
-        if (my_begin > base && off < (uint32_t)kWave) L.start[off] = base;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const bool starts_here = L.start[lane] == base;
-        const unsigned long long starts = __ballot(starts_here);
-        const uint32_t j = base + (uint32_t)lane;
-        if (j >= stage_end) continue;
-        const int k = __popcll(before) - 1 +
-                      (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
-                      (starts_here ? 1 : 0);
-        // sample -> (fragment a, walk index q), pa = op * a, in exact binary32
-        // integers (a float quotient estimate corrected by one step each way)
-        const float slf = (float)(j - L.begin[k]);
-        float faf = floorf(slf * (1.0f / (float)(kOp + 1)));
-        float of = fmaf(-faf, (float)(kOp + 1), slf);
-        faf = of < 0.f ? faf - 1.f : (of >= (float)(kOp + 1) ? faf + 1.f : faf);
-        of = fmaf(-faf, (float)(kOp + 1), slf);
-        const float paf = faf * (float)kOp;
-        const float qf = paf + of;
-        const float w = L.w[k], iw = L.iw[k], min_y = L.min_y[k];
-        const float y = row_of(qf, iw, min_y);
-        // cursor column: (r % w) + min_x + (q - r), r = max(pa, row start)
-        const float rs = run_start(y, w, iw, min_y);
-        const float r = paf > rs ? paf : rs;
-        const float m = floorf(r * iw);
-        float e = fmaf(-m, w, r);
-        e = e < 0.f ? e + w : e;
-        e = e >= w ? e - w : e;
-        const float xo = e + (qf - r);
-        if (xo >= w) continue;   // x >= max_x
-        const float x = L.min_x[k] + xo;
-        const float s_ = L.ek[0][k] + L.ek[1][k] * x + L.ek[2][k] * y;
-        const float t_ = L.ek[3][k] + L.ek[4][k] * x + L.ek[5][k] * y;
-        if (!(s_ > -0.0001f && t_ > -0.0001f && (s_ + t_) < L.ek[6][k])) continue;
-        const float fd = fmaf(L.pa[k], x, fmaf(L.pb[k], y, L.pc[k]));
-        const uint32_t d = depth_quant(fd);
-        uint32_t pitch;
-        uint32_t* dst;
-        if constexpr (kMode == RasterMode::Shadow) {
-          pitch = sh.L;
-          dst = L.dst[k];
-        } else if constexpr (kBatch) {
-          pitch = L.pitch[k];
-          dst = L.dst[k];
-        } else {
-          pitch = a.view0.v.width;
-          dst = a.view0.depth;
-        }
-        const size_t pix = (size_t)((uint32_t)(int)y * pitch + (uint32_t)(int)x);
-        if constexpr (kMode == RasterMode::Id) {
-          unsigned long long* ids = kBatch ? L.ids[kBatch ? k : 0] : a.view0.idkey;
-          const uint32_t val = dst[pix];
-          if (d > val - kBufError && d < val + kBufError)
-            atomicMax(&ids[pix], ((unsigned long long)(L.key[k] + 1u) << 32) | (c0 + (uint32_t)k));
-        } else {
-          // The L2 performs atomics at ~26.6 G cache-line operations/s; a
-          // plain load first drops the samples that cannot win (the stored
-          // depth only decreases, so d >= any value a load returns loses).
-          if (d < dst[pix]) atomicMin(&dst[pix], d);
-        }
-      }
+      sweep_samples<kMode, kOp, kBatch>(L, a.view0.v.width, kMode == RasterMode::Shadow ? sh.map : a.view0.depth,
+                                        a.view0.idkey, sh.L, my_begin, s, stage_end, c0);
       __builtin_amdgcn_wave_barrier();
       s = stage_end;
       c0 += kWave;
@@ -638,91 +697,223 @@ hipError_t launch_id(const PassArgs& a, int blocks_per_cu, hipStream_t s) {
 // ---------------------------------------------------------------- shadows
 struct FaceTable { SinCos f[6]; };
 
-// prearrange_realtime_shadowing (cl2.cl:4419-4636) for one light.  Phase 1:
-// one triangle per work item — its world-space vertices, the faces they fall
-// in (ret_cubeface), each (triangle, face) pair appended to a block-local
-// work list in LDS.  Phase 2: the pairs, one per work item per
-// round, clipped, projected, culled and set up for their face, with one
-// record allocation per round.  (Looping over the six faces per triangle
-// would leave most lanes idle: a triangle touches one to three faces.)
+// prearrange_realtime_shadowing (cl2.cl:4419-4636) and kernel1's shadow
+// raster (cl2.cl:4638-4700) for a group of lights in one launch.  Phase 1:
+// one triangle per work item — its world-space vertices into LDS, read from
+// HBM once for the whole group.  Then for each light of the group: the faces
+// the vertices fall in (ret_cubeface), each (triangle, face) pair appended
+// to a block-local work list in LDS, and phase 2: the pairs, one per work
+// item per round, clipped, projected, culled and set up for their face.
+// Pieces become records, key = slot * 6 + face with slot the light's cube
+// map in the scene's map array, for one raster launch over the whole group;
+// pieces of at most local_max samples, and any that no longer fit the list,
+// are rasterised at once by the wavefront, their samples spread over its
+// lanes.  (Looping over the six faces per
+// triangle would leave most lanes idle: a triangle touches one to three
+// faces.)
+constexpr int kMaxShadowGroup = 16;
+struct ShadowGroup {
+  float pos[kMaxShadowGroup][3];
+  uint32_t slot[kMaxShadowGroup];
+  uint32_t n;
+};
+
+// Shadow-pass allocation for the lanes with `take` set, per wavefront: one
+// atomicAdd on the packed counter.  An allocation that passes the list
+// capacity or kShadowMaxSamples is not written: the wavefront rasterises
+// those pieces itself and lowers the valid-prefix word to its start, and
+// every later allocation fails too (the counter only grows), so the raster
+// launch reads the list up to min(counter, valid prefix).
+__device__ __forceinline__ bool wave_alloc_fit(const Lists& L, uint32_t ncut, uint32_t nsmp, uint32_t& cut0,
+                                               uint32_t& smp0) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t ic = wave_incl_scan(ncut), is = wave_incl_scan(nsmp);
+  const uint32_t tc = __shfl(ic, kWave - 1, kWave), ts = __shfl(is, kWave - 1, kWave);
+  cut0 = smp0 = 0;
+  if (tc == 0) return true;
+  uint32_t base_lo = 0, base_hi = 0, ok = 0;
+  if (lane == 0) {
+    unsigned long long* p = reinterpret_cast<unsigned long long*>(L.counters + kPacked);
+    const unsigned long long seen = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((seen >> kShadowSampleBits) < L.cap_cut && (seen & kShadowSampleMask) < kShadowMaxSamples) {
+      const unsigned long long old = atomicAdd(p, ((unsigned long long)tc << kShadowSampleBits) + ts);
+      ok = (old >> kShadowSampleBits) + tc <= L.cap_cut && (old & kShadowSampleMask) + ts <= kShadowMaxSamples;
+      if (!ok) atomicMin(reinterpret_cast<unsigned long long*>(L.counters + kValidEnd), old);
+      base_lo = (uint32_t)(old & kShadowSampleMask);   // below 2^30 when ok
+      base_hi = (uint32_t)(old >> kShadowSampleBits);
+    }
+  }
+  ok = __shfl(ok, 0, kWave);
+  cut0 = __shfl(base_hi, 0, kWave) + ic - ncut;
+  smp0 = __shfl(base_lo, 0, kWave) + is - nsmp;
+  return ok != 0;
+}
+
+using ShadowStage = TriLds<RasterMode::Shadow, false>;
+
+// Rasterises the piece each lane holds (n samples, 0 for none) with the
+// whole wavefront, straight into the maps (called wave-uniformly).  A map
+// keeps the minimum depth of its samples, so the order in which samples
+// arrive does not matter.
+__device__ __forceinline__ void local_sweep(ShadowStage& W, const RasterTri& rt, uint32_t n, uint32_t key,
+                                            const ShadowTarget& sh) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t incl = wave_incl_scan(n);   // n < 2^24 per lane: the sum fits 32 bits
+  const uint32_t total = __shfl(incl, kWave - 1, kWave);
+  if (total == 0) return;
+  const unsigned long long m = __ballot(n > 0);
+  const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  const uint32_t nloc = (uint32_t)__popcll(m);
+  W.start[lane] = W.start[kWave + lane] = 0xFFFFFFFFu;   // sample indices of this sweep are below 2^31
+  if (n) {
+    W.min_x[pos] = rt.min_x;
+    W.min_y[pos] = rt.min_y;
+    W.w[pos] = rt.w;
+    W.iw[pos] = rt.iw;
+    W.pa[pos] = rt.A;
+    W.pb[pos] = rt.B;
+    W.pc[pos] = rt.C;
+#pragma unroll
+    for (int e = 0; e < 7; e++) W.ek[e][pos] = rt.ek[e];
+    W.key[pos] = key;
+    W.begin[pos] = incl - n;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint32_t my_begin = (uint32_t)lane < nloc ? W.begin[lane] : total;
+  sweep_samples<RasterMode::Shadow, kOpSizeLight, false>(W, 0u, sh.map, nullptr, sh.L, my_begin, 0u, total, 0u);
+  // the stage is rewritten by the wavefront's next sweep
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct ShadowLds {
   float wpos[kShadowPreThreads][9];
-  uint32_t oid[kShadowPreThreads];
-  uint32_t items[kShadowPreThreads * 3];   // local triangle << 4 | two_sided << 3 | face
-  uint32_t n_items;
+  // (local triangle << 4 | two_sided << 3 | face) work items, one list per
+  // light, two lists alternating
+  uint32_t items[2][kShadowPreThreads * 3];
+  uint32_t n_items[kMaxShadowGroup];
 };
 
 __global__ void __launch_bounds__(kShadowPreThreads) k_prearrange_shadow(const TriPos* __restrict__ world, uint32_t ntri,
                                                            const ObjDesc* __restrict__ objs, uint32_t nobj,
-                                                           float lx, float ly, float lz, uint32_t L,
-                                                           int only_static, FaceTable ft, Lists lists) {
-  __shared__ BlockAllocLds<kShadowPreThreads / kWave> sh;
+                                                           ShadowGroup group, ShadowTarget target,
+                                                           int only_static, uint32_t local_max, FaceTable ft,
+                                                           Lists lists) {
   __shared__ ShadowLds S;
+  __shared__ ShadowStage stage_all[kShadowPreThreads / kWave];
+  ShadowStage& W = stage_all[threadIdx.x / kWave];
   const uint32_t t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const uint32_t wave = t / kWave;
   const uint32_t id = blockIdx.x * blockDim.x + t;
+  const uint32_t L = target.L;
   const float ew = (float)L, eh = (float)L, efov = (float)L / 2.0f;
-  const f3 lpos = mk3(lx, ly, lz);
-  if (t == 0) S.n_items = 0;
-  __syncthreads();
-  // ---- phase 1
-  uint32_t touched = 0, two_sided = 0;
+  // ---- phase 1: the triangle, once for every light of the group
+  bool live = false;
+  uint32_t two_sided = 0;
+  f3 wp = mk3(0.f, 0.f, 0.f);
   if (id < ntri) {
     const TriPos T = world[id];
     if (T.oid < nobj) {
       const ObjDesc G = objs[T.oid];
       const bool is_static = (G.flags & OCR_FEATURE_IS_STATIC) > 0;
-      bool live = only_static ? is_static : !is_static;
-      const f3 wp = mk3(G.wp[0], G.wp[1], G.wp[2]);
-      if (live && fast_length(wp - lpos) > kDepthFar) live = false;
+      live = only_static ? is_static : !is_static;
+      wp = mk3(G.wp[0], G.wp[1], G.wp[2]);
       if (live) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-          const f3 r = mk3(T.p[3 * k], T.p[3 * k + 1], T.p[3 * k + 2]);
-          S.wpos[t][3 * k] = r.x;
-          S.wpos[t][3 * k + 1] = r.y;
-          S.wpos[t][3 * k + 2] = r.z;
-          touched |= 1u << cubeface(r, lpos);
-        }
-        S.oid[t] = T.oid;
+        for (int k = 0; k < 9; k++) S.wpos[t][k] = T.p[k];
         two_sided = (G.flags & OCR_FEATURE_TWO_SIDED) > 0 ? 1u : 0u;
       }
     }
   }
-  const uint32_t mine = (uint32_t)__popc(touched);
-  const uint32_t incl = wave_incl_scan(mine);
-  const uint32_t wave_total = __shfl(incl, kWave - 1, kWave);
-  uint32_t wbase = 0;
-  if ((t & (kWave - 1)) == 0 && wave_total) wbase = atomicAdd(&S.n_items, wave_total);
-  wbase = __shfl(wbase, 0, kWave);
-  uint32_t slot = wbase + incl - mine;
-  for (uint32_t f = 0; f < 6; f++)
-    if ((touched >> f) & 1u) S.items[slot++] = t << 4 | two_sided << 3 | f;
+  if (t < (uint32_t)kMaxShadowGroup) S.n_items[t] = 0;
   __syncthreads();
-  const uint32_t n_items = S.n_items;
-  // ---- phase 2: same rounds for every work item of the block
-  for (uint32_t r0 = 0; r0 < n_items; r0 += kShadowPreThreads) {
-    f3 tp[2][3];
-    float rc[2] = {0.f, 0.f};
-    RasterTri rt[2];
-    uint32_t nf[2] = {0u, 0u}, ns[2] = {0u, 0u};
-    uint32_t face = 0, oid = 0;
-    if (r0 + t < n_items) {
-      const uint32_t it = S.items[r0 + t];
-      const uint32_t lt = it >> 4;
-      face = it & 7u;
-      oid = S.oid[lt];
-      f3 w[3];
+  for (uint32_t li = 0; li < group.n; li++) {
+    const f3 lpos = mk3(group.pos[li][0], group.pos[li][1], group.pos[li][2]);
+    const uint32_t key0 = group.slot[li] * 6u;
+    uint32_t* const items = S.items[li & 1u];
+    // Light li's items go to list li & 1.  Every wavefront has finished
+    // light li - 2's rounds on that list before it passed light li - 1's
+    // barrier, and writes here come after that barrier.
+    uint32_t touched = 0;
+    if (live && !(fast_length(wp - lpos) > kDepthFar)) {
 #pragma unroll
-      for (int k = 0; k < 3; k++) w[k] = mk3(S.wpos[lt][3 * k], S.wpos[lt][3 * k + 1], S.wpos[lt][3 * k + 2]);
-      const int num = face_clip_project(w, lpos, ft.f[face], efov, ew, eh, tp);
-      cut_pieces(tp, num, ew, eh, ((it >> 3) & 1u) != 0u, kOpSizeLight, rc, rt, nf, ns);
+      for (int k = 0; k < 3; k++)
+        touched |= 1u << cubeface(mk3(S.wpos[t][3 * k], S.wpos[t][3 * k + 1], S.wpos[t][3 * k + 2]), lpos);
     }
-    append_cuts<false>(lists, sh, tp, rc, rt, nf, ns, face, 0u, oid, 0u);
+    const uint32_t mine = (uint32_t)__popc(touched);
+    const uint32_t incl = wave_incl_scan(mine);
+    const uint32_t wave_total = __shfl(incl, kWave - 1, kWave);
+    uint32_t wbase = 0;
+    if (lane == 0 && wave_total) wbase = atomicAdd(&S.n_items[li], wave_total);
+    wbase = __shfl(wbase, 0, kWave);
+    uint32_t slot = wbase + incl - mine;
+    for (uint32_t f = 0; f < 6; f++)
+      if ((touched >> f) & 1u) items[slot++] = t << 4 | two_sided << 3 | f;
+    __syncthreads();
+    const uint32_t n_items = S.n_items[li];
+    // ---- phase 2: 64 work items per wavefront per round, the wavefronts
+    // independent of each other
+    for (uint32_t r0 = wave * kWave; r0 < n_items; r0 += kShadowPreThreads) {
+      f3 tp[2][3];
+      float rc[2] = {0.f, 0.f};
+      RasterTri rt[2];
+      uint32_t nf[2] = {0u, 0u}, ns[2] = {0u, 0u};
+      uint32_t face = 0;
+      if (r0 + lane < n_items) {
+        const uint32_t it = items[r0 + lane];
+        const uint32_t lt = it >> 4;
+        face = it & 7u;
+        f3 w[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) w[k] = mk3(S.wpos[lt][3 * k], S.wpos[lt][3 * k + 1], S.wpos[lt][3 * k + 2]);
+        const int num = face_clip_project(w, lpos, ft.f[face], efov, ew, eh, tp);
+        cut_pieces(tp, num, ew, eh, ((it >> 3) & 1u) != 0u, kOpSizeLight, rc, rt, nf, ns);
+      }
+      const uint32_t key = key0 + face;
+      // pieces above local_max samples to the list, in allocations of at
+      // most kShadowWaveSamples samples (each takes the lanes whose running
+      // sum fits, at least the first waiting lane); pieces of the list's
+      // failed allocations and the small ones rasterised here
+      const bool big0 = ns[0] > local_max, big1 = ns[1] > local_max;
+      const uint32_t mine = (big0 ? ns[0] : 0u) + (big1 ? ns[1] : 0u);   // < 2^25
+      bool waiting = mine > 0, fit = false;
+      while (__ballot(waiting)) {
+        const bool take = waiting && wave_incl_scan(waiting ? mine : 0u) <= kShadowWaveSamples;
+        uint32_t c, smp;
+        const bool ok = wave_alloc_fit(lists, take ? (big0 ? 1u : 0u) + (big1 ? 1u : 0u) : 0u, take ? mine : 0u,
+                                       c, smp);
+        if (take) {
+          waiting = false;
+          fit = ok;
+          if (ok) {
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+              if (!(i == 0 ? big0 : big1)) continue;
+              rt[i].key = key;
+              rt[i].sample_begin = smp;
+              lists.rast[c] = rt[i];
+              smp += ns[i];
+              c++;
+            }
+          }
+        }
+      }
+      uint32_t nl[2];
+      nl[0] = fit && big0 ? 0u : ns[0];
+      nl[1] = fit && big1 ? 0u : ns[1];
+      local_sweep(W, rt[0], nl[0], key, target);
+      local_sweep(W, rt[1], nl[1], key, target);
+    }
   }
 }
 
 __global__ void k_zero_counters(uint32_t* c) {
-  if (threadIdx.x < kCounterWords) c[threadIdx.x] = 0u;
+  if (threadIdx.x < kCounterWords)
+    c[threadIdx.x] = threadIdx.x == kValidEnd || threadIdx.x == kValidEnd + 1 ? 0xFFFFFFFFu : 0u;
 }
 
 hipError_t launch_zero_counters(uint32_t* counters, hipStream_t s) {
@@ -730,22 +921,47 @@ hipError_t launch_zero_counters(uint32_t* counters, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_shadow_light(Scene* sc, const float lpos[3], int only_static, uint32_t* map,
-                               hipStream_t s) {
+// Tuning of the shadow launches, read at every pass: pieces of at most
+// OCR_SHADOW_LOCAL_SAMPLES samples (default 32) are rasterised by the
+// prearrange launch itself, larger ones go to the list for the chip-wide
+// raster launch; OCR_SHADOW_LIST_RECORDS lowers the list's capacity (the
+// pieces that do not fit are rasterised by the prearrange launch too).
+// Neither changes the maps, only where their samples are rasterised.
+static uint32_t env_u32(const char* name, uint32_t dflt) {
+  const char* e = getenv(name);
+  if (!e || !*e) return dflt;
+  char* end = nullptr;
+  const unsigned long v = strtoul(e, &end, 10);
+  return end && *end == 0 && v <= 0xFFFFFFFFul ? (uint32_t)v : dflt;
+}
+constexpr uint32_t kShadowLocalSamples = 32;
+
+hipError_t launch_shadow_group(Scene* sc, const float (*lpos)[3], const uint32_t* slots, uint32_t n,
+                               int only_static, uint32_t* maps, hipStream_t s) {
+  if (n == 0 || n > (uint32_t)kMaxShadowGroup) return hipErrorInvalidValue;
   FaceTable ft;
   for (int i = 0; i < 6; i++) ft.f[i] = sc->ctx->face_sc[i];
-  const Lists lists = make_lists(sc->sh_cuts, sc->sh_counters, sc->sh_cap_cut, false);
+  ShadowGroup g;
+  memset(&g, 0, sizeof(g));
+  for (uint32_t i = 0; i < n; i++) {
+    for (int k = 0; k < 3; k++) g.pos[i][k] = lpos[i][k];
+    g.slot[i] = slots[i];
+  }
+  g.n = n;
+  // raster setups only: shadow records have no CutTri
+  Lists lists{nullptr, sc->sh_rast, sc->sh_counters,
+              std::min(sc->sh_cap_cut, env_u32("OCR_SHADOW_LIST_RECORDS", sc->sh_cap_cut)), 0u};
+  const ShadowTarget target{maps, sc->lbuf_dim};
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, sc->sh_counters);
   if (sc->ntri)
     hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + kShadowPreThreads - 1) / kShadowPreThreads),
-                       dim3(kShadowPreThreads), 0, s, sc->world,
-                       sc->ntri, sc->objs, sc->nobj, lpos[0], lpos[1], lpos[2], sc->lbuf_dim, only_static,
-                       ft, lists);
+                       dim3(kShadowPreThreads), 0, s, sc->world, sc->ntri, sc->objs, sc->nobj, g, target,
+                       only_static, env_u32("OCR_SHADOW_LOCAL_SAMPLES", kShadowLocalSamples), ft, lists);
   PassArgs a;
   memset(&a, 0, sizeof(a));
   a.lists = lists;
   hipLaunchKernelGGL((k_raster<RasterMode::Shadow, kOpSizeLight, false>), dim3(device_cu_count() * 8), dim3(256), 0, s,
-                     a, ShadowTarget{map, sc->lbuf_dim});
+                     a, target);
   return hipGetLastError();
 }
 

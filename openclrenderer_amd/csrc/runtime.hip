@@ -418,7 +418,7 @@ void ocr_scene_destroy(ocr_scene* s) {
   for (int v = 0; v < 2; v++) dev_free(s->maps_v[v]);
   s->shadow_maps = nullptr;
   dev_free(s->static_maps);
-  dev_free(s->sh_cuts);
+  dev_free(s->sh_rast);
   dev_free(s->sh_counters);
   if (s->written) (void)hipEventDestroy(s->written);
   if (s->geom_written) (void)hipEventDestroy(s->geom_written);
@@ -631,12 +631,35 @@ int ocr_lights_upload(ocr_scene* s, const ocr_light* lights, uint32_t n, uint32_
 }
 
 static int ensure_shadow_lists(ocr_scene* s, uint32_t cap_cut) {
-  if (s->sh_cuts && s->sh_cap_cut >= cap_cut) return OCR_OK;
-  dev_free(s->sh_cuts);
+  if (s->sh_rast && s->sh_cap_cut >= cap_cut) return OCR_OK;
+  dev_free(s->sh_rast);
   s->sh_cap_cut = 0;
-  int rc = dev_alloc(&s->sh_cuts, (size_t)cap_cut * 2, "shadow cut list");
+  int rc = dev_alloc(&s->sh_rast, (size_t)cap_cut, "shadow work list");
   if (rc) return rc;
   s->sh_cap_cut = cap_cut;
+  return OCR_OK;
+}
+
+// Lights per shadow prearrange launch.  The launch rasterises small pieces
+// itself and lists the large ones while the list has room (then rasterises
+// the rest too), so any list size is correct; the list holds up to 6
+// records per triangle per light, at most kShadowMaxRecords (896 MB).
+constexpr uint32_t kShadowGroup = 16;
+static uint32_t shadow_list_cap(uint32_t ntri) {
+  return (uint32_t)std::min<uint64_t>(((uint64_t)ntri * 6 + 64) * kShadowGroup, kShadowMaxRecords);
+}
+
+// Renders the maps of `n` lights (positions, map slots) in groups.
+static int shadow_groups(ocr_scene* s, const std::vector<const ocr_light*>& ls, const std::vector<uint32_t>& slots,
+                         int only_static, uint32_t* maps, hipStream_t st) {
+  const uint32_t g = kShadowGroup;
+  for (size_t b = 0; b < ls.size(); b += g) {
+    const uint32_t n = (uint32_t)std::min<size_t>(g, ls.size() - b);
+    float pos[16][3];
+    for (uint32_t i = 0; i < n; i++)
+      for (int k = 0; k < 3; k++) pos[i][k] = ls[b + i]->pos[k];
+    OCR_HIP_CHECK(launch_shadow_group(s, pos, &slots[b], n, only_static, maps, st));
+  }
   return OCR_OK;
 }
 
@@ -663,30 +686,38 @@ int ocr_shadow_pass(ocr_scene* s, void* stream) {
   if (s->static_dirty && s->n_static) OCR_HIP_CHECK(s->wait_readers(s->map_readers[s->mver], st));
   OCR_HIP_CHECK(s->wait_geometry(st));   // descriptor updates enqueued on another stream
   {
-    // A triangle reaches the faces its three vertices fall in, each as at
-    // most 2 clipped pieces, so 6 records per triangle always fit.
-    uint64_t cut = std::min<uint64_t>((uint64_t)s->ntri * 6 + 64, 0xFFFFFFF0ull);
-    int rc = ensure_shadow_lists(s, (uint32_t)cut);
+    int rc = ensure_shadow_lists(s, shadow_list_cap(s->ntri));
     if (rc) return rc;
   }
   uint32_t* const maps = s->maps_v[nm];
   // engine.cpp:1613-1625: clear every dynamic map, then render each
   // shadow-casting light from the dynamic geometry; static lights' maps are
-  // rendered from static geometry when the lights were (re)uploaded.
+  // rendered from static geometry when the lights were (re)uploaded.  Lights
+  // are rendered in groups, one prearrange and one raster launch per group.
   OCR_HIP_CHECK(launch_fill_u32(maps, 0xFFFFFFFFu, per * std::max<uint32_t>(s->n_shadow, 1), st));
+  std::vector<const ocr_light*> dyn, stat;
+  std::vector<uint32_t> dyn_slot, stat_slot;
   uint32_t nn = 0, kk = 0;
   for (uint32_t i = 0; i < s->nlights; i++) {
     const ocr_light& l = s->host_lights[i];
     if (l.shadow == 1) {
-      OCR_HIP_CHECK(launch_shadow_light(s, l.pos, 0, maps + per * nn, st));
-      nn++;
+      dyn.push_back(&l);
+      dyn_slot.push_back(nn++);
     }
-    if (l.shadow && l.is_static && s->static_dirty) {
-      OCR_HIP_CHECK(launch_fill_u32(s->static_maps + per * kk, 0xFFFFFFFFu, per, st));
-      OCR_HIP_CHECK(launch_shadow_light(s, l.pos, 1, s->static_maps + per * kk, st));
+    if (l.shadow && l.is_static) {
+      if (s->static_dirty) {
+        stat.push_back(&l);
+        stat_slot.push_back(kk);
+      }
+      kk++;
     }
-    if (l.shadow && l.is_static) kk++;
   }
+  int rc = shadow_groups(s, dyn, dyn_slot, 0, maps, st);
+  if (!rc && !stat.empty()) {
+    OCR_HIP_CHECK(launch_fill_u32(s->static_maps, 0xFFFFFFFFu, per * s->n_static, st));
+    rc = shadow_groups(s, stat, stat_slot, 1, s->static_maps, st);
+  }
+  if (rc) return rc;
   s->static_dirty = false;
   OCR_HIP_CHECK(s->note_read(st));
   OCR_HIP_CHECK(hipEventRecord(s->written, st));

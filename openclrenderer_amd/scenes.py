@@ -455,6 +455,49 @@ def config4_scene_sizes(n_scenes=8, seed=SEED_BASE + 4):
             for _ in range(n_scenes)]
 
 
+def config4_workload():
+    """The batched workload: 8 resident scenes of log-uniform 50k-2M
+    triangles (4 lights, one 1024^2 texture each) and the 64 views over them
+    ((scene index, view) pairs)."""
+    sizes = config4_scene_sizes()
+    scs = [small_scene(SEED_BASE + 40 + i, max(20, min(400, n // 5000)), n, n_lights=4, ntex=1, tex_size=1024,
+                       name=f"cfg4-{i}") for i, n in enumerate(sizes)]
+    return scs, config4_views()
+
+
+def config5_workload(seed=None):
+    """The served workload: 12 resident scenes of log-uniform 50k-4M
+    triangles, each with 1-16 lights that all cast shadows, and the seeded
+    generator the request stream continues from."""
+    rng = SplitMix64(SEED_BASE + 5 if seed is None else seed)
+    scs = []
+    for i in range(12):
+        n = int(round(math.exp(math.log(50_000) + (math.log(4_000_000) - math.log(50_000)) * rng.uniform())))
+        nl = 1 + int(rng.uniform() * 16)
+        scs.append(small_scene(SEED_BASE + 50 + i, max(20, min(400, n // 5000)), n, n_lights=nl,
+                               shadow_lights=nl, ntex=1, tex_size=1024, name=f"cfg5-{i}"))
+    return scs, rng
+
+
+CONFIG5_RESOLUTIONS = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+CONFIG5_RES_P = [0.4, 0.3, 0.2, 0.1]
+
+
+def config5_requests(rng, n, rate, n_scenes=12):
+    """n requests of the served stream: Poisson arrival times at `rate` per
+    second, a scene index, and a view at a resolution drawn with
+    CONFIG5_RES_P on a seeded orbit of radius 3-20 units."""
+    cum = np.cumsum(CONFIG5_RES_P)
+    t, out = 0.0, []
+    for _ in range(n):
+        t += -math.log(max(rng.uniform(), 1e-12)) / rate
+        u = rng.uniform()
+        w, h = CONFIG5_RESOLUTIONS[int(np.searchsorted(cum, u, side="right"))] if u < 1.0 else CONFIG5_RESOLUTIONS[-1]
+        k = int(rng.uniform() * n_scenes)
+        out.append((t, k, orbit_view(rng, w, h, 3 + 17 * rng.uniform())))
+    return out
+
+
 def random_scene(seed, n_tris=60, width=64, height=48, n_objs=4, near_cross=True,
                  duplicate_ties=True, vertex_colours=True, two_sided=True, shadow=0):
     """Small seeded scenes for parity tests: random triangles in front of a
