@@ -2,7 +2,11 @@
 // draw API of ocr_engine.hpp: containers with load functions, textures,
 // lights, shadow pass, draw, blit, read-back.
 //
-// usage: ocr_facade_demo <scene.bin> <out.bin> [frames]
+// usage: ocr_facade_demo <scene.bin> <out.bin> [frames] [async|motion]
+//        ocr_facade_demo <scene.bin> <out.bin> <frames> obj <model.obj> <scale>
+//   obj: one more container loads <model.obj> (OBJ + MTL + PNG textures)
+//   through its default load function, objects_container::set_file and
+//   request_scale (main.cpp:68-87); the scene file's containers come first.
 //   scene.bin: "OCRSCN01", u32 ncontainers, per container: u32 nobj, per
 //   object: u32 ntri, u32 tex_index, obj_desc (144 B), ntri x triangle (144 B);
 //   u32 ntex, per texture: u32 w, u32 h, w*h*4 bytes; u32 nlights, n x 64 B;
@@ -28,7 +32,7 @@ static bool rd(FILE* f, void* p, size_t n) { return fread(p, 1, n, f) == n; }
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: %s scene.bin out.bin [frames] [async|motion]\n", argv[0]);
+    fprintf(stderr, "usage: %s scene.bin out.bin [frames] [async|motion|obj model.obj scale]\n", argv[0]);
     return 2;
   }
   int frames = argc > 3 ? atoi(argv[3]) : 2;
@@ -38,6 +42,7 @@ int main(int argc, char** argv) {
   // motion: after the frames, swap_buffers (main.cpp:283), move the camera
   // by (40, -25, 30), draw once more and apply do_motion_blur (main.cpp:270)
   const bool motion = argc > 4 && !strcmp(argv[4], "motion");
+  const bool obj_mode = argc > 6 && !strcmp(argv[4], "obj");
   FILE* f = fopen(argv[1], "rb");
   if (!f) return 3;
   char magic[8];
@@ -90,6 +95,12 @@ int main(int argc, char** argv) {
     });
     oc->set_active(!async_build || oc->id < conts.size() / 2);
   }
+  if (obj_mode) {
+    objects_container* oc = context.make_new();
+    oc->set_file(argv[5]);
+    oc->request_scale((float)atof(argv[6]));
+    oc->set_active(true);
+  }
   engine window;
   window.set_fov(hfov);
   window.load(view.width, view.height, 1000, "demo", "cl2.cl", true);
@@ -100,6 +111,7 @@ int main(int argc, char** argv) {
     context.load_active();
     size_t ci = 0;
     for (auto* oc : context.containers) {
+      if (ci >= conts.size()) break;   // the model's container keeps its loaded defaults
       for (size_t k = 0; k < oc->objs.size(); k++) {
         const ocr_obj_desc& d = conts[ci][k].desc;
         object& ob = oc->objs[k];

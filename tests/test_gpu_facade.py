@@ -1,7 +1,10 @@
 """The reference's draw API (engine / object_context / objects_container /
 texture_context / light, include/ocr_engine.hpp) driven by a main.cpp-style
-program renders the same frame as the C ABI and the oracle."""
+program renders the same frame as the C ABI and the oracle; a model ingested
+from OBJ + MTL + PNG files through objects_container::set_file renders the
+frame the oracle renders from its own reading of the same files."""
 import os
+import struct
 import subprocess
 
 import numpy as np
@@ -9,6 +12,9 @@ import pytest
 
 from openclrenderer_amd import abi, scenes
 from openclrenderer_amd import build as b
+from oracle import obj_load as obj_ref
+
+MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "obj_model", "model.obj")
 
 pytestmark = pytest.mark.gpu
 
@@ -53,3 +59,79 @@ def test_facade_demo_matches_oracle(oracle, tmp_path, shadow, mode):
         assert (blurred != ref["rgba"]).any()
         return
     assert np.abs(rgba.astype(int) - ref["rgba"].astype(int)).max() <= 1
+
+
+def _read_out(path):
+    raw = open(path, "rb").read()
+    used = np.frombuffer(raw[:64], dtype=abi.VIEW).copy()
+    W, H = int(used["width"][0]), int(used["height"][0])
+    n = W * H
+    depth = np.frombuffer(raw[64:64 + 4 * n], np.uint32).reshape(H, W)
+    tri = np.frombuffer(raw[64 + 4 * n:64 + 8 * n], np.uint32).reshape(H, W)
+    rgba = np.frombuffer(raw[64 + 8 * n:64 + 12 * n], np.uint8).reshape(H, W, 4)
+    return used, depth, tri, rgba
+
+
+def model_scene(path, scale, lights):
+    """The scene objects_container::set_file + load_active + object_context::
+    build make of an OBJ model (obj_load.cpp:181-568, objects_container.cpp
+    :22, object_context.cpp:228-339, texture_context.cpp:350-420), from the
+    oracle's reading of the files: one object per usemtl part in file order;
+    textures in creation order, a map_Kd PNG or Kd colour reused by its cache
+    key, magenta 32x32 for a part with neither; object defaults specular 0.9,
+    spec_mult 1, diffuse 1 (object.cpp:72-74) at the container's origin."""
+    tris, parts = obj_ref.load(path, scale)
+    t = np.zeros(tris["pos"].shape[0], dtype=abi.TRIANGLE)
+    t["v"]["pos"] = tris["pos"]
+    t["v"]["normal"] = tris["normal"]
+    t["v"]["vt"] = tris["vt"]
+    textures, cache, objs = [], {}, []
+    for k, p in enumerate(parts):
+        if p["texture_kind"] in (0, 1) and p["texture_cache"] in cache:
+            tid = cache[p["texture_cache"]]
+        else:
+            if p["texture_kind"] == 0:
+                rgba = obj_ref.png_decode(p["texture_path"])
+            else:
+                rgba = np.tile(np.array(p["colour"], np.uint8), (32, 32, 1))
+            tid = len(textures)
+            textures.append(scenes.Texture(rgba.shape[1], rgba.shape[0], rgba))
+            if p["texture_kind"] in (0, 1):
+                cache[p["texture_cache"]] = tid
+        t["v"]["object_id"][p["first_tri"]:p["first_tri"] + p["ntri"], 0] = k
+        objs.append(scenes._desc((0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0), 1.0, tid=tid))
+    return scenes.Scene(t, np.concatenate(objs), lights, textures, 1024, "obj"), parts
+
+
+def _write_lights_view(path, lights, view, hfov):
+    """ocr_facade_demo's scene file with no containers and no textures."""
+    with open(path, "wb") as f:
+        f.write(b"OCRSCN01" + struct.pack("<II", 0, 0))
+        f.write(struct.pack("<I", lights.shape[0]))
+        f.write(np.ascontiguousarray(lights).tobytes())
+        f.write(np.ascontiguousarray(view).tobytes())
+        f.write(struct.pack("<f", hfov))
+
+
+def test_facade_obj_model_matches_oracle(oracle, tmp_path):
+    lights = np.concatenate([scenes.make_light((250.0, -400.0, -350.0), shadow=1),
+                             scenes.make_light((-300.0, -200.0, -500.0), col=(0.6, 0.7, 1.0), brightness=0.8)])
+    view = scenes.make_view(320, 240, (0.0, -120.0, -700.0))
+    scene_bin, out_bin = tmp_path / "lights.bin", tmp_path / "out.bin"
+    _write_lights_view(str(scene_bin), lights, view, 60.0)
+    assert os.path.exists(b.FACADE_DEMO), "build() first"
+    r = subprocess.run([b.FACADE_DEMO, str(scene_bin), str(out_bin), "2", "obj", MODEL, "100"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "4 objects" in r.stdout, r.stdout
+    used, depth, tri, rgba = _read_out(str(out_bin))
+    sc, parts = model_scene(MODEL, 100.0, lights)
+    assert [p["group_name"] for p in parts] == ["crate", "roof", "floor", "ghost"]
+    assert len(sc.textures) == 3          # checker (shared by crate and floor), paint colour, magenta
+    ref = oracle.render(sc, used)
+    assert np.array_equal(depth, ref["depth"])
+    assert np.array_equal(tri, ref["tri_id"])
+    assert np.abs(rgba.astype(int) - ref["rgba"].astype(int)).max() <= 1
+    # every part is on screen
+    seen = np.unique(sc.tris["v"]["object_id"][tri[tri != 0xFFFFFFFF], 0])
+    assert set(seen.tolist()) == {0, 1, 2, 3}, seen

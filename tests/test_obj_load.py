@@ -254,3 +254,27 @@ def test_png_decodes_known_pixels(eng, tmp_path, ctype):
 def test_reference_textures_decode_like_oracle(eng, rel):
     path = os.path.join(REFERENCE, rel)
     assert np.array_equal(_decode(eng, path), ref.png_decode(path))
+
+
+def test_committed_model_fixture(eng):
+    """tests/golden/obj_model (make_obj_fixture.py): the native loader equals
+    the oracle's reading, the four parts take a PNG map, a Kd colour, the
+    PNG again (same cache key) and magenta, and both PNG decoders give the
+    generator's pixels although every PNG filter type occurs."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_obj_fixture import checker_rgba
+    model = os.path.join(ROOT, "tests", "golden", "obj_model", "model.obj")
+    tris, parts = assert_same(eng, model, 100.0)
+    assert tris.shape[0] == 21
+    kinds = [(p.group_name.decode(), p.texture_kind, tuple(p.colour)) for p in parts]
+    assert kinds == [("crate", 0, (0, 0, 0, 0)), ("roof", 1, (204, 76, 25, 255)), ("floor", 0, (0, 0, 0, 0)),
+                     ("ghost", 2, (255, 0, 255, 255))]
+    assert parts[0].texture_cache == parts[2].texture_cache
+    png = os.path.join(ROOT, "tests", "golden", "obj_model", "checker.png")
+    assert np.array_equal(ref.png_decode(png), checker_rgba())
+    w, h, px = C.c_uint32(), C.c_uint32(), C.POINTER(C.c_uint8)()
+    assert eng.ocr_png_load(png.encode(), C.byref(w), C.byref(h), C.byref(px)) == 0
+    got = np.ctypeslib.as_array(px, shape=(h.value, w.value, 4)).copy()
+    eng.ocr_png_free(px)
+    assert np.array_equal(got, checker_rgba())
