@@ -35,6 +35,17 @@ cl_float4 engine::c_rot = {{0, 0, 0, 0}};
 
 std::vector<light*> light::lightlist;
 std::vector<cl_uint> light::active;
+namespace {
+// The global list owns the lights add_light copies; those never removed are
+// freed at exit (destroyed before the list, which is defined first).
+struct LightListOwner {
+  ~LightListOwner() {
+    for (light* l : light::lightlist) delete l;
+    light::lightlist.clear();
+    light::active.clear();
+  }
+} g_light_list_owner;
+}  // namespace
 bool light::static_lights_are_dirty = true;
 bool light::dirty_gpu = true;
 

@@ -324,6 +324,9 @@ int ocr_png_load(const char* path, uint32_t* width, uint32_t* height, uint8_t** 
   if (!w || !h || w > 32768 || h > 32768 || depth != 8 || !chans || interlace != 0) return OCR_ERR_UNSUPPORTED;
   if (ctype == 3 && plte.size() < 3) return OCR_ERR_INVALID;
   const size_t stride = (size_t)w * chans;
+  // deflate expands at most 1032:1, so a header promising more than that
+  // is refused before its buffer is allocated
+  if ((uint64_t)h * (stride + 1) > (uint64_t)idat.size() * 1032 + 64) return OCR_ERR_INVALID;
   std::vector<uint8_t> raw(h * (stride + 1));
   uLongf raw_len = (uLongf)raw.size();
   if (uncompress(raw.data(), &raw_len, idat.data(), (uLong)idat.size()) != Z_OK || raw_len != raw.size())

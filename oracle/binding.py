@@ -62,7 +62,10 @@ def camera_sincos(c_rot):
 
 
 class Oracle:
-    def __init__(self, path=LIB_PATH):
+    def __init__(self, path=None):
+        # OCR_ORACLE_LIB: another build of the same source (the sanitizer
+        # build of oracle/Makefile's `asan` target)
+        path = path or os.environ.get("OCR_ORACLE_LIB") or LIB_PATH
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run oracle/Makefile or build())")
         self.lib = L = C.CDLL(path)
