@@ -462,7 +462,7 @@ struct TriLds {
   float ek[7][kWave];
   uint32_t key[kWave];
   uint32_t begin[kWave + 1];
-  uint32_t start[2 * kWave];   // start[p] == base: a staged triangle begins at sample base + p
+  uint32_t start[kWave];   // start[p] == base: a staged triangle begins at sample base + p
   uint32_t pitch[kBatch ? kWave : 1];
   uint32_t* dst[kBatch ? kWave : 1];
   unsigned long long* ids[kBatch && kMode == RasterMode::Id ? kWave : 1];
@@ -499,89 +499,67 @@ __device__ __forceinline__ bool sample_xy(float slf, float min_x, float min_y, f
 }
 
 // Every sample of [s, stage_end) of the staged triangles with all lanes of
-// the wavefront, two samples per lane per step (called wave-uniformly).
-// my_begin is the first sample of the lane's staged triangle (stage_end or
-// more for lanes without one); staged triangles hold at least one sample
-// each, in lane order, and L.start holds no value in [s, stage_end) on
-// entry.  A single view's target (width, depth, id keys) comes in view_*;
-// shadow maps come in view_depth.
+// the wavefront, one sample per lane per step (called wave-uniformly; two
+// per lane measured slower: stages of small triangles end in part-filled
+// steps).  my_begin is the first sample of the lane's staged triangle
+// (stage_end or more for lanes without one); staged triangles hold at least
+// one sample each, in lane order, and L.start holds no value in
+// [s, stage_end) on entry.  A single view's target (width, depth, id keys)
+// comes in view_*; shadow maps come in view_depth.
 template <RasterMode kMode, int kOp, bool kBatch>
 __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t view_width, uint32_t* view_depth,
                                               unsigned long long* view_ids, uint32_t shadow_dim, uint32_t my_begin,
                                               uint32_t s, uint32_t stage_end, uint32_t c0) {
   const int lane = threadIdx.x & (kWave - 1);
-  for (uint32_t base = s; base < stage_end; base += 2 * kWave) {
-    // Lane p takes samples j = base + p + 64h (h = 0, 1).  Its staged
-    // triangle is one less than the number of staged triangles beginning at
-    // or before j: those beginning at or before base + 64h (a prefix of the
-    // lanes, one ballot) plus those beginning in (base + 64h, j] (flags
-    // written by the triangles' lanes into start[], counted below p with
-    // mbcnt).  Offset 64 is left unflagged: a triangle beginning there is
-    // in the second half's prefix.
-    const unsigned long long before0 = __ballot(my_begin <= base);
-    const unsigned long long before1 = __ballot(my_begin <= base + kWave);
+  for (uint32_t base = s; base < stage_end; base += kWave) {
+    // Lane p takes sample j = base + p.  Its staged triangle k is one less
+    // than the number of staged triangles beginning at or before j: those
+    // beginning at or before `base` (a prefix of the lanes, one ballot)
+    // plus those beginning in (base, j] (flags written by the triangles'
+    // lanes, counted below p with mbcnt).
+    const unsigned long long before = __ballot(my_begin <= base);
     const uint32_t off = my_begin - base;
-    if (my_begin > base && off < 2u * kWave && off != (uint32_t)kWave) L.start[off] = base;
+    if (my_begin > base && off < (uint32_t)kWave) L.start[off] = base;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    uint32_t d[2], pix[2];
-    uint32_t* dst[2];
-    int k[2];
-    bool live[2];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const bool starts_here = L.start[h * kWave + lane] == base;
-      const unsigned long long starts = __ballot(starts_here);
-      const uint32_t j = base + (uint32_t)(h * kWave + lane);
-      k[h] = __popcll(h ? before1 : before0) - 1 +
-             (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
-                                            __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
-             (starts_here ? 1 : 0);
-      live[h] = false;
-      d[h] = pix[h] = 0u;
-      dst[h] = nullptr;
-      if (j < stage_end) {
-        const int kk = k[h];
-        float x, y;
-        live[h] = sample_xy<kOp>((float)(j - L.begin[kk]), L.min_x[kk], L.min_y[kk], L.w[kk], L.iw[kk],
-                                 L.ek[0][kk], L.ek[1][kk], L.ek[2][kk], L.ek[3][kk], L.ek[4][kk], L.ek[5][kk],
-                                 L.ek[6][kk], x, y);
-        if (live[h]) {
-          d[h] = depth_quant(fmaf(L.pa[kk], x, fmaf(L.pb[kk], y, L.pc[kk])));
-          uint32_t pitch;
-          if constexpr (kMode == RasterMode::Shadow) {
-            // view_depth: the maps, 6 faces of shadow_dim^2 per light
-            pitch = shadow_dim;
-            dst[h] = view_depth + (size_t)L.key[kk] * shadow_dim * shadow_dim;
-          } else if constexpr (kBatch) {
-            pitch = L.pitch[kk];
-            dst[h] = L.dst[kk];
-          } else {
-            pitch = view_width;
-            dst[h] = view_depth;
-          }
-          pix[h] = (uint32_t)(int)y * pitch + (uint32_t)(int)x;
-        }
-      }
+    const bool starts_here = L.start[lane] == base;
+    const unsigned long long starts = __ballot(starts_here);
+    const uint32_t j = base + (uint32_t)lane;
+    if (j >= stage_end) continue;
+    const int k = __popcll(before) - 1 +
+                  (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
+                  (starts_here ? 1 : 0);
+    float x, y;
+    if (!sample_xy<kOp>((float)(j - L.begin[k]), L.min_x[k], L.min_y[k], L.w[k], L.iw[k], L.ek[0][k], L.ek[1][k],
+                        L.ek[2][k], L.ek[3][k], L.ek[4][k], L.ek[5][k], L.ek[6][k], x, y))
+      continue;
+    const uint32_t d = depth_quant(fmaf(L.pa[k], x, fmaf(L.pb[k], y, L.pc[k])));
+    uint32_t pitch;
+    uint32_t* dst;
+    if constexpr (kMode == RasterMode::Shadow) {
+      // view_depth: the maps, 6 faces of shadow_dim^2 per light
+      pitch = shadow_dim;
+      dst = view_depth + (size_t)L.key[k] * shadow_dim * shadow_dim;
+    } else if constexpr (kBatch) {
+      pitch = L.pitch[k];
+      dst = L.dst[k];
+    } else {
+      pitch = view_width;
+      dst = view_depth;
     }
-    // both samples' loads before either test: one memory latency per step
-    uint32_t cur[2];
-#pragma unroll
-    for (int h = 0; h < 2; h++) cur[h] = live[h] ? dst[h][pix[h]] : 0u;
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      if (!live[h]) continue;
-      if constexpr (kMode == RasterMode::Id) {
-        unsigned long long* ids = kBatch ? L.ids[kBatch ? k[h] : 0] : view_ids;
-        if (d[h] > cur[h] - kBufError && d[h] < cur[h] + kBufError)
-          atomicMax(&ids[pix[h]], ((unsigned long long)(L.key[k[h]] + 1u) << 32) | (c0 + (uint32_t)k[h]));
-      } else {
-        // The L2 performs atomics at ~26.6 G cache-line operations/s; a
-        // plain load first drops the samples that cannot win (the stored
-        // depth only decreases, so d >= any value a load returns loses).
-        if (d[h] < cur[h]) atomicMin(&dst[h][pix[h]], d[h]);
-      }
+    const size_t pix = (size_t)((uint32_t)(int)y * pitch + (uint32_t)(int)x);
+    if constexpr (kMode == RasterMode::Id) {
+      unsigned long long* ids = kBatch ? L.ids[kBatch ? k : 0] : view_ids;
+      const uint32_t val = dst[pix];
+      if (d > val - kBufError && d < val + kBufError)
+        atomicMax(&ids[pix], ((unsigned long long)(L.key[k] + 1u) << 32) | (c0 + (uint32_t)k));
+    } else {
+      // The L2 performs atomics at ~26.6 G cache-line operations/s; a
+      // plain load first drops the samples that cannot win (the stored
+      // depth only decreases, so d >= any value a load returns loses).
+      if (d < dst[pix]) atomicMin(&dst[pix], d);
     }
   }
 }
@@ -606,7 +584,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
   TriLds<kMode, kBatch>& L = lds_all[threadIdx.x / kWave];
   const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  L.start[lane] = L.start[kWave + lane] = 0xFFFFFFFFu;   // never a sample index (total < 2^31)
+  L.start[lane] = 0xFFFFFFFFu;   // never a sample index (total < 2^31)
   for (uint32_t s0 = wave0 * kChunk; s0 < total; s0 += nwaves * kChunk) {
     const uint32_t s1 = min(s0 + kChunk, total);
     // ---- the chunk's first triangle: last record with sample_begin <= s0
@@ -764,7 +742,7 @@ __device__ __forceinline__ void local_sweep(ShadowStage& W, const RasterTri& rt,
   const unsigned long long m = __ballot(n > 0);
   const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
   const uint32_t nloc = (uint32_t)__popcll(m);
-  W.start[lane] = W.start[kWave + lane] = 0xFFFFFFFFu;   // sample indices of this sweep are below 2^31
+  W.start[lane] = 0xFFFFFFFFu;   // sample indices of this sweep are below 2^31
   if (n) {
     W.min_x[pos] = rt.min_x;
     W.min_y[pos] = rt.min_y;
