@@ -468,11 +468,19 @@ struct TriLds {
   unsigned long long* ids[kBatch && kMode == RasterMode::Id ? kWave : 1];
 };
 
+// point_in_tri (cl2.cl:4798-4807) with the triangle's loop-invariant terms.
+__device__ __forceinline__ bool inside_tri(float x, float y, float e0, float e1, float e2, float e3, float e4,
+                                           float e5, float e6) {
+  const float s_ = e0 + e1 * x + e2 * y;
+  const float t_ = e3 + e4 * x + e5 * y;
+  return s_ > -0.0001f && t_ > -0.0001f && (s_ + t_) < e6;
+}
+
 // Sample `j` (index within its triangle, an exact binary32 integer) of a
-// set-up triangle: its column x, row y and whether it lies inside
-// (point_in_tri).  Sample j is sample o = j % (op + 1) of fragment
-// a = j / (op + 1), at walk index q = op*a + o; the quotient is a float
-// estimate corrected by one step each way.
+// set-up triangle: its column x, row y and whether it lies inside.  Sample j
+// is sample o = j % (op + 1) of fragment a = j / (op + 1), at walk index
+// q = op*a + o; the quotient is a float estimate corrected by one step each
+// way.
 template <int kOp>
 __device__ __forceinline__ bool sample_xy(float slf, float min_x, float min_y, float w, float iw, float e0, float e1,
                                           float e2, float e3, float e4, float e5, float e6, float& x, float& y) {
@@ -493,9 +501,28 @@ __device__ __forceinline__ bool sample_xy(float slf, float min_x, float min_y, f
   const float xo = e + (qf - r);
   x = min_x + xo;
   if (xo >= w) return false;   // x >= max_x
-  const float s_ = e0 + e1 * x + e2 * y;
-  const float t_ = e3 + e4 * x + e5 * y;
-  return s_ > -0.0001f && t_ > -0.0001f && (s_ + t_) < e6;
+  return inside_tri(x, y, e0, e1, e2, e3, e4, e5, e6);
+}
+
+// Whether sample_xy reduces, for every sample of a set-up triangle of n
+// samples, to its short form
+//   y = row_of(j),  x = min_x + (j - (y - min_y) * w):
+// one fragment (n <= op + 1: a = 0, q = j) and rows starting exactly at
+// the multiples of w (run_start(y) == (y - min_y) * w: row_of neither leads
+// nor lags at any of the triangle's row boundaries, checked for up to 32
+// rows).  Then r = (y - min_y) * w, floor(r / w) gives e = 0 after the
+// corrections, and xo = j - r, the same binary32 values.
+template <int kOp>
+__device__ __forceinline__ bool short_form_ok(uint32_t n, float min_y, float w, float iw) {
+  if (n == 0 || n > (uint32_t)kOp + 1 || !(w > 0.f)) return false;
+  const float rows = row_of((float)(n - 1), iw, min_y) - min_y;
+  if (!(rows <= 32.f)) return false;
+  bool ok = true;
+  for (float k = 1.f; k <= rows; k += 1.f) {
+    const float n0 = k * w;
+    ok = ok && row_of(n0 - 1.f, iw, min_y) == min_y + (k - 1.f) && row_of(n0, iw, min_y) == min_y + k;
+  }
+  return ok;
 }
 
 // Every sample of [s, stage_end) of the staged triangles with all lanes of
@@ -506,10 +533,13 @@ __device__ __forceinline__ bool sample_xy(float slf, float min_x, float min_y, f
 // one sample each, in lane order, and L.start holds no value in
 // [s, stage_end) on entry.  A single view's target (width, depth, id keys)
 // comes in view_*; shadow maps come in view_depth.
+// Bit k of short_bits: staged triangle k maps by the short form
+// (short_form_ok); steps whose lanes all sample such triangles take it.
 template <RasterMode kMode, int kOp, bool kBatch>
 __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t view_width, uint32_t* view_depth,
                                               unsigned long long* view_ids, uint32_t shadow_dim, uint32_t my_begin,
-                                              uint32_t s, uint32_t stage_end, uint32_t c0) {
+                                              uint32_t s, uint32_t stage_end, uint32_t c0,
+                                              unsigned long long short_bits) {
   const int lane = threadIdx.x & (kWave - 1);
   for (uint32_t base = s; base < stage_end; base += kWave) {
     // Lane p takes sample j = base + p.  Its staged triangle k is one less
@@ -532,9 +562,19 @@ __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t
                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
                   (starts_here ? 1 : 0);
     float x, y;
-    if (!sample_xy<kOp>((float)(j - L.begin[k]), L.min_x[k], L.min_y[k], L.w[k], L.iw[k], L.ek[0][k], L.ek[1][k],
-                        L.ek[2][k], L.ek[3][k], L.ek[4][k], L.ek[5][k], L.ek[6][k], x, y))
+    const float slf = (float)(j - L.begin[k]);
+    if (__all((short_bits >> k) & 1ull)) {
+      const float min_y = L.min_y[k], w = L.w[k];
+      y = row_of(slf, L.iw[k], min_y);
+      const float xo = slf - (y - min_y) * w;
+      x = L.min_x[k] + xo;
+      if (xo >= w || !inside_tri(x, y, L.ek[0][k], L.ek[1][k], L.ek[2][k], L.ek[3][k], L.ek[4][k], L.ek[5][k],
+                                 L.ek[6][k]))
+        continue;
+    } else if (!sample_xy<kOp>(slf, L.min_x[k], L.min_y[k], L.w[k], L.iw[k], L.ek[0][k], L.ek[1][k], L.ek[2][k],
+                               L.ek[3][k], L.ek[4][k], L.ek[5][k], L.ek[6][k], x, y)) {
       continue;
+    }
     const uint32_t d = depth_quant(fmaf(L.pa[k], x, fmaf(L.pb[k], y, L.pc[k])));
     uint32_t pitch;
     uint32_t* dst;
@@ -605,9 +645,13 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
       // ---- stage triangles c0 .. c0 + 63, one per lane
       const uint32_t ci = c0 + lane;
       uint32_t my_begin = total;   // this lane's staged triangle's first sample
+      float t_min_y = 0.f, t_w = 0.f, t_iw = 0.f;
       if (ci < ncut) {
         const RasterTri t = Ls.rast[ci];
         my_begin = t.sample_begin;
+        t_min_y = t.min_y;
+        t_w = t.w;
+        t_iw = t.iw;
         L.min_x[lane] = t.min_x;
         L.min_y[lane] = t.min_y;
         L.w[lane] = t.w;
@@ -639,9 +683,11 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const uint32_t stage_end = min(s1, L.begin[kWave]);
+      const unsigned long long short_bits =
+          __ballot(ci < ncut && short_form_ok<kOp>(L.begin[lane + 1] - my_begin, t_min_y, t_w, t_iw));
       // ---- every sample of [s, stage_end) with all lanes
       sweep_samples<kMode, kOp, kBatch>(L, a.view0.v.width, kMode == RasterMode::Shadow ? sh.map : a.view0.depth,
-                                        a.view0.idkey, sh.L, my_begin, s, stage_end, c0);
+                                        a.view0.idkey, sh.L, my_begin, s, stage_end, c0, short_bits);
       __builtin_amdgcn_wave_barrier();
       s = stage_end;
       c0 += kWave;
@@ -760,7 +806,12 @@ __device__ __forceinline__ void local_sweep(ShadowStage& W, const RasterTri& rt,
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const uint32_t my_begin = (uint32_t)lane < nloc ? W.begin[lane] : total;
-  sweep_samples<RasterMode::Shadow, kOpSizeLight, false>(W, 0u, sh.map, nullptr, sh.L, my_begin, 0u, total, 0u);
+  const uint32_t my_end = (uint32_t)lane + 1 < nloc ? W.begin[lane + 1] : total;
+  const unsigned long long short_bits =
+      __ballot((uint32_t)lane < nloc &&
+               short_form_ok<kOpSizeLight>(my_end - my_begin, W.min_y[lane], W.w[lane], W.iw[lane]));
+  sweep_samples<RasterMode::Shadow, kOpSizeLight, false>(W, 0u, sh.map, nullptr, sh.L, my_begin, 0u, total, 0u,
+                                                         short_bits);
   // the stage is rewritten by the wavefront's next sweep
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
