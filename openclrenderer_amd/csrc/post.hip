@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256) k_pseudo_aa(const uint32_t* __restrict__ 
   const uint32_t my_raw = depth[p];
   if ((float)my_raw == kMulint) return;
   const f3 mn = fast_normalize(decode_normal_u16(normals[p]));
-  const float my_depth = ((float)my_raw / kMulint) * kDepthFar;
+  const float my_depth = depth_of(my_raw);
   int nx[2] = {0, 0}, ny[2] = {0, 0}, nc[2] = {0, 0};
   f3 mine[2] = {mk3(0, 0, 0), mk3(0, 0, 0)}, theirs[2] = {mk3(0, 0, 0), mk3(0, 0, 0)};
   float ms[2] = {0.f, 0.f}, ts[2] = {0.f, 0.f};
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) k_pseudo_aa(const uint32_t* __restrict__ 
     for (int i = -1; i < 2; i++) {
       if (i == 0 && j == 0) continue;
       const uint32_t q = (uint32_t)((int)p + j * (int)W + i);
-      const float d = ((float)depth[q] / kMulint) * kDepthFar;
+      const float d = depth_of(depth[q]);
       const f3 fnrm = decode_normal_u16(normals[q]);
       const f3 val = unorm_rgb(in[q]);
       const bool tests[2] = {fabsf(d - my_depth) > 100.f, dot(mn, fnrm) < cos_limit};
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(256) k_motion_blur(const MotionArgs a) {
   const f3 old_pos = mk3(m[rd_pos], m[rd_pos + 1], m[rd_pos + 2]);
   const f4 old_q = f4{m[rd_q], m[rd_q + 1], m[rd_q + 2], m[rd_q + 3]};
   const float fov = a.fov;
-  const float actual_depth = ((float)ft / kMulint) * kDepthFar;
+  const float actual_depth = depth_of(ft);
   const bool fov_fast = rcp_safe(fov);
   const float ifov = fov_fast ? 1.f / fov : 0.f;
   auto by_fov = [&](float n) { return fov_fast ? div_rcp(n, fov, ifov) : n / fov; };

@@ -186,7 +186,7 @@ struct SsaoTaps {
   // (rounding is monotone), so the count comes from the middle threshold and
   // the pair on its side
   __device__ __forceinline__ uint32_t at(float wx, float wy) const {
-    const float d2 = ((float)ldg(a->depth, (uint32_t)(int)wy * a->v.width + (uint32_t)(int)wx) / kMulint) * kDepthFar;
+    const float d2 = depth_of(ldg(a->depth, (uint32_t)(int)wy * a->v.width + (uint32_t)(int)wx));
     const bool hi = d2 > thr[2];
     const float ta = hi ? thr[3] : thr[0], tb = hi ? thr[4] : thr[1];
     return (hi ? 3u : 0u) + (d2 > ta ? 1u : 0u) + (d2 > tb ? 1u : 0u);
@@ -197,7 +197,7 @@ __device__ __forceinline__ float ssao(const ShadeArgs& a, int sx, int sy) {
   const uint32_t W = a.v.width, H = a.v.height;
   uint32_t seed2 = xorshift(wang_hash((uint32_t)sx + W * H * (uint32_t)sy));
   float foffset = (float)seed2 / kMulint;
-  float depth = ((float)ldg(a.depth, (uint32_t)sy * W + (uint32_t)sx) / kMulint) * kDepthFar;
+  float depth = depth_of(ldg(a.depth, (uint32_t)sy * W + (uint32_t)sx));
   float rad = kSsaoRad;
   rad += foffset / 2.f;
   // (rad * fov) / depth.  Depths of drawn pixels lie in [20, 350000]; with
@@ -305,7 +305,7 @@ __device__ __forceinline__ float shadow_taps(const ShadeArgs& a, const ShadowSet
     for (int x = 0; x < 4; x++) {
       const uint32_t t = map_off + (uint32_t)y * L + (uint32_t)x;
       touch<kCount>(touch_bits, t);
-      float ldp = ((float)ldg(buf, t) / kMulint) * kDepthFar;
+      float ldp = depth_of(ldg(buf, t));
       cond |= (s.dpth > ldp + s.bias ? 1u : 0u) << (y * 4 + x);
     }
   if (cond == 0u) return 0.f;
@@ -471,7 +471,7 @@ __device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
   const f4 qn = f4{G.qn[0], G.qn[1], G.qn[2], G.qn[3]};
   const f4 qb = f4{G.qb[0], G.qb[1], G.qb[2], G.qb[3]};
 
-  const float actual_depth = ((float)ft / kMulint) * kDepthFar;
+  const float actual_depth = depth_of(ft);
   // quotients by the view's fov from its correctly rounded reciprocal (the
   // view's fov is checked to lie in [2^-60, 2^60] on the host)
   const float ifov = a.v.ifov;
