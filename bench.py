@@ -437,9 +437,9 @@ def run_plain(args):
     post = run_blend(ctx, g, fr, sc, view, W, H) if extras else None
 
     if args.calib:
-        cal = C.CDLL(os.path.join(ROOT, "openclrenderer_amd", "libocr_calib.so"))
-        cal.ocr_calib_copy.argtypes = [C.c_size_t, C.c_int]
-        if cal.ocr_calib_copy(256 << 20, 3) != 0:
+        copy_lib = C.CDLL(os.path.join(ROOT, "openclrenderer_amd", "libocr_calib.so"))
+        copy_lib.ocr_calib_copy.argtypes = [C.c_size_t, C.c_int]
+        if copy_lib.ocr_calib_copy(256 << 20, 3) != 0:
             raise RuntimeError("calibration copy failed")
 
     cpu, parity, ref_stats = (None, None, None) if args.no_cpu else \
