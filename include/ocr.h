@@ -228,6 +228,32 @@ void ocr_frame_destroy(ocr_frame* f);
  * too small sets the overflow word, ocr_frame_read then reports
  * OCR_ERR_OVERFLOW and the next render grows the list to fit. */
 int  ocr_frame_reserve(ocr_frame* f, uint32_t max_cut_triangles);
+/* Raster path of the frame's renders (ocr_render).  Both give the same
+ * depth and triangle-id buffers, bit for bit.
+ *   OCR_RASTER_SAMPLES  kernel1 / kernel2 (cl2.cl:4985-5127, :5390-5546):
+ *       every cut triangle's fragment walk in one global sample space, a
+ *       depth pass with u32 atomicMin and an id pass with u64 atomicMax on
+ *       the frame's buffers (the default).
+ *   OCR_RASTER_TILES    engine::draw_tiled_deferred (engine.cpp:2447-2535:
+ *       split_into_tiled_chunks, tile_render, cl2.cl:3850-4210): cut
+ *       triangles binned to screen tiles of 2^tile_w_log2 x 2^tile_h_log2
+ *       pixels (256 to 2048 pixels, sides 8 to 64), each tile's depth and
+ *       ids resolved in on-chip memory by one workgroup and written once.
+ *       The bins hold bin_capacity entries (0: two per cut-triangle slot);
+ *       a render that needs more reports OCR_ERR_OVERFLOW at ocr_frame_read
+ *       and the next render grows them.
+ * ocr_frame_stats then reports the bin count + scatter as ms_clear, the
+ * tile pass as ms_depth and 0 as ms_id. */
+#define OCR_RASTER_SAMPLES 0u
+#define OCR_RASTER_TILES   1u
+typedef struct ocr_raster_config {
+    uint32_t mode;              /* OCR_RASTER_SAMPLES or OCR_RASTER_TILES */
+    uint32_t tile_w_log2;       /* 3..6 */
+    uint32_t tile_h_log2;       /* 3..6, tile_w_log2 + tile_h_log2 in 8..11 */
+    uint32_t bin_capacity;      /* bin entries (0 = automatic) */
+} ocr_raster_config;
+int  ocr_frame_set_raster(ocr_frame* f, const ocr_raster_config* cfg);
+int  ocr_frame_get_raster(ocr_frame* f, ocr_raster_config* cfg);
 /* Record per-pass hipEvent timings on every render of this frame (up to
  * 1024 renders are averaged; enabling again restarts the average). */
 int  ocr_frame_set_profiling(ocr_frame* f, int enable);
@@ -324,7 +350,10 @@ typedef struct ocr_batch_tuning {
                                         the views split in that many pixel-balanced groups,
                                         each on a stream of the batch's own, so one group's
                                         shading overlaps another's raster passes */
-    uint32_t _pad;
+    uint32_t raster_tile;            /* 0: the sample-space depth and id passes; otherwise
+                                        the tiled raster path (see ocr_frame_set_raster) with
+                                        tiles of 2^(raster_tile & 0xFF) x 2^(raster_tile >> 8)
+                                        pixels */
 } ocr_batch_tuning;
 
 int  ocr_batch_create(ocr_ctx* ctx, uint32_t max_views, ocr_batch** out);

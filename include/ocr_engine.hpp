@@ -288,6 +288,17 @@ struct engine {
   void append_opencl_extra_command_line(const std::string&) {}
   compute_event generate_realtime_shadowing(object_context_data& dat);
   compute_event draw_bulk_objs_n(object_context_data& dat);
+  // engine::draw_tiled_deferred (engine.cpp:2447-2535): the same draw with
+  // the depth and id passes resolved per screen tile of tile_size x
+  // min(tile_size, 2048 / tile_size) pixels (ocr_frame_set_raster,
+  // OCR_RASTER_TILES; tile_size a power of two, 16..64).  The reference's
+  // experimental tiled draw stops at a depth buffer of its own coverage
+  // rule; this one draws the whole frame, bit for bit as draw_bulk_objs_n.
+  compute_event draw_tiled_deferred(object_context_data& dat);
+  cl_int tile_size = 32;             // engine.hpp:139 (the reference sets 64, engine.cpp:684)
+ private:
+  compute_event draw(object_context_data& dat);   // the render both draws share
+ public:
   // engine.cpp:2550-2570: src's screen over dst's where src is at least as
   // near (both contexts drawn at this engine's size)
   compute_event blend_with_depth(object_context_data& src, object_context_data& dst);

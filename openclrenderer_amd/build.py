@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libocr.so")
 CALIB_LIB = os.path.join(HERE, "libocr_calib.so")
 FACADE_LIB = os.path.join(HERE, "libocr_engine.so")
 FACADE_DEMO = os.path.join(HERE, "ocr_facade_demo")
-SOURCES = ["runtime.hip", "raster.hip", "shade.hip", "atlas.hip", "batch.hip", "server.hip", "post.hip"]
+SOURCES = ["runtime.hip", "raster.hip", "tiles.hip", "shade.hip", "atlas.hip", "batch.hip", "server.hip", "post.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")

@@ -41,6 +41,9 @@ struct ocr_batch {
   CutTri* cuts = nullptr;
   uint32_t* counters = nullptr;
   uint32_t cap_cut = 0;
+  // tiled raster path (tune.raster_tile != 0): bins over every view's tiles
+  TileBins tb{};
+  uint32_t box_cap = 0, bin_tiles = 0, want_bin = 0;
   ocr_batch_tuning tune{8, 4, 1, 0};
   // tune.groups > 1: sub-batches over pixel-balanced groups of the views,
   // each rendered on a stream of its own between a fork and a join event
@@ -84,34 +87,55 @@ static PassArgs batch_pass(const ocr_batch* b) {
   for (const Scene* s : b->scenes)
     if (s->n_shadow) a.unshadowed = 0u;
   a.lists = make_lists(b->cuts, b->counters, b->cap_cut);
+  if (b->tune.raster_tile) a.lists.tb = b->tb;
   return a;
 }
+
+static uint32_t raster_twl(const ocr_batch* b) { return b->tune.raster_tile & 0xFFu; }
+static uint32_t raster_thl(const ocr_batch* b) { return b->tune.raster_tile >> 8; }
 
 // Recompute the per-view ranges (they depend on the tile shape) and upload
 // the view table.
 static int upload_views(ocr_batch* b) {
-  uint32_t tri = 0, pix = 0, tiles = 0;
+  uint32_t tri = 0, pix = 0, tiles = 0, bins = 0;
+  const bool tiled = b->tune.raster_tile != 0;
   for (auto& v : b->host_views) {
     const Scene* s = b->scenes[v.scene];
     v.tri_begin = tri;
     v.pix_begin = pix;
     v.tile_begin = tiles;
+    v.bin_begin = bins;
+    v.bin_tiles_x = tiled ? bin_tiles_x(v.v.width, raster_twl(b)) : 0u;
     tri += (s->ntri + 255) / 256 * 256;
     pix += (v.v.width * v.v.height + 255) / 256 * 256;
     tiles += view_tiles(v.v.width, v.v.height, b->tune.shade_tile_w_log2);
+    if (tiled) bins += bin_tiles(v.v.width, v.v.height, raster_twl(b), raster_thl(b));
   }
   b->tri_slots = tri;
   b->pix_slots = pix;
   b->tiles = tiles;
+  b->bin_tiles = bins;
   OCR_HIP_CHECK(hipMemcpy(b->d_views, b->host_views.data(), sizeof(DevView) * b->nviews, hipMemcpyHostToDevice));
   return OCR_OK;
 }
 
 // At most 2 cut pieces per triangle of every view, so the list never
 // overflows.
+static int ensure_bins_of(ocr_batch* b) {
+  if (!b->tune.raster_tile) return OCR_OK;
+  const uint64_t dflt = std::max<uint64_t>(2ull * b->cap_cut, 4096ull);
+  const uint32_t cap = (uint32_t)std::min<uint64_t>(b->want_bin ? (uint64_t)b->want_bin : dflt, 0xFFFFFFF0ull);
+  const TileBins before = b->tb;
+  const int rc = ensure_bins(b->tb, b->box_cap, b->cap_cut, b->bin_tiles, raster_twl(b), raster_thl(b), cap,
+                             b->want_bin != 0);
+  // a captured graph holds the bins' addresses and sizes
+  if (memcmp(&before, &b->tb, sizeof(TileBins)) != 0) drop_graph(b);
+  return rc;
+}
+
 static int ensure_lists(ocr_batch* b) {
   uint64_t want_cut = std::min<uint64_t>(b->sum_tris * 2 + 64, 0xFFFFFFF0ull);
-  if (b->cuts && b->cap_cut >= want_cut) return OCR_OK;
+  if (b->cuts && b->cap_cut >= want_cut) return ensure_bins_of(b);
   OCR_HIP_CHECK(hipDeviceSynchronize());
   drop_graph(b);
   if (b->cuts) (void)hipFree(b->cuts);
@@ -123,7 +147,7 @@ static int ensure_lists(ocr_batch* b) {
     return fail(OCR_ERR_OOM, std::string("batch cut list: ") + hipGetErrorString(e));
   }
   b->cap_cut = (uint32_t)want_cut;
-  return OCR_OK;
+  return ensure_bins_of(b);
 }
 
 int ocr::batch_reserve(ocr_batch* b, uint64_t sum_tris) {
@@ -204,10 +228,20 @@ static hipError_t note_scene_reads(ocr_batch* b, hipStream_t st) {
 
 static hipError_t enqueue_batch(ocr_batch* b, hipStream_t st) {
   const PassArgs pa = batch_pass(b);
-  hipError_t e = launch_clear(pa, st);
-  if (e == hipSuccess) e = launch_prearrange(pa, st);
-  if (e == hipSuccess) e = launch_depth(pa, (int)b->tune.raster_blocks_per_cu, st);
-  if (e == hipSuccess) e = launch_id(pa, (int)b->tune.raster_blocks_per_cu, st);
+  hipError_t e;
+  if (b->tune.raster_tile) {
+    // counters and tile counts, prearrange (+ boxes and counts), bins, then
+    // depth and ids per tile for every view (the tiles write every pixel)
+    e = launch_bin_reset(pa.lists.tb, pa.lists.counters, st);
+    if (e == hipSuccess) e = launch_prearrange(pa, st);
+    if (e == hipSuccess) e = launch_bins(pa, st);
+    if (e == hipSuccess) e = launch_tile_raster(pa, st);
+  } else {
+    e = launch_clear(pa, st);
+    if (e == hipSuccess) e = launch_prearrange(pa, st);
+    if (e == hipSuccess) e = launch_depth(pa, (int)b->tune.raster_blocks_per_cu, st);
+    if (e == hipSuccess) e = launch_id(pa, (int)b->tune.raster_blocks_per_cu, st);
+  }
   if (e == hipSuccess) e = launch_shade(pa, b->ctx->face_sc, st);
   return e;
 }
@@ -332,6 +366,7 @@ void ocr_batch_destroy(ocr_batch* b) {
   if (b->d_views) (void)hipFree(b->d_views);
   if (b->d_scenes) (void)hipFree(b->d_scenes);
   if (b->cuts) (void)hipFree(b->cuts);
+  free_bins(b->tb, b->box_cap);
   if (b->counters) (void)hipFree(b->counters);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
   if (b->ds_host) (void)hipHostFree(b->ds_host);
@@ -414,9 +449,13 @@ int ocr_batch_set_tuning(ocr_batch* b, const ocr_batch_tuning* t) {
   if (!b || !t || t->raster_blocks_per_cu == 0 || t->raster_blocks_per_cu > 32 || t->shade_tile_w_log2 > 8 ||
       t->groups > 8)
     return fail(OCR_ERR_INVALID, "ocr_batch_set_tuning: bad tuning");
+  if (t->raster_tile) {
+    const uint32_t a = t->raster_tile & 0xFFu, c = t->raster_tile >> 8;
+    if (a < 3 || a > 6 || c < 3 || c > 6 || a + c < 8 || a + c > 11)
+      return fail(OCR_ERR_INVALID, "ocr_batch_set_tuning: raster tiles are 2^3..2^6 on a side and 256..2048 pixels");
+  }
   b->tune = *t;
   b->tune.groups = std::max<uint32_t>(t->groups, 1);
-  b->tune._pad = 0;
   drop_graph(b);
   if (!b->nviews) return OCR_OK;
   int rc = upload_views(b);
@@ -528,9 +567,14 @@ int ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out) {
       c[kPacked + 1] += g[kPacked + 1];
       c[kFragCount] += g[kFragCount];
       c[kOverflow] |= g[kOverflow];
+      // a group whose bins overflowed grows them for its next render
+      if (g[kOverflow] & kOverflowBins)
+        b->subs[k]->want_bin = (uint32_t)std::min<uint64_t>((uint64_t)g[kBinTotal] + g[kBinTotal] / 4 + 64, 0xFFFFFFF0ull);
     }
   } else {
     OCR_HIP_CHECK(hipMemcpy(c, b->counters, sizeof(c), hipMemcpyDeviceToHost));
+    if (c[kOverflow] & kOverflowBins)
+      b->want_bin = (uint32_t)std::min<uint64_t>((uint64_t)c[kBinTotal] + c[kBinTotal] / 4 + 64, 0xFFFFFFF0ull);
   }
   memset(out, 0, sizeof(*out));
   out->n_cut = c[kPacked + 1];
@@ -548,12 +592,14 @@ int ocr_batch_get_stats(ocr_batch* b, void* stream, ocr_frame_stats* out) {
   }
   if (c[kOverflow] & kOverflowSamples)
     return fail(OCR_ERR_UNSUPPORTED, "ocr_batch_get_stats: the batch needs more than 2^31 - 1 raster samples");
-  if (c[kOverflow]) return fail(OCR_ERR_OVERFLOW, "ocr_batch_get_stats: cut list overflowed");
+  if (c[kOverflow]) return fail(OCR_ERR_OVERFLOW, "ocr_batch_get_stats: a work list overflowed (it grows for the next render)");
   return OCR_OK;
 }
 
-// Times every candidate tuning on the current views (graph launches, after
-// one warm-up each) and keeps the fastest.
+// Times candidate tunings on the current views (graph launches, after one
+// warm-up each) and keeps the fastest: raster blocks per CU x shading tile
+// width x view groups with the sample-space raster passes, then, with the
+// best of those, the tiled raster path at each raster tile shape.
 int ocr_batch_autotune(ocr_batch* b, void* stream, uint32_t reps, ocr_batch_tuning* best_out) {
   if (!b || b->nviews == 0) return fail(OCR_ERR_INVALID, "ocr_batch_autotune: no views");
   OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
@@ -562,53 +608,75 @@ int ocr_batch_autotune(ocr_batch* b, void* stream, uint32_t reps, ocr_batch_tuni
   hipEvent_t e0, e1;
   OCR_HIP_CHECK(hipEventCreate(&e0));
   OCR_HIP_CHECK(hipEventCreate(&e1));
-  const uint32_t blocks[] = {2, 4, 8, 16};
-  const uint32_t tiles[] = {3, 4, 5, 6};     // tile widths 8, 16, 32, 64
-  const uint32_t groups[] = {1, 2, 4};
-  ocr_batch_tuning best = b->tune;
-  float best_ms = 1e30f;
   const bool prof = b->profiling;
   b->profiling = false;
+  // one candidate: set it, one warm-up launch, then reps timed launches
+  auto time_one = [&](const ocr_batch_tuning& t, float& ms) -> int {
+    int status = ocr_batch_set_tuning(b, &t);
+    if (!status) status = ocr_batch_launch_graph(b, stream);
+    if (status) return status;
+    hipError_t e = hipSuccess;
+    if (group_count(b) > 1) {
+      e = hipEventRecord(e0, st);
+      for (uint32_t r = 0; r < reps && e == hipSuccess && status == OCR_OK; r++)
+        status = ocr_batch_launch_graph(b, stream);
+      if (status) return status;
+    } else {
+      // the timed replays read the scenes like any launch
+      std::vector<std::unique_lock<std::mutex>> locks;
+      lock_scenes(b->scenes, locks);
+      if ((status = check_scenes(b))) return status;
+      e = bind_scenes(b, st);
+      if (e == hipSuccess) e = hipEventRecord(e0, st);
+      for (uint32_t r = 0; r < reps && e == hipSuccess; r++) e = hipGraphLaunch(b->exec, st);
+      if (e == hipSuccess) e = note_scene_reads(b, st);
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) return fail(OCR_ERR_HIP, std::string("ocr_batch_autotune: ") + hipGetErrorString(e));
+    // a candidate whose lists overflowed is not a result (they grew)
+    ocr_frame_stats fs;
+    status = ocr_batch_get_stats(b, stream, &fs);
+    if (status == OCR_ERR_OVERFLOW) {
+      ms = 1e30f;
+      status = OCR_OK;
+    }
+    return status;
+  };
+  const uint32_t blocks[] = {2, 4, 8, 16};
+  const uint32_t tiles[] = {3, 4, 5, 6};     // shading tile widths 8, 16, 32, 64
+  const uint32_t groups[] = {1, 2, 4};
+  // raster tiles (w_log2 | h_log2 << 8): 16x16, 32x16, 16x32, 32x32, 64x32
+  const uint32_t raster_tiles[] = {4u | 4u << 8, 5u | 4u << 8, 4u | 5u << 8, 5u | 5u << 8, 6u | 5u << 8};
+  ocr_batch_tuning best = b->tune;
+  float best_ms = 1e30f;
   int status = OCR_OK;
   for (uint32_t gr : groups) {
-    if (gr > b->nviews) break;
+    if (gr > b->nviews || status) break;
     for (uint32_t bl : blocks)
-    for (uint32_t tl : tiles) {
-      if (status) break;
-      ocr_batch_tuning t{bl, tl, gr, 0};
-      if ((status = ocr_batch_set_tuning(b, &t))) break;
-      if ((status = ocr_batch_launch_graph(b, stream))) break;
-      hipError_t e = hipSuccess;
-      if (gr > 1) {
-        e = hipEventRecord(e0, st);
-        for (uint32_t r = 0; r < reps && e == hipSuccess && status == OCR_OK; r++)
-          status = ocr_batch_launch_graph(b, stream);
+      for (uint32_t tl : tiles) {
         if (status) break;
-      } else {
-        // the timed replays read the scenes like any launch
-        std::vector<std::unique_lock<std::mutex>> locks;
-        lock_scenes(b->scenes, locks);
-        if ((status = check_scenes(b))) break;
-        e = bind_scenes(b, st);
-        if (e == hipSuccess) e = hipEventRecord(e0, st);
-        for (uint32_t r = 0; r < reps && e == hipSuccess; r++)
-          e = hipGraphLaunch(b->exec, st);
-        if (e == hipSuccess) e = note_scene_reads(b, st);
+        const ocr_batch_tuning t{bl, tl, gr, 0};
+        float ms = 0.f;
+        status = time_one(t, ms);
+        if (!status && ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
       }
-      if (e == hipSuccess) e = hipEventRecord(e1, st);
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      if (e != hipSuccess) {
-        status = fail(OCR_ERR_HIP, std::string("ocr_batch_autotune: ") + hipGetErrorString(e));
-        break;
-      }
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = t;
-      }
-    }
+  }
+  const ocr_batch_tuning base = best;
+  for (uint32_t rt : raster_tiles) {
     if (status) break;
+    ocr_batch_tuning t = base;
+    t.raster_tile = rt;
+    float ms = 0.f;
+    status = time_one(t, ms);
+    if (!status && ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
   }
   b->profiling = prof;
   (void)hipEventDestroy(e0);

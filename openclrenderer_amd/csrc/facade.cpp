@@ -561,8 +561,31 @@ compute_event engine::do_motion_blur(object_context_data& dat, cl_float strength
   return record_event(nullptr);
 }
 
+compute_event engine::draw_tiled_deferred(object_context_data& dat) {
+  if (!dat.scene || !dat.ensure_screen_buffers((int)width, (int)height)) return compute_event();
+  int twl = 3;
+  while (twl < 6 && (1 << twl) < tile_size) twl++;
+  ocr_raster_config rc;
+  memset(&rc, 0, sizeof(rc));
+  rc.mode = OCR_RASTER_TILES;
+  rc.tile_w_log2 = (uint32_t)twl;
+  rc.tile_h_log2 = (uint32_t)std::min(twl, 11 - twl);
+  last_status = ocr_frame_set_raster(dat.frame, &rc);
+  report("ocr_frame_set_raster", last_status);
+  if (last_status) return compute_event();
+  const compute_event ev = draw(dat);
+  // draw_bulk_objs_n keeps the sample-space raster passes
+  rc.mode = OCR_RASTER_SAMPLES;
+  (void)ocr_frame_set_raster(dat.frame, &rc);
+  return ev;
+}
+
 compute_event engine::draw_bulk_objs_n(object_context_data& dat) {
   if (!dat.scene || !dat.ensure_screen_buffers((int)width, (int)height)) return compute_event();
+  return draw(dat);
+}
+
+compute_event engine::draw(object_context_data& dat) {
   sync_lights(dat);
   if (!light_data) {
     last_status = ocr_lights_upload(dat.scene, nullptr, 0, l_size);

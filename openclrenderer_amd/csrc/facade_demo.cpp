@@ -2,7 +2,7 @@
 // draw API of ocr_engine.hpp: containers with load functions, textures,
 // lights, shadow pass, draw, blit, read-back.
 //
-// usage: ocr_facade_demo <scene.bin> <out.bin> [frames] [async|motion]
+// usage: ocr_facade_demo <scene.bin> <out.bin> [frames] [async|motion|tiled]
 //        ocr_facade_demo <scene.bin> <out.bin> <frames> obj <model.obj> <scale>
 //   obj: one more container loads <model.obj> (OBJ + MTL + PNG textures)
 //   through its default load function, objects_container::set_file and
@@ -32,13 +32,15 @@ static bool rd(FILE* f, void* p, size_t n) { return fread(p, 1, n, f) == n; }
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: %s scene.bin out.bin [frames] [async|motion|obj model.obj scale]\n", argv[0]);
+    fprintf(stderr, "usage: %s scene.bin out.bin [frames] [async|motion|tiled|obj model.obj scale]\n", argv[0]);
     return 2;
   }
   int frames = argc > 3 ? atoi(argv[3]) : 2;
   // async: build the first half of the containers, render, then add the
   // rest with an async build while rendering, flipping when it is ready
   const bool async_build = argc > 4 && !strcmp(argv[4], "async");
+  // tiled: every frame drawn by engine::draw_tiled_deferred
+  const bool tiled = argc > 4 && !strcmp(argv[4], "tiled");
   // motion: after the frames, swap_buffers (main.cpp:283), move the camera
   // by (40, -25, 30), draw once more and apply do_motion_blur (main.cpp:270)
   const bool motion = argc > 4 && !strcmp(argv[4], "motion");
@@ -148,7 +150,7 @@ int main(int argc, char** argv) {
   // main.cpp:243-342: the frame loop
   auto frame = [&]() {
     window.generate_realtime_shadowing(*context.fetch());
-    compute_event ev = window.draw_bulk_objs_n(*context.fetch());
+    compute_event ev = tiled ? window.draw_tiled_deferred(*context.fetch()) : window.draw_bulk_objs_n(*context.fetch());
     window.blit_to_screen(*context.fetch());
     window.render_block();
     window.flip();

@@ -129,21 +129,44 @@ struct DevView {
   uint32_t tri_begin;      // first triangle slot of this view (multiple of 256)
   uint32_t pix_begin;      // first pixel slot (multiple of 256)
   uint32_t tile_begin;     // first 16x16 shading tile
+  uint32_t bin_begin;      // first raster tile (tiled raster path)
+  uint32_t bin_tiles_x;    // raster tiles per row
   uint32_t* depth;
   unsigned long long* idkey;
   uint32_t* rgba;
   uint32_t* normals;
 };
 
+// Screen-tile bins of the tiled raster path (tiles.hip): every cut record
+// is listed in each tile of tw x th pixels its bounding box touches.
+// Prearrange writes each record's box and counts the tiles' entries, a scan
+// turns the counts into offsets, a scatter fills the bins, and one
+// workgroup per tile resolves its depth and ids in LDS.
+struct TileBins {
+  uint32_t* cnt;      // per tile: entries (zeroed before prearrange)
+  uint32_t* off;      // per tile: first entry; off[ntiles] = total
+  uint32_t* cur;      // per tile: scatter cursor
+  uint32_t* sums;     // per scan chunk: its tiles' entries
+  uint32_t* bin;      // cut-record indices, grouped by tile
+  uint4* box;         // per cut record: see TileBox
+  uint32_t ntiles, cap_bin;
+  uint32_t tw_log2, th_log2;
+};
+// A cut record's tiles: x = tx0 | tx1 << 16, y = ty0 | ty1 << 16 (inclusive
+// tile columns / rows of its view), z = the view's first tile, w = max_y |
+// tiles_x << 16 | walk_is_rect << 31.
+constexpr uint32_t kBoxRect = 1u << 31;
+
 // The cut list and counters shared by the views of one render.  One
 // allocation of 2 * cap_cut records holds the CutTri records followed by
-// their RasterTri setups.
+// their RasterTri setups.  tb.bin is null unless the render is tiled.
 struct Lists {
   CutTri* cuts;
   RasterTri* rast;
   uint32_t* counters;
   uint32_t cap_cut;
   uint32_t count_frags;   // keep the reference's fragment count (statistics)
+  TileBins tb;
 };
 inline Lists make_lists(CutTri* cuts, uint32_t* counters, uint32_t cap_cut, bool count_frags = true) {
   return Lists{cuts, reinterpret_cast<RasterTri*>(cuts + cap_cut), counters, cap_cut, count_frags ? 1u : 0u};
@@ -179,10 +202,11 @@ enum Counter : int {
   kOverflow = 2,
   kValidEnd = 16,          // shadow passes: u64, end of the list's valid prefix, byte 64
   kFragCount = 32,         // reference fragments (statistics), byte 128
+  kBinTotal = 48,          // tiled path: bin entries the render needed, byte 192
   kCounterWords = 64,
 };
 // Bits of the overflow word: the cut list is full / the sample space is full.
-constexpr uint32_t kOverflowCuts = 1u, kOverflowSamples = 2u;
+constexpr uint32_t kOverflowCuts = 1u, kOverflowSamples = 2u, kOverflowBins = 4u;
 // Raster samples per render, and per cut triangle.  Frames and shadow faces
 // hold at most 16,000,000 pixels, so a triangle has at most
 // area * (1 + 1/op) + op + 1 < 2^24 samples: sample indices, bounding-box
@@ -344,6 +368,9 @@ struct Frame {
     CutTri* cuts = nullptr;
     uint32_t* counters = nullptr;
     uint32_t cap_cut = 0;
+    // tiled raster path (tiles.hip): bins of this set's cut list
+    TileBins tb{};
+    uint32_t box_cap = 0;   // records the box array holds
   };
   static constexpr int kSets = 2;   // three measured slower: renders compete for VALU
   OutBuf out[kSets];
@@ -357,6 +384,10 @@ struct Frame {
   hipEvent_t done[kSets] = {};  // a render of the set finished
   bool entry_valid[kSets] = {};
   uint32_t want_cut = 0;             // capacity set by ocr_frame_reserve (0 = automatic)
+  // raster path of ocr_render (ocr_frame_set_raster): the sample space or
+  // screen tiles of 2^tw_log2 x 2^th_log2 pixels, with bins of want_bin
+  // entries (0 = automatic; raised after an overflow)
+  uint32_t raster_mode = OCR_RASTER_SAMPLES, tw_log2 = 4, th_log2 = 4, want_bin = 0;
   uint32_t* post_in = nullptr;       // colour snapshot of the screen-space post passes
   bool profiling = false;
   static constexpr int kEvents = 6;       // prearrange, clear, depth, id, shade boundaries
@@ -382,6 +413,14 @@ struct Frame {
 
 // Allocates output set b of a frame (idempotent); counters zeroed.
 int frame_alloc_set(Frame* f, int b);
+// Tile bins for a cut list of cap_cut records over `ntiles` tiles of
+// 2^twl x 2^thl pixels with room for cap_bin entries: (re)allocates what is
+// missing or too small (synchronising the device first), zeroes the counts.
+int ensure_bins(TileBins& tb, uint32_t& box_cap, uint32_t cap_cut, uint32_t ntiles, uint32_t twl, uint32_t thl,
+                uint32_t cap_bin, bool exact);
+void free_bins(TileBins& tb, uint32_t& box_cap);
+hipError_t launch_bin_reset(const TileBins& tb, uint32_t* counters, hipStream_t s);
+uint32_t bin_scan_chunks(uint32_t ntiles);
 
 // ---- error plumbing -------------------------------------------------------
 void set_error(const std::string& msg);
@@ -398,6 +437,14 @@ hipError_t launch_clear(const PassArgs& a, hipStream_t s);
 hipError_t launch_prearrange(const PassArgs& a, hipStream_t s);
 hipError_t launch_depth(const PassArgs& a, int blocks_per_cu, hipStream_t s);
 hipError_t launch_id(const PassArgs& a, int blocks_per_cu, hipStream_t s);
+// Tiled raster path: bin scan and scatter, then depth and ids per tile.
+hipError_t launch_bins(const PassArgs& a, hipStream_t s);
+hipError_t launch_tile_raster(const PassArgs& a, hipStream_t s);
+// Raster tiles of a view for tiles of 2^twl x 2^thl pixels.
+__host__ __device__ inline uint32_t bin_tiles_x(uint32_t w, uint32_t twl) { return (w + (1u << twl) - 1) >> twl; }
+__host__ __device__ inline uint32_t bin_tiles(uint32_t w, uint32_t h, uint32_t twl, uint32_t thl) {
+  return bin_tiles_x(w, twl) * ((h + (1u << thl) - 1) >> thl);
+}
 // sin/cos of a rotation's three angles: binary64 evaluation rounded once
 void sincos_of(const float ang[3], SinCos& sc);
 hipError_t launch_shade(const PassArgs& a, const SinCos face[6], hipStream_t s);

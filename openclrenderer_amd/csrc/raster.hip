@@ -27,19 +27,10 @@
 #include <algorithm>
 
 #include "ocr_internal.h"
+#include "tiles.h"
+#include "walk.h"
 
 namespace ocr {
-
-// ------------------------------------------------------------ wave helpers
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    uint32_t t = __shfl_up(v, d, kWave);
-    if (lane >= d) v += t;
-  }
-  return v;
-}
 
 // Prearrange blocks: one triangle per work item.  Cut records and sample
 // ranges are allocated once per block (one 64-bit atomicAdd on the packed
@@ -113,29 +104,6 @@ __device__ __forceinline__ bool block_alloc(const Lists& L, BlockAllocLds<kWaves
 }
 
 // ---------------------------------------------------------- setup helpers
-// The walk's row of sample q: floor(mad(q, 1/width, min_y)) (cl2.cl:5083).
-// Sample indices are integers below 2^24, exact in binary32.
-__device__ __forceinline__ float row_of(float q, float iw, float min_y) {
-  return floorf(fmaf(q, iw, min_y));
-}
-
-// First sample q >= 0 whose row is >= Y, from the nominal n0 = (Y - min_y) * w
-// (exact: an integer below 2^24).  The fma rounds across a row boundary for at
-// most a few samples, so the steps below run only on such rows.
-__device__ __forceinline__ float run_start_steps(float Y, float q, float iw, float min_y) {
-#pragma clang loop unroll(disable)
-  for (int g = 0; g < 64 && q > 0.f && row_of(q - 1.f, iw, min_y) >= Y; g++) q -= 1.f;
-#pragma clang loop unroll(disable)
-  for (int g = 0; g < 64 && row_of(q, iw, min_y) < Y; g++) q += 1.f;
-  return q;
-}
-
-__device__ __forceinline__ float run_start(float Y, float w, float iw, float min_y) {
-  const float n0 = (Y - min_y) * w;
-  const bool lead = n0 > 0.f && row_of(n0 - 1.f, iw, min_y) >= Y;
-  const bool lag = row_of(n0, iw, min_y) < Y;
-  return (lead || lag) ? run_start_steps(Y, n0, iw, min_y) : n0;
-}
 
 // Setup of one cut triangle for the raster passes (kernel1 cl2.cl:5000-5046).
 __device__ __forceinline__ void raster_setup(const f3 p[3], float rc, float ew, float eh, RasterTri& t,
@@ -294,14 +262,18 @@ __device__ __forceinline__ int face_clip_project(const f3 w[3], f3 cpos, const S
 // pass the list capacity or the sample limit writes nothing and sets the
 // overflow word, which every later pass of the render checks before reading
 // the list.
+// For a tiled render (L.tb.bin set) also the records' tile boxes and their
+// tiles' entry counts.
 template <bool kCamera, int kWaves>
 __device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds<kWaves>& sh, const f3 tp[2][3],
                                             const float rc[2], RasterTri rt[2], const uint32_t nf[2],
                                             const uint32_t ns[2], uint32_t key0, uint32_t key_step, uint32_t oid,
-                                            uint32_t view) {
+                                            uint32_t view, const float* my = nullptr, uint32_t bin_begin = 0,
+                                            uint32_t tiles_x = 0) {
   const uint32_t k0 = ns[0] > 0, k1 = ns[1] > 0;
   uint32_t c, smp;
   if (!block_alloc(L, sh, k0 + k1, ns[0] + ns[1], nf[0] * k0 + nf[1] * k1, c, smp)) return;
+  uint4 box[2] = {};
 #pragma unroll
   for (int i = 0; i < 2; i++) {
     if (!ns[i]) continue;
@@ -311,22 +283,42 @@ __device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds<kWaves
     rt[i].key = key;
     rt[i].sample_begin = smp;
     L.rast[c] = rt[i];
+    if constexpr (kCamera) {
+      if (L.tb.bin) {
+        box[i] = tile_box(rt[i], my[i], L.tb, bin_begin, tiles_x);
+        L.tb.box[c] = box[i];
+      }
+    }
     smp += ns[i];
     c++;
+  }
+  if constexpr (kCamera) {
+    if (L.tb.bin) {
+      uint32_t* cnt = L.tb.cnt;
+      const uint32_t ntiles = L.tb.ntiles;
+      expand_tile_pairs(
+          box[0], k0 != 0, box[1], k1 != 0,
+          [cnt, ntiles](uint32_t t, uint32_t len) {
+            if (t < ntiles) atomicAdd(&cnt[t], len);
+            return 0u;
+          },
+          [](uint32_t, int, uint32_t, uint32_t) {});
+    }
   }
 }
 
 // Cull, count and set up the (up to 2) clipped pieces of one triangle.
+// my[i]: piece i's max_y (its box's end row).
 __device__ __forceinline__ void cut_pieces(const f3 tp[2][3], int num, float ew, float eh, bool two_sided, int op,
-                                           float rc[2], RasterTri rt[2], uint32_t nf[2], uint32_t ns[2]) {
+                                           float rc[2], RasterTri rt[2], uint32_t nf[2], uint32_t ns[2],
+                                           float my[2]) {
 #pragma unroll
   for (int i = 0; i < 2; i++) {
     if (i >= num) continue;
     nf[i] = cull_count(tp[i], ew, eh, two_sided, op, rc[i]);
     if (!nf[i]) continue;
-    float max_y;
-    raster_setup(tp[i], rc[i], ew, eh, rt[i], max_y);
-    ns[i] = triangle_samples(rt[i], max_y, op, nf[i]);
+    raster_setup(tp[i], rc[i], ew, eh, rt[i], my[i]);
+    ns[i] = triangle_samples(rt[i], my[i], op, nf[i]);
   }
 }
 
@@ -404,6 +396,7 @@ __global__ void __launch_bounds__(kPreThreads) k_prearrange(PassArgs a) {
   float rc[2] = {0.f, 0.f};
   RasterTri rt[2];
   uint32_t nf[2] = {0u, 0u}, ns[2] = {0u, 0u};
+  float my[2] = {0.f, 0.f};
   uint32_t oid = 0;
   if (id < S.ntri) {
     const TriPos T = S.world[id];
@@ -414,11 +407,11 @@ __global__ void __launch_bounds__(kPreThreads) k_prearrange(PassArgs a) {
       if (!(fast_length(mk3(G.wp[0], G.wp[1], G.wp[2]) - cpos) > kDepthFar)) {
         const f3 w[3] = {mk3(T.p[0], T.p[1], T.p[2]), mk3(T.p[3], T.p[4], T.p[5]), mk3(T.p[6], T.p[7], T.p[8])};
         const int num = face_clip_project(w, cpos, v.cam, v.fov, ew, eh, tp);
-        cut_pieces(tp, num, ew, eh, (G.flags & OCR_FEATURE_TWO_SIDED) > 0, kOpSize, rc, rt, nf, ns);
+        cut_pieces(tp, num, ew, eh, (G.flags & OCR_FEATURE_TWO_SIDED) > 0, kOpSize, rc, rt, nf, ns, my);
       }
     }
   }
-  append_cuts<true>(a.lists, sh, tp, rc, rt, nf, ns, 2u * id, 1u, oid, vi);
+  append_cuts<true>(a.lists, sh, tp, rc, rt, nf, ns, 2u * id, 1u, oid, vi, my, V.bin_begin, V.bin_tiles_x);
 }
 
 hipError_t launch_prearrange(const PassArgs& a, hipStream_t s) {
@@ -468,41 +461,6 @@ struct TriLds {
   unsigned long long* ids[kBatch && kMode == RasterMode::Id ? kWave : 1];
 };
 
-// point_in_tri (cl2.cl:4798-4807) with the triangle's loop-invariant terms.
-__device__ __forceinline__ bool inside_tri(float x, float y, float e0, float e1, float e2, float e3, float e4,
-                                           float e5, float e6) {
-  const float s_ = e0 + e1 * x + e2 * y;
-  const float t_ = e3 + e4 * x + e5 * y;
-  return s_ > -0.0001f && t_ > -0.0001f && (s_ + t_) < e6;
-}
-
-// Sample `j` (index within its triangle, an exact binary32 integer) of a
-// set-up triangle: its column x, row y and whether it lies inside.  Sample j
-// is sample o = j % (op + 1) of fragment a = j / (op + 1), at walk index
-// q = op*a + o; the quotient is a float estimate corrected by one step each
-// way.
-template <int kOp>
-__device__ __forceinline__ bool sample_xy(float slf, float min_x, float min_y, float w, float iw, float e0, float e1,
-                                          float e2, float e3, float e4, float e5, float e6, float& x, float& y) {
-  float faf = floorf(slf * (1.0f / (float)(kOp + 1)));
-  float of = fmaf(-faf, (float)(kOp + 1), slf);
-  faf = of < 0.f ? faf - 1.f : (of >= (float)(kOp + 1) ? faf + 1.f : faf);
-  of = fmaf(-faf, (float)(kOp + 1), slf);
-  const float paf = faf * (float)kOp;
-  const float qf = paf + of;
-  y = row_of(qf, iw, min_y);
-  // cursor column: (r % w) + min_x + (q - r), r = max(pa, row start)
-  const float rs = run_start(y, w, iw, min_y);
-  const float r = paf > rs ? paf : rs;
-  const float m = floorf(r * iw);
-  float e = fmaf(-m, w, r);
-  e = e < 0.f ? e + w : e;
-  e = e >= w ? e - w : e;
-  const float xo = e + (qf - r);
-  x = min_x + xo;
-  if (xo >= w) return false;   // x >= max_x
-  return inside_tri(x, y, e0, e1, e2, e3, e4, e5, e6);
-}
 
 // Whether sample_xy reduces, for every sample of a set-up triangle of n
 // samples, to its short form
@@ -900,7 +858,8 @@ __global__ void __launch_bounds__(kShadowPreThreads) k_prearrange_shadow(const T
 #pragma unroll
         for (int k = 0; k < 3; k++) w[k] = mk3(S.wpos[lt][3 * k], S.wpos[lt][3 * k + 1], S.wpos[lt][3 * k + 2]);
         const int num = face_clip_project(w, lpos, ft.f[face], efov, ew, eh, tp);
-        cut_pieces(tp, num, ew, eh, ((it >> 3) & 1u) != 0u, kOpSizeLight, rc, rt, nf, ns);
+        float my[2];
+        cut_pieces(tp, num, ew, eh, ((it >> 3) & 1u) != 0u, kOpSizeLight, rc, rt, nf, ns, my);
       }
       const uint32_t key = key0 + face;
       // pieces above local_max samples to the list, in allocations of at

@@ -149,7 +149,71 @@ static PassArgs single_pass(const Scene* s, const ocr_view* view, Frame* f) {
   a.tiles = view_tiles(view->width, view->height, a.tile_w_log2);
   const Frame::OutBuf& ob = f->out[f->cur];
   a.lists = make_lists(ob.cuts, ob.counters, ob.cap_cut);
+  if (f->raster_mode == OCR_RASTER_TILES && ob.tb.bin) {
+    a.lists.tb = ob.tb;
+    a.view0.bin_begin = 0;
+    a.view0.bin_tiles_x = bin_tiles_x(view->width, ob.tb.tw_log2);
+  }
   return a;
+}
+
+void free_bins(TileBins& tb, uint32_t& box_cap) {
+  dev_free(tb.cnt);
+  dev_free(tb.off);
+  dev_free(tb.cur);
+  dev_free(tb.sums);
+  dev_free(tb.bin);
+  dev_free(tb.box);
+  tb.ntiles = tb.cap_bin = 0;
+  box_cap = 0;
+}
+
+int ensure_bins(TileBins& tb, uint32_t& box_cap, uint32_t cap_cut, uint32_t ntiles, uint32_t twl, uint32_t thl,
+                uint32_t cap_bin, bool exact) {
+  const bool tiles_ok = tb.cnt && tb.ntiles == ntiles;
+  const bool bins_ok = tb.bin && (exact ? tb.cap_bin == cap_bin : tb.cap_bin >= cap_bin);
+  const bool box_ok = tb.box && box_cap >= cap_cut;
+  tb.tw_log2 = twl;
+  tb.th_log2 = thl;
+  if (tiles_ok && bins_ok && box_ok) return OCR_OK;
+  OCR_HIP_CHECK(hipDeviceSynchronize());
+  int rc;
+  if (!tiles_ok) {
+    dev_free(tb.cnt);
+    dev_free(tb.off);
+    dev_free(tb.cur);
+    dev_free(tb.sums);
+      tb.ntiles = 0;
+    if ((rc = dev_alloc(&tb.cnt, ntiles, "tile counts")) || (rc = dev_alloc(&tb.off, (size_t)ntiles + 1, "tile offsets")) ||
+        (rc = dev_alloc(&tb.cur, ntiles, "tile cursors")) ||
+        (rc = dev_alloc(&tb.sums, 2 * (size_t)bin_scan_chunks(ntiles), "tile scan sums")))
+      return rc;
+    OCR_HIP_CHECK(hipMemset(tb.cnt, 0, 4 * (size_t)ntiles));
+    tb.ntiles = ntiles;
+  }
+  if (!bins_ok) {
+    dev_free(tb.bin);
+    tb.cap_bin = 0;
+    if ((rc = dev_alloc(&tb.bin, cap_bin, "tile bins"))) return rc;
+    tb.cap_bin = cap_bin;
+  }
+  if (!box_ok) {
+    dev_free(tb.box);
+    box_cap = 0;
+    if ((rc = dev_alloc(&tb.box, cap_cut, "tile boxes"))) return rc;
+    box_cap = cap_cut;
+  }
+  return OCR_OK;
+}
+
+// Bins of output set b for the frame's tile size: by default two entries
+// per cut-list slot; an overflow raises want_bin for the renders after it.
+static int ensure_frame_bins(Frame* f, int b) {
+  Frame::OutBuf& o = f->out[b];
+  const uint32_t ntiles = bin_tiles(f->width, f->height, f->tw_log2, f->th_log2);
+  const uint64_t dflt = std::max<uint64_t>(2ull * o.cap_cut, 4096ull);
+  const uint32_t cap = (uint32_t)std::min<uint64_t>(f->want_bin ? (uint64_t)f->want_bin : dflt, 0xFFFFFFF0ull);
+  return ensure_bins(o.tb, o.box_cap, o.cap_cut, ntiles, f->tw_log2, f->th_log2, cap, f->want_bin != 0);
 }
 
 int frame_alloc_set(Frame* f, int b) {
@@ -798,6 +862,7 @@ void ocr_frame_destroy(ocr_frame* f) {
     dev_free(o.normals);
     dev_free(o.cuts);
     dev_free(o.counters);
+    free_bins(o.tb, o.box_cap);
   }
   dev_free(f->post_in);
   for (int b = 0; b < Frame::kSets; b++) {
@@ -819,6 +884,31 @@ int ocr_frame_set_profiling(ocr_frame* f, int enable) {
     f->ev.assign((size_t)Frame::kRing * Frame::kEvents, nullptr);
     for (auto& e : f->ev) OCR_HIP_CHECK(hipEventCreate(&e));
   }
+  return OCR_OK;
+}
+
+int ocr_frame_set_raster(ocr_frame* f, const ocr_raster_config* cfg) {
+  if (!f || !cfg) return fail(OCR_ERR_INVALID, "ocr_frame_set_raster: null argument");
+  if (cfg->mode != OCR_RASTER_SAMPLES && cfg->mode != OCR_RASTER_TILES)
+    return fail(OCR_ERR_INVALID, "ocr_frame_set_raster: unknown mode");
+  if (cfg->mode == OCR_RASTER_TILES) {
+    const uint32_t a = cfg->tile_w_log2, b = cfg->tile_h_log2;
+    if (a < 3 || a > 6 || b < 3 || b > 6 || a + b < 8 || a + b > 11)
+      return fail(OCR_ERR_INVALID, "ocr_frame_set_raster: tiles are 2^3..2^6 on a side and 256..2048 pixels");
+    f->tw_log2 = a;
+    f->th_log2 = b;
+    f->want_bin = cfg->bin_capacity;
+  }
+  f->raster_mode = cfg->mode;
+  return OCR_OK;
+}
+
+int ocr_frame_get_raster(ocr_frame* f, ocr_raster_config* cfg) {
+  if (!f || !cfg) return fail(OCR_ERR_INVALID, "ocr_frame_get_raster: null argument");
+  cfg->mode = f->raster_mode;
+  cfg->tile_w_log2 = f->tw_log2;
+  cfg->tile_h_log2 = f->th_log2;
+  cfg->bin_capacity = f->want_bin;
   return OCR_OK;
 }
 
@@ -865,8 +955,10 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
   // this render's buffer set: the one the previous render did not use
   const int prev = f->cur, b = (f->cur + 1) % Frame::kSets;
+  const bool tiled = f->raster_mode == OCR_RASTER_TILES;
   int rc = frame_alloc_set(f, b);
   if (!rc) rc = ensure_frame_list(f, b, s->ntri);
+  if (!rc && tiled) rc = ensure_frame_bins(f, b);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (!s->lights) {
@@ -895,16 +987,31 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   const PassArgs pa = single_pass(s, view, f);
   PassArgs pc = pa;
   pc.lists.counters = nullptr;   // zeroed before prearrange below
-  OCR_HIP_CHECK(launch_zero_counters(o.counters, sb));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[0], sb));
-  OCR_HIP_CHECK(launch_prearrange(pa, sb));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[1], sb));
-  OCR_HIP_CHECK(launch_clear(pc, sb));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[2], sb));
-  OCR_HIP_CHECK(launch_depth(pa, kRasterBlocksPerCu, sb));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], sb));
-  OCR_HIP_CHECK(launch_id(pa, kRasterBlocksPerCu, sb));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
+  if (tiled) {
+    // counters and tile counts, prearrange (+ boxes and counts), bin scan
+    // and scatter, then depth and ids per tile (no clear: tiles write
+    // every pixel)
+    OCR_HIP_CHECK(launch_bin_reset(o.tb, o.counters, sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[0], sb));
+    OCR_HIP_CHECK(launch_prearrange(pa, sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[1], sb));
+    OCR_HIP_CHECK(launch_bins(pa, sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[2], sb));
+    OCR_HIP_CHECK(launch_tile_raster(pa, sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
+  } else {
+    OCR_HIP_CHECK(launch_zero_counters(o.counters, sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[0], sb));
+    OCR_HIP_CHECK(launch_prearrange(pa, sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[1], sb));
+    OCR_HIP_CHECK(launch_clear(pc, sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[2], sb));
+    OCR_HIP_CHECK(launch_depth(pa, kRasterBlocksPerCu, sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], sb));
+    OCR_HIP_CHECK(launch_id(pa, kRasterBlocksPerCu, sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
+  }
   // Shading reads the shadow maps: it waits for the scene's last shadow
   // pass, wherever that was enqueued (the raster passes do not read them).
   OCR_HIP_CHECK(s->wait_shadow_maps(sb));
@@ -1069,10 +1176,16 @@ int ocr_frame_read(ocr_frame* f, uint32_t* depth, uint32_t* tri_id, uint8_t* rgb
   if (c[kOverflow] & kOverflowSamples)
     return fail(OCR_ERR_UNSUPPORTED, "ocr_frame_read: the render needs more than 2^31 - 1 raster samples");
   if (c[kOverflow]) {
-    // A reserved cut list was too small: size the next render's list for
-    // this one's count (high word of the packed counter).
-    f->want_cut = (uint32_t)std::min<uint64_t>((uint64_t)c[kPacked + 1] + c[kPacked + 1] / 4 + 64, 0xFFFFFFF0ull);
-    return fail(OCR_ERR_OVERFLOW, "ocr_frame_read: cut list overflowed; re-render the frame");
+    // A reserved cut list (or the tile bins) was too small: size the next
+    // render's for this one's count (high word of the packed counter, the
+    // bin total).
+    if (c[kOverflow] & kOverflowCuts)
+      f->want_cut = (uint32_t)std::min<uint64_t>((uint64_t)c[kPacked + 1] + c[kPacked + 1] / 4 + 64, 0xFFFFFFF0ull);
+    if (c[kOverflow] & kOverflowBins)
+      f->want_bin = (uint32_t)std::min<uint64_t>((uint64_t)c[kBinTotal] + c[kBinTotal] / 4 + 64, 0xFFFFFFF0ull);
+    return fail(OCR_ERR_OVERFLOW, (c[kOverflow] & kOverflowCuts)
+                                      ? "ocr_frame_read: cut list overflowed; re-render the frame"
+                                      : "ocr_frame_read: tile bins overflowed; re-render the frame");
   }
   size_t npix = (size_t)f->width * f->height;
   if (depth) OCR_HIP_CHECK(hipMemcpy(depth, f->depth, 4 * npix, hipMemcpyDeviceToHost));

@@ -22,7 +22,8 @@ EXPORTS = [
     "ocr_scene_read_shadow_maps", "ocr_scene_read_motion", "ocr_frame_create", "ocr_frame_destroy",
     "ocr_frame_set_profiling", "ocr_render", "ocr_render_noalloc", "ocr_frame_read", "ocr_frame_device_buffers",
     "ocr_frame_blend_with_depth", "ocr_frame_pseudo_aa", "ocr_frame_motion_blur",
-    "ocr_frame_get_stats", "ocr_frame_count_texels", "ocr_frame_reserve",
+    "ocr_frame_get_stats", "ocr_frame_count_texels", "ocr_frame_reserve", "ocr_frame_set_raster",
+    "ocr_frame_get_raster",
     "ocr_batch_create", "ocr_batch_destroy", "ocr_batch_set_views", "ocr_batch_set_tuning",
     "ocr_batch_get_tuning", "ocr_batch_set_profiling", "ocr_batch_render", "ocr_batch_launch_graph",
     "ocr_batch_get_stats", "ocr_batch_autotune",
@@ -43,7 +44,15 @@ class TexSrc(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("raster_blocks_per_cu", C.c_uint32), ("shade_tile_w_log2", C.c_uint32),
-                ("groups", C.c_uint32), ("_pad", C.c_uint32)]
+                ("groups", C.c_uint32), ("raster_tile", C.c_uint32)]
+
+
+class RasterConfig(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("tile_w_log2", C.c_uint32), ("tile_h_log2", C.c_uint32),
+                ("bin_capacity", C.c_uint32)]
+
+
+RASTER_SAMPLES, RASTER_TILES = 0, 1
 
 
 class FrameStats(C.Structure):
@@ -109,6 +118,8 @@ def load_library(path=LIB_PATH):
     L.ocr_frame_destroy.restype = None
     L.ocr_frame_set_profiling.argtypes = [vp, i32]
     L.ocr_frame_reserve.argtypes = [vp, u32]
+    L.ocr_frame_set_raster.argtypes = [vp, C.POINTER(RasterConfig)]
+    L.ocr_frame_get_raster.argtypes = [vp, C.POINTER(RasterConfig)]
     L.ocr_render.argtypes = [vp, vp, vp, vp]
     L.ocr_render_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.ocr_frame_read.argtypes = [vp, vp, vp, vp, vp, vp]
@@ -295,6 +306,18 @@ class Frame:
     def set_profiling(self, on=True):
         check(self.lib.ocr_frame_set_profiling(self.h, 1 if on else 0))
 
+    def set_raster(self, mode, tile_w_log2=4, tile_h_log2=4, bin_capacity=0):
+        """RASTER_SAMPLES (the sample-space depth and id passes) or
+        RASTER_TILES (tiles of 2^tile_w_log2 x 2^tile_h_log2 pixels resolved
+        in LDS); both give the same buffers."""
+        c = RasterConfig(mode, tile_w_log2, tile_h_log2, bin_capacity)
+        check(self.lib.ocr_frame_set_raster(self.h, C.byref(c)))
+
+    def raster(self):
+        c = RasterConfig()
+        check(self.lib.ocr_frame_get_raster(self.h, C.byref(c)))
+        return c.mode, c.tile_w_log2, c.tile_h_log2, c.bin_capacity
+
     def count_texels(self, gscene, view, stream=None):
         v = np.ascontiguousarray(view, dtype=abi.VIEW)
         a, s = C.c_uint64(), C.c_uint64()
@@ -350,15 +373,17 @@ class Batch:
         self.frames = frames
         check(self.lib.ocr_batch_set_views(self.h, C.cast(sc, C.c_void_p), _p(vv), C.cast(fr, C.c_void_p), n))
 
-    def set_tuning(self, raster_blocks_per_cu, shade_tile_w_log2, groups=1):
-        t = BatchTuning(raster_blocks_per_cu, shade_tile_w_log2, groups)
+    def set_tuning(self, raster_blocks_per_cu, shade_tile_w_log2, groups=1, raster_tile=0):
+        """raster_tile: 0 for the sample-space raster passes, else
+        tile_w_log2 | tile_h_log2 << 8 for the tiled raster path."""
+        t = BatchTuning(raster_blocks_per_cu, shade_tile_w_log2, groups, raster_tile)
         check(self.lib.ocr_batch_set_tuning(self.h, C.byref(t)))
 
     def tuning(self):
-        """(raster blocks per CU, shading tile width log2, view groups)."""
+        """(raster blocks per CU, shading tile width log2, view groups, raster tile)."""
         t = BatchTuning()
         check(self.lib.ocr_batch_get_tuning(self.h, C.byref(t)))
-        return t.raster_blocks_per_cu, t.shade_tile_w_log2, max(t.groups, 1)
+        return t.raster_blocks_per_cu, t.shade_tile_w_log2, max(t.groups, 1), t.raster_tile
 
     def set_profiling(self, on=True):
         check(self.lib.ocr_batch_set_profiling(self.h, 1 if on else 0))
@@ -377,7 +402,7 @@ class Batch:
     def autotune(self, reps=5, stream=None):
         t = BatchTuning()
         check(self.lib.ocr_batch_autotune(self.h, stream, reps, C.byref(t)))
-        return t.raster_blocks_per_cu, t.shade_tile_w_log2, max(t.groups, 1)
+        return t.raster_blocks_per_cu, t.shade_tile_w_log2, max(t.groups, 1), t.raster_tile
 
     def render_checked(self, stream=None, graph=False):
         """Render, and once more if the shared lists overflowed (they grow)."""

@@ -110,6 +110,10 @@ int ocr_frame_create(ocr_ctx* ctx, uint32_t w, uint32_t h, ocr_frame** out) {
   return OCR_OK;
 }
 void ocr_frame_destroy(ocr_frame* f) { delete f; }
+int ocr_frame_set_raster(ocr_frame* f, const ocr_raster_config* c) {
+  if (!f || !c || c->mode > OCR_RASTER_TILES) return fail(OCR_ERR_INVALID, "ocr_frame_set_raster");
+  return OCR_OK;
+}
 
 int ocr_render(ocr_scene* s, const ocr_view* v, ocr_frame* f, void*) {
   if (!s || !v || !f || v->width != f->w || v->height != f->h) return fail(OCR_ERR_INVALID, "ocr_render");

@@ -60,21 +60,26 @@ def test_batch_graph_replay_and_autotune(gpu_ctx):
             got = f.read()
             for k in a:
                 assert np.array_equal(got[k], a[k]), k
-    blocks, tile, groups = b.autotune(reps=2)
+    blocks, tile, groups, _rt = b.autotune(reps=2)
     assert 1 <= blocks <= 32 and 0 <= tile <= 8 and groups in (1, 2, 4)
     b.render_checked(graph=True)
     for f, a in zip(frames, first):
         got = f.read()
         for k in a:
             assert np.array_equal(got[k], a[k]), k
-    for bl, tl, gr in ((2, 3, 1), (16, 6, 2), (4, 5, 4), (8, 4, 3)):
-        b.set_tuning(bl, tl, gr)
+    # raster_tile != 0: the tiled raster path (tiles of 2^(rt & 255) x 2^(rt >> 8))
+    for bl, tl, gr, rt in ((2, 3, 1, 0), (16, 6, 2, 0), (4, 5, 4, 0), (8, 4, 3, 0), (8, 4, 1, 4 | 4 << 8),
+                           (8, 5, 2, 5 | 4 << 8), (8, 4, 4, 5 | 5 << 8), (8, 4, 1, 3 | 5 << 8), (8, 4, 3, 6 | 5 << 8)):
+        b.set_tuning(bl, tl, gr, rt)
+        assert b.tuning()[3] == rt
         for graph in (False, True):
             b.render_checked(graph=graph)
             for f, a in zip(frames, first):
                 got = f.read()
                 for k in a:
-                    assert np.array_equal(got[k], a[k]), (bl, tl, gr, graph, k)
+                    assert np.array_equal(got[k], a[k]), (bl, tl, gr, rt, graph, k)
+    with pytest.raises(runtime.OcrError):
+        b.set_tuning(8, 4, 1, 6 | 6 << 8)   # 4096-pixel tiles: beyond the 2048-pixel limit
     b.close()
 
 

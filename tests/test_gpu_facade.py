@@ -19,14 +19,15 @@ MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "obj_
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shadow,mode", [(0, "sync"), (1, "sync"), (1, "async"), (0, "motion")])
+@pytest.mark.parametrize("shadow,mode", [(0, "sync"), (1, "sync"), (1, "async"), (0, "motion"), (1, "tiled")])
 def test_facade_demo_matches_oracle(oracle, tmp_path, shadow, mode):
     """mode async: half the containers are built and drawn first, the rest
     are added by an asynchronous object_context::build while frames of the
     previous scene are drawn, and flip() makes the full scene current.
     mode motion: swap_buffers records the camera, the camera moves, one more
     frame is drawn and engine::do_motion_blur blurs it (objects still, so the
-    motion is the camera's alone)."""
+    motion is the camera's alone).  mode tiled: the frames are drawn by
+    engine::draw_tiled_deferred (screen tiles of engine::tile_size)."""
     sc, view = scenes.config1(shadow=shadow)
     scene_bin = tmp_path / "scene.bin"
     out_bin = tmp_path / "out.bin"

@@ -139,7 +139,8 @@ def test_facade_under_asan_ubsan(bins):
     s1, s2 = _scene_files(d)
     model = os.path.join(ROOT, "tests", "golden", "obj_model", "model.obj")
     for args in ([s1, str(d / "o.bin"), "2"], [s1, str(d / "o.bin"), "2", "async"],
-                 [s1, str(d / "o.bin"), "2", "motion"], [s2, str(d / "o.bin"), "2", "obj", model, "100"]):
+                 [s1, str(d / "o.bin"), "2", "motion"], [s1, str(d / "o.bin"), "2", "tiled"],
+                 [s2, str(d / "o.bin"), "2", "obj", model, "100"]):
         rc, out = _run([bins["facade_asan"], *args], timeout=300)
         _clean(rc, out, " ".join(args[3:]) or "sync")
 
