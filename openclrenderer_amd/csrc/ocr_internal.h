@@ -157,6 +157,20 @@ struct TileBins {
 // tiles_x << 16 | walk_is_rect << 31.
 constexpr uint32_t kBoxRect = 1u << 31;
 
+// Covered-sample entries of a single view's render: the depth pass writes,
+// per chunk of kChunk raster samples, each covered sample that can still
+// fall in the id pass's window ({pixel, depth, record, 2*tri + piece + 1}),
+// up to kEntCap, and the chunk's count; the id pass reads the entries
+// instead of sweeping the chunk's samples again (chunks with more, or past
+// `chunks`, are swept).
+constexpr uint32_t kChunk = 64 * 16;   // raster samples per wavefront work item
+constexpr uint32_t kEntCap = 512;
+struct CoverEntries {
+  uint4* ent;         // chunks x kEntCap
+  uint32_t* cnt;      // per chunk
+  uint32_t chunks;
+};
+
 // The cut list and counters shared by the views of one render.  One
 // allocation of 2 * cap_cut records holds the CutTri records followed by
 // their RasterTri setups.  tb.bin is null unless the render is tiled.
@@ -167,6 +181,7 @@ struct Lists {
   uint32_t cap_cut;
   uint32_t count_frags;   // keep the reference's fragment count (statistics)
   TileBins tb;
+  CoverEntries ent;
 };
 inline Lists make_lists(CutTri* cuts, uint32_t* counters, uint32_t cap_cut, bool count_frags = true) {
   return Lists{cuts, reinterpret_cast<RasterTri*>(cuts + cap_cut), counters, cap_cut, count_frags ? 1u : 0u};
@@ -371,6 +386,8 @@ struct Frame {
     // tiled raster path (tiles.hip): bins of this set's cut list
     TileBins tb{};
     uint32_t box_cap = 0;   // records the box array holds
+    // covered-sample entries of the sample-space depth pass
+    CoverEntries ent{};
   };
   static constexpr int kSets = 2;   // three measured slower: renders compete for VALU
   OutBuf out[kSets];

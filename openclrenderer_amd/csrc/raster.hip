@@ -433,7 +433,6 @@ hipError_t launch_prearrange(const PassArgs& a, hipStream_t s) {
 // (Y - min_y) * width by stepping while y disagrees, so rows whose fma
 // rounding lags or leads by a sample are reproduced exactly.
 
-constexpr uint32_t kChunk = kWave * 16;   // samples per wavefront work item
 
 // s_waitcnt immediate for vmcnt(0), expcnt(7), lgkmcnt(15) (gfx9 encoding:
 // vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14])
@@ -493,11 +492,13 @@ __device__ __forceinline__ bool short_form_ok(uint32_t n, float min_y, float w, 
 // comes in view_*; shadow maps come in view_depth.
 // Bit k of short_bits: staged triangle k maps by the short form
 // (short_form_ok); steps whose lanes all sample such triangles take it.
+// ent (depth pass of a single view): the chunk's covered-sample entries,
+// nent of them so far (see CoverEntries).
 template <RasterMode kMode, int kOp, bool kBatch>
 __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t view_width, uint32_t* view_depth,
                                               unsigned long long* view_ids, uint32_t shadow_dim, uint32_t my_begin,
                                               uint32_t s, uint32_t stage_end, uint32_t c0,
-                                              unsigned long long short_bits) {
+                                              unsigned long long short_bits, uint4* ent, uint32_t& nent) {
   const int lane = threadIdx.x & (kWave - 1);
   for (uint32_t base = s; base < stage_end; base += kWave) {
     // Lane p takes sample j = base + p.  Its staged triangle k is one less
@@ -514,50 +515,71 @@ __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t
     const bool starts_here = L.start[lane] == base;
     const unsigned long long starts = __ballot(starts_here);
     const uint32_t j = base + (uint32_t)lane;
-    if (j >= stage_end) continue;
-    const int k = __popcll(before) - 1 +
-                  (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
-                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
-                  (starts_here ? 1 : 0);
-    float x, y;
-    const float slf = (float)(j - L.begin[k]);
-    if (__all((short_bits >> k) & 1ull)) {
-      const float min_y = L.min_y[k], w = L.w[k];
-      y = row_of(slf, L.iw[k], min_y);
-      const float xo = slf - (y - min_y) * w;
-      x = L.min_x[k] + xo;
-      if (xo >= w || !inside_tri(x, y, L.ek[0][k], L.ek[1][k], L.ek[2][k], L.ek[3][k], L.ek[4][k], L.ek[5][k],
-                                 L.ek[6][k]))
-        continue;
-    } else if (!sample_xy<kOp>(slf, L.min_x[k], L.min_y[k], L.w[k], L.iw[k], L.ek[0][k], L.ek[1][k], L.ek[2][k],
-                               L.ek[3][k], L.ek[4][k], L.ek[5][k], L.ek[6][k], x, y)) {
-      continue;
+    bool hit = false, rec = false;
+    int k = 0;
+    float x = 0.f, y = 0.f;
+    if (j < stage_end) {
+      k = __popcll(before) - 1 +
+          (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
+          (starts_here ? 1 : 0);
+      const float slf = (float)(j - L.begin[k]);
+      if (__all((short_bits >> k) & 1ull)) {
+        const float min_y = L.min_y[k], w = L.w[k];
+        y = row_of(slf, L.iw[k], min_y);
+        const float xo = slf - (y - min_y) * w;
+        x = L.min_x[k] + xo;
+        hit = !(xo >= w) &&
+              inside_tri(x, y, L.ek[0][k], L.ek[1][k], L.ek[2][k], L.ek[3][k], L.ek[4][k], L.ek[5][k], L.ek[6][k]);
+      } else {
+        hit = sample_xy<kOp>(slf, L.min_x[k], L.min_y[k], L.w[k], L.iw[k], L.ek[0][k], L.ek[1][k], L.ek[2][k],
+                             L.ek[3][k], L.ek[4][k], L.ek[5][k], L.ek[6][k], x, y);
+      }
     }
-    const uint32_t d = depth_quant(fmaf(L.pa[k], x, fmaf(L.pb[k], y, L.pc[k])));
-    uint32_t pitch;
-    uint32_t* dst;
-    if constexpr (kMode == RasterMode::Shadow) {
-      // view_depth: the maps, 6 faces of shadow_dim^2 per light
-      pitch = shadow_dim;
-      dst = view_depth + (size_t)L.key[k] * shadow_dim * shadow_dim;
-    } else if constexpr (kBatch) {
-      pitch = L.pitch[k];
-      dst = L.dst[k];
-    } else {
-      pitch = view_width;
-      dst = view_depth;
+    uint32_t d = 0, pix32 = 0;
+    if (hit) {
+      d = depth_quant(fmaf(L.pa[k], x, fmaf(L.pb[k], y, L.pc[k])));
+      uint32_t pitch;
+      uint32_t* dst;
+      if constexpr (kMode == RasterMode::Shadow) {
+        // view_depth: the maps, 6 faces of shadow_dim^2 per light
+        pitch = shadow_dim;
+        dst = view_depth + (size_t)L.key[k] * shadow_dim * shadow_dim;
+      } else if constexpr (kBatch) {
+        pitch = L.pitch[k];
+        dst = L.dst[k];
+      } else {
+        pitch = view_width;
+        dst = view_depth;
+      }
+      pix32 = (uint32_t)(int)y * pitch + (uint32_t)(int)x;
+      const size_t pix = (size_t)pix32;
+      if constexpr (kMode == RasterMode::Id) {
+        unsigned long long* ids = kBatch ? L.ids[kBatch ? k : 0] : view_ids;
+        const uint32_t val = dst[pix];
+        if (d > val - kBufError && d < val + kBufError)
+          atomicMax(&ids[pix], ((unsigned long long)(L.key[k] + 1u) << 32) | (c0 + (uint32_t)k));
+      } else {
+        // The L2 performs atomics at ~26.6 G cache-line operations/s; a
+        // plain load first drops the samples that cannot win (the stored
+        // depth only decreases, so d >= any value a load returns loses).
+        const uint32_t val = dst[pix];
+        if (d < val) atomicMin(&dst[pix], d);
+        // a sample at or beyond val + BUF_ERROR stays outside the id
+        // pass's window of the final depth (<= val) unless val + BUF_ERROR
+        // wraps
+        rec = val > 0xFFFFFFFFu - kBufError || d < val + kBufError;
+      }
     }
-    const size_t pix = (size_t)((uint32_t)(int)y * pitch + (uint32_t)(int)x);
-    if constexpr (kMode == RasterMode::Id) {
-      unsigned long long* ids = kBatch ? L.ids[kBatch ? k : 0] : view_ids;
-      const uint32_t val = dst[pix];
-      if (d > val - kBufError && d < val + kBufError)
-        atomicMax(&ids[pix], ((unsigned long long)(L.key[k] + 1u) << 32) | (c0 + (uint32_t)k));
-    } else {
-      // The L2 performs atomics at ~26.6 G cache-line operations/s; a
-      // plain load first drops the samples that cannot win (the stored
-      // depth only decreases, so d >= any value a load returns loses).
-      if (d < dst[pix]) atomicMin(&dst[pix], d);
+    if constexpr (kMode == RasterMode::Depth && !kBatch) {
+      if (ent) {
+        const unsigned long long m = __ballot(rec);
+        if (rec) {
+          const uint32_t slot =
+              nent + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+          if (slot < kEntCap) ent[slot] = uint4{pix32, d, c0 + (uint32_t)k, L.key[k] + 1u};
+        }
+        nent += (uint32_t)__popcll(m);
+      }
     }
   }
 }
@@ -585,6 +607,47 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
   L.start[lane] = 0xFFFFFFFFu;   // never a sample index (total < 2^31)
   for (uint32_t s0 = wave0 * kChunk; s0 < total; s0 += nwaves * kChunk) {
     const uint32_t s1 = min(s0 + kChunk, total);
+    // covered-sample entries of the chunk: written by the depth pass of a
+    // single view, read by its id pass instead of a second sweep
+    const uint32_t chunk = s0 / kChunk;
+    uint4* ent = nullptr;
+    uint32_t nent = 0;
+    if constexpr (!kBatch && kMode != RasterMode::Shadow) {
+      if (Ls.ent.cnt && chunk < Ls.ent.chunks) {
+        if constexpr (kMode == RasterMode::Id) {
+          const uint32_t n = Ls.ent.cnt[chunk];
+          if (n <= kEntCap) {
+            // four of the lane's entries at a time, then their depths, then
+            // the window tests: two memory latencies per 256 entries
+            constexpr int kPer = 4;
+            const uint4* E = Ls.ent.ent + (size_t)chunk * kEntCap;
+            for (uint32_t e0 = 0; e0 < n; e0 += kPer * kWave) {
+              uint4 v[kPer];
+              uint32_t val[kPer];
+#pragma unroll
+              for (int i = 0; i < kPer; i++) {
+                const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
+                if (e < n) v[i] = E[e];
+              }
+#pragma unroll
+              for (int i = 0; i < kPer; i++) {
+                const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
+                if (e < n) val[i] = a.view0.depth[v[i].x];
+              }
+#pragma unroll
+              for (int i = 0; i < kPer; i++) {
+                const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
+                if (e < n && v[i].y > val[i] - kBufError && v[i].y < val[i] + kBufError)
+                  atomicMax(&a.view0.idkey[v[i].x], ((unsigned long long)v[i].w << 32) | v[i].z);
+              }
+            }
+            continue;   // else: more covered samples than the entries hold, swept below
+          }
+        } else {
+          ent = Ls.ent.ent + (size_t)chunk * kEntCap;
+        }
+      }
+    }
     // ---- the chunk's first triangle: last record with sample_begin <= s0
     // (record 0 begins at sample 0; 64-ary search, one probe per lane)
     uint32_t lo = 0, hi = ncut;
@@ -645,11 +708,12 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
           __ballot(ci < ncut && short_form_ok<kOp>(L.begin[lane + 1] - my_begin, t_min_y, t_w, t_iw));
       // ---- every sample of [s, stage_end) with all lanes
       sweep_samples<kMode, kOp, kBatch>(L, a.view0.v.width, kMode == RasterMode::Shadow ? sh.map : a.view0.depth,
-                                        a.view0.idkey, sh.L, my_begin, s, stage_end, c0, short_bits);
+                                        a.view0.idkey, sh.L, my_begin, s, stage_end, c0, short_bits, ent, nent);
       __builtin_amdgcn_wave_barrier();
       s = stage_end;
       c0 += kWave;
     }
+    if (ent && lane == 0) Ls.ent.cnt[chunk] = nent;
   }
 }
 
@@ -768,8 +832,9 @@ __device__ __forceinline__ void local_sweep(ShadowStage& W, const RasterTri& rt,
   const unsigned long long short_bits =
       __ballot((uint32_t)lane < nloc &&
                short_form_ok<kOpSizeLight>(my_end - my_begin, W.min_y[lane], W.w[lane], W.iw[lane]));
+  uint32_t nent_unused = 0;
   sweep_samples<RasterMode::Shadow, kOpSizeLight, false>(W, 0u, sh.map, nullptr, sh.L, my_begin, 0u, total, 0u,
-                                                         short_bits);
+                                                         short_bits, nullptr, nent_unused);
   // the stage is rewritten by the wavefront's next sweep
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

@@ -149,6 +149,7 @@ static PassArgs single_pass(const Scene* s, const ocr_view* view, Frame* f) {
   a.tiles = view_tiles(view->width, view->height, a.tile_w_log2);
   const Frame::OutBuf& ob = f->out[f->cur];
   a.lists = make_lists(ob.cuts, ob.counters, ob.cap_cut);
+  a.lists.ent = ob.ent;
   if (f->raster_mode == OCR_RASTER_TILES && ob.tb.bin) {
     a.lists.tb = ob.tb;
     a.view0.bin_begin = 0;
@@ -216,15 +217,25 @@ static int ensure_frame_bins(Frame* f, int b) {
   return ensure_bins(o.tb, o.box_cap, o.cap_cut, ntiles, f->tw_log2, f->th_log2, cap, f->want_bin != 0);
 }
 
+// Covered-sample entries of a frame's output set: room for the chunks of
+// 10 raster samples per pixel (the 1080p, 1 M-triangle frame has 8.2; a
+// render with more sweeps its later chunks in the id pass), 80 bytes per
+// pixel.
+constexpr uint32_t kEntSamplesPerPixel = 10;
+
 int frame_alloc_set(Frame* f, int b) {
   Frame::OutBuf& o = f->out[b];
   if (o.depth) return OCR_OK;
   const size_t npix = (size_t)f->width * f->height;
+  const uint32_t chunks = (uint32_t)((npix * kEntSamplesPerPixel + kChunk - 1) / kChunk);
   int rc;
   if ((rc = dev_alloc(&o.depth, npix, "depth")) || (rc = dev_alloc(&o.idkey, npix, "id buffer")) ||
       (rc = dev_alloc(&o.rgba, npix, "rgba")) || (rc = dev_alloc(&o.normals, npix, "normals")) ||
-      (rc = dev_alloc(&o.counters, kCounterWords, "counters")))
+      (rc = dev_alloc(&o.counters, kCounterWords, "counters")) ||
+      (rc = dev_alloc(&o.ent.ent, (size_t)chunks * kEntCap, "covered-sample entries")) ||
+      (rc = dev_alloc(&o.ent.cnt, chunks, "covered-sample counts")))
     return rc;
+  o.ent.chunks = chunks;
   OCR_HIP_CHECK(hipMemset(o.counters, 0, 4 * kCounterWords));
   return OCR_OK;
 }
@@ -862,6 +873,8 @@ void ocr_frame_destroy(ocr_frame* f) {
     dev_free(o.normals);
     dev_free(o.cuts);
     dev_free(o.counters);
+    dev_free(o.ent.ent);
+    dev_free(o.ent.cnt);
     free_bins(o.tb, o.box_cap);
   }
   dev_free(f->post_in);
