@@ -44,6 +44,10 @@ struct ocr_batch {
   // tiled raster path (tune.raster_tile != 0): bins over every view's tiles
   TileBins tb{};
   uint32_t box_cap = 0, bin_tiles = 0, want_bin = 0;
+  // covered-sample entries of the depth pass (read by the id pass); fixed
+  // at 256 MiB for a batch reserved for serving
+  CoverEntries ent{};
+  bool ent_fixed = false;
   ocr_batch_tuning tune{8, 4, 1, 0};
   // tune.groups > 1: sub-batches over pixel-balanced groups of the views,
   // each rendered on a stream of its own between a fork and a join event
@@ -87,6 +91,7 @@ static PassArgs batch_pass(const ocr_batch* b) {
   for (const Scene* s : b->scenes)
     if (s->n_shadow) a.unshadowed = 0u;
   a.lists = make_lists(b->cuts, b->counters, b->cap_cut);
+  a.lists.ent = b->ent;
   if (b->tune.raster_tile) a.lists.tb = b->tb;
   return a;
 }
@@ -133,9 +138,40 @@ static int ensure_bins_of(ocr_batch* b) {
   return rc;
 }
 
+// Covered-sample entries for 10 raster samples per pixel of the views (80
+// bytes per pixel); a batch reserved for serving (batch_reserve) keeps a
+// fixed 256 MiB and sweeps the chunks beyond.  They only grow, so batches
+// of changing views keep theirs.
+constexpr uint32_t kBatchEntMaxChunks = (uint32_t)((256ull << 20) / (kEntCap * sizeof(uint4)));
+
+static int ensure_entries(ocr_batch* b) {
+  const uint64_t need = (b->sum_pix * 10 + kChunk - 1) / kChunk;
+  const uint32_t want = b->ent_fixed ? kBatchEntMaxChunks
+                                     : (uint32_t)std::min<uint64_t>(std::max<uint64_t>(need, 1), 0x7FFFFFFFull / kChunk);
+  if (b->ent.ent && b->ent.chunks >= want) return OCR_OK;
+  OCR_HIP_CHECK(hipDeviceSynchronize());
+  drop_graph(b);
+  if (b->ent.ent) (void)hipFree(b->ent.ent);
+  if (b->ent.cnt) (void)hipFree(b->ent.cnt);
+  b->ent = CoverEntries{};
+  hipError_t e = hipMalloc((void**)&b->ent.ent, sizeof(uint4) * (size_t)kEntCap * want);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->ent.cnt, sizeof(uint32_t) * (size_t)want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    if (b->ent.ent) (void)hipFree(b->ent.ent);
+    b->ent = CoverEntries{};
+    return fail(OCR_ERR_OOM, std::string("batch covered-sample entries: ") + hipGetErrorString(e));
+  }
+  b->ent.chunks = want;
+  return OCR_OK;
+}
+
 static int ensure_lists(ocr_batch* b) {
   uint64_t want_cut = std::min<uint64_t>(b->sum_tris * 2 + 64, 0xFFFFFFF0ull);
-  if (b->cuts && b->cap_cut >= want_cut) return ensure_bins_of(b);
+  if (b->cuts && b->cap_cut >= want_cut) {
+    const int rc = ensure_entries(b);
+    return rc ? rc : ensure_bins_of(b);
+  }
   OCR_HIP_CHECK(hipDeviceSynchronize());
   drop_graph(b);
   if (b->cuts) (void)hipFree(b->cuts);
@@ -147,13 +183,18 @@ static int ensure_lists(ocr_batch* b) {
     return fail(OCR_ERR_OOM, std::string("batch cut list: ") + hipGetErrorString(e));
   }
   b->cap_cut = (uint32_t)want_cut;
-  return ensure_bins_of(b);
+  const int rc = ensure_entries(b);
+  return rc ? rc : ensure_bins_of(b);
 }
 
+// Sizes the lists ahead of use (a server's batch slots): the cut list for
+// sum_tris triangles and the covered-sample entries at their largest, so
+// batches of any views later allocate nothing.
 int ocr::batch_reserve(ocr_batch* b, uint64_t sum_tris) {
   OCR_HIP_CHECK(hipSetDevice(b->ctx->device));
   const uint64_t keep = b->sum_tris;
   b->sum_tris = sum_tris;
+  b->ent_fixed = true;
   const int rc = ensure_lists(b);
   b->sum_tris = keep;
   return rc;
@@ -265,6 +306,9 @@ static int regroup(ocr_batch* b) {
     OCR_HIP_CHECK(hipFree(b->cuts));
     b->cuts = nullptr;
     b->cap_cut = 0;
+    if (b->ent.ent) OCR_HIP_CHECK(hipFree(b->ent.ent));
+    if (b->ent.cnt) OCR_HIP_CHECK(hipFree(b->ent.cnt));
+    b->ent = CoverEntries{};
   }
   while (b->subs.size() < g) {
     // a group's sub-batch, stream and join event exist together or not at all
@@ -366,6 +410,8 @@ void ocr_batch_destroy(ocr_batch* b) {
   if (b->d_views) (void)hipFree(b->d_views);
   if (b->d_scenes) (void)hipFree(b->d_scenes);
   if (b->cuts) (void)hipFree(b->cuts);
+  if (b->ent.ent) (void)hipFree(b->ent.ent);
+  if (b->ent.cnt) (void)hipFree(b->ent.cnt);
   free_bins(b->tb, b->box_cap);
   if (b->counters) (void)hipFree(b->counters);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);

@@ -570,7 +570,7 @@ __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t
         rec = val > 0xFFFFFFFFu - kBufError || d < val + kBufError;
       }
     }
-    if constexpr (kMode == RasterMode::Depth && !kBatch) {
+    if constexpr (kMode == RasterMode::Depth) {
       if (ent) {
         const unsigned long long m = __ballot(rec);
         if (rec) {
@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
     const uint32_t chunk = s0 / kChunk;
     uint4* ent = nullptr;
     uint32_t nent = 0;
-    if constexpr (!kBatch && kMode != RasterMode::Shadow) {
+    if constexpr (kMode != RasterMode::Shadow) {
       if (Ls.ent.cnt && chunk < Ls.ent.chunks) {
         if constexpr (kMode == RasterMode::Id) {
           const uint32_t n = Ls.ent.cnt[chunk];
@@ -629,16 +629,32 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
                 const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
                 if (e < n) v[i] = E[e];
               }
+              // a batch's entries hold pixels of their record's view
+              uint32_t* dep[kPer];
+              unsigned long long* ids[kPer];
+#pragma unroll
+              for (int i = 0; i < kPer; i++) {
+                dep[i] = a.view0.depth;
+                ids[i] = a.view0.idkey;
+                if constexpr (kBatch) {
+                  const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
+                  if (e < n) {
+                    const DevView& V = a.views[Ls.cuts[v[i].z].view];
+                    dep[i] = V.depth;
+                    ids[i] = V.idkey;
+                  }
+                }
+              }
 #pragma unroll
               for (int i = 0; i < kPer; i++) {
                 const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
-                if (e < n) val[i] = a.view0.depth[v[i].x];
+                if (e < n) val[i] = dep[i][v[i].x];
               }
 #pragma unroll
               for (int i = 0; i < kPer; i++) {
                 const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
                 if (e < n && v[i].y > val[i] - kBufError && v[i].y < val[i] + kBufError)
-                  atomicMax(&a.view0.idkey[v[i].x], ((unsigned long long)v[i].w << 32) | v[i].z);
+                  atomicMax(&ids[i][v[i].x], ((unsigned long long)v[i].w << 32) | v[i].z);
               }
             }
             continue;   // else: more covered samples than the entries hold, swept below
