@@ -387,13 +387,29 @@ __device__ __forceinline__ void uniform(ShadeArgs& a) {
   a.normals = rfl(a.normals);
 }
 
-__device__ __forceinline__ const DevView& shade_view(const PassArgs& p, uint32_t& vi) {
+// The shading tile of this block.  Workgroups go to the eight XCDs in
+// turn (block b to XCD b mod 8), each XCD with an L2 of its own.  Groups of
+// kXcdRun consecutive tiles (a strip of a tile row) go to one XCD, the
+// groups to the XCDs in turn: neighbouring tiles of a strip share the depth
+// and id lines that SSAO's 5x5 taps and the gathers read in one L2, while
+// the load of busy screen regions stays spread over all eight.  A bijection
+// of [0, n) whatever the dispatch order.
+constexpr uint32_t kXcdRun = 8;
+__device__ __forceinline__ uint32_t shade_tile_index() {
+  const uint32_t b = blockIdx.x, n = gridDim.x;
+  const uint32_t full = n / (8u * kXcdRun) * (8u * kXcdRun);   // blocks in whole rounds
+  if (b >= full) return b;
+  const uint32_t k = b & 7u, j = b >> 3;
+  return ((j / kXcdRun) * 8u + k) * kXcdRun + j % kXcdRun;
+}
+
+__device__ __forceinline__ const DevView& shade_view(const PassArgs& p, uint32_t tile, uint32_t& vi) {
   vi = 0;
   if (p.views) {
     uint32_t lo = 0, hi = p.nviews;
     while (hi - lo > 1) {
       uint32_t mid = (lo + hi) >> 1;
-      if (p.views[mid].tile_begin <= blockIdx.x) lo = mid; else hi = mid;
+      if (p.views[mid].tile_begin <= tile) lo = mid; else hi = mid;
     }
     vi = (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
     return p.views[vi];
@@ -406,7 +422,8 @@ __device__ __forceinline__ const DevView& shade_view(const PassArgs& p, uint32_t
 template <bool kCount, bool kShadows>
 __device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
   uint32_t vi;
-  const DevView& V = shade_view(sl.p, vi);
+  const uint32_t block_tile = shade_tile_index();
+  const DevView& V = shade_view(sl.p, block_tile, vi);
   const DevScene& S = sl.p.scenes ? sl.p.scenes[V.scene] : sl.p.scene0;
   ShadeArgs a;
   a.v = V.v;
@@ -437,7 +454,7 @@ __device__ __forceinline__ void shade_pixel(const ShadeLaunch& sl) {
   const uint32_t W = a.v.width, H = a.v.height;
   const uint32_t lw = sl.p.tile_w_log2, tw = 1u << lw, th = 256u >> lw;
   const uint32_t tiles_x = (W + tw - 1) / tw;
-  const uint32_t tile = blockIdx.x - V.tile_begin;
+  const uint32_t tile = block_tile - V.tile_begin;
   const int x = (int)((tile % tiles_x) * tw + (threadIdx.x & (tw - 1)));
   const int y = (int)((tile / tiles_x) * th + (threadIdx.x >> lw));
   if (x >= (int)W || y >= (int)H) return;
