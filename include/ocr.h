@@ -254,8 +254,10 @@ typedef struct ocr_raster_config {
 } ocr_raster_config;
 int  ocr_frame_set_raster(ocr_frame* f, const ocr_raster_config* cfg);
 int  ocr_frame_get_raster(ocr_frame* f, ocr_raster_config* cfg);
-/* Record per-pass hipEvent timings on every render of this frame (up to
- * 1024 renders are averaged; enabling again restarts the average). */
+/* Record hipEvent timings on every render of this frame (up to 1024
+ * renders are averaged; enabling again restarts the average): enable 1
+ * times every pass, 2 only the shading pass (two events per render, for
+ * timing a stream of renders with as little added as possible), 0 stops. */
 int  ocr_frame_set_profiling(ocr_frame* f, int enable);
 
 /* engine::draw_bulk_objs_n -> render_tris (engine.cpp:1794-2161):

@@ -406,7 +406,7 @@ struct Frame {
   // entries (0 = automatic; raised after an overflow)
   uint32_t raster_mode = OCR_RASTER_SAMPLES, tw_log2 = 4, th_log2 = 4, want_bin = 0;
   uint32_t* post_in = nullptr;       // colour snapshot of the screen-space post passes
-  bool profiling = false;
+  int profiling = 0;   // 0 off, 1 every pass, 2 the shading pass only
   static constexpr int kEvents = 6;       // prearrange, clear, depth, id, shade boundaries
   static constexpr int kRing = 1024;      // renders averaged per profiling window
   std::vector<hipEvent_t> ev;             // kRing x kEvents, created on first use

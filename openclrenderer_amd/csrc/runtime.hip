@@ -890,7 +890,8 @@ void ocr_frame_destroy(ocr_frame* f) {
 
 int ocr_frame_set_profiling(ocr_frame* f, int enable) {
   if (!f) return fail(OCR_ERR_INVALID, "ocr_frame_set_profiling: null frame");
-  f->profiling = enable != 0;
+  if (enable < 0 || enable > 2) return fail(OCR_ERR_INVALID, "ocr_frame_set_profiling: 0, 1 or 2");
+  f->profiling = enable;
   f->n_profiled = 0;
   if (f->profiling && f->ev.empty()) {
     OCR_HIP_CHECK(hipSetDevice(f->ctx->device));
@@ -979,8 +980,11 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
     rc = ocr_lights_upload(s, nullptr, 0, s->lbuf_dim);
     if (rc) return rc;
   }
-  const bool prof = f->profiling && f->n_profiled < Frame::kRing;
-  hipEvent_t* ev = prof ? &f->ev[(size_t)f->n_profiled * Frame::kEvents] : nullptr;
+  // profiling: events around every pass (mode 1) or the shading pass only
+  // (mode 2: two events per render)
+  const bool prof_shade = f->profiling && f->n_profiled < Frame::kRing;
+  const bool prof = prof_shade && f->profiling == 1;
+  hipEvent_t* ev = prof_shade ? &f->ev[(size_t)f->n_profiled * Frame::kEvents] : nullptr;
   hipStream_t sb = f->ss[b];
   // Render N runs on the stream of its set once render N-kSets (same
   // buffers, same stream) and everything enqueued on the caller's stream
@@ -1012,7 +1016,7 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
     if (prof) OCR_HIP_CHECK(hipEventRecord(ev[2], sb));
     OCR_HIP_CHECK(launch_tile_raster(pa, sb));
     if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], sb));
-    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
+    if (prof_shade) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
   } else {
     OCR_HIP_CHECK(launch_zero_counters(o.counters, sb));
     if (prof) OCR_HIP_CHECK(hipEventRecord(ev[0], sb));
@@ -1023,17 +1027,17 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
     OCR_HIP_CHECK(launch_depth(pa, kRasterBlocksPerCu, sb));
     if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], sb));
     OCR_HIP_CHECK(launch_id(pa, kRasterBlocksPerCu, sb));
-    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
+    if (prof_shade) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
   }
   // Shading reads the shadow maps: it waits for the scene's last shadow
   // pass, wherever that was enqueued (the raster passes do not read them).
   OCR_HIP_CHECK(s->wait_shadow_maps(sb));
   OCR_HIP_CHECK(launch_shade(pa, s->ctx->face_sc, sb));
   OCR_HIP_CHECK(s->note_read(sb, true));
-  if (prof) OCR_HIP_CHECK(hipEventRecord(ev[5], sb));
+  if (prof_shade) OCR_HIP_CHECK(hipEventRecord(ev[5], sb));
   OCR_HIP_CHECK(hipEventRecord(f->done[b], sb));
   OCR_HIP_CHECK(hipStreamWaitEvent(st, f->done[b], 0));
-  if (prof) f->n_profiled++;
+  if (prof_shade) f->n_profiled++;
   f->rendered = true;
   return OCR_OK;
 }
@@ -1113,6 +1117,7 @@ int ocr_frame_get_stats(ocr_frame* f, void* stream, ocr_frame_stats* out) {
       hipEvent_t* ev = &f->ev[(size_t)r * Frame::kEvents];
       float ms = 0.f;
       for (int i = 0; i < 6; i++) {
+        if (f->profiling == 2 && i != 4) continue;   // the shading pass alone
         OCR_HIP_CHECK(hipEventElapsedTime(&ms, ev[from[i]], ev[to[i]]));
         sum[i] += ms;
       }

@@ -303,8 +303,10 @@ class Frame:
         OCR_ERR_OVERFLOW and grows it."""
         check(self.lib.ocr_frame_reserve(self.h, max_cut_triangles))
 
-    def set_profiling(self, on=True):
-        check(self.lib.ocr_frame_set_profiling(self.h, 1 if on else 0))
+    def set_profiling(self, on=True, shade_only=False):
+        """Per-pass hipEvent timings of every render (shade_only: of the
+        shading pass alone, two events per render)."""
+        check(self.lib.ocr_frame_set_profiling(self.h, (2 if shade_only else 1) if on else 0))
 
     def set_raster(self, mode, tile_w_log2=4, tile_h_log2=4, bin_capacity=0):
         """RASTER_SAMPLES (the sample-space depth and id passes) or
