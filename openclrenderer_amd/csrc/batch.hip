@@ -142,7 +142,7 @@ static int ensure_bins_of(ocr_batch* b) {
 // bytes per pixel); a batch reserved for serving (batch_reserve) keeps a
 // fixed 256 MiB and sweeps the chunks beyond.  They only grow, so batches
 // of changing views keep theirs.
-constexpr uint32_t kBatchEntMaxChunks = (uint32_t)((256ull << 20) / (kEntCap * sizeof(uint4)));
+constexpr uint32_t kBatchEntMaxChunks = (uint32_t)((256ull << 20) / (kEntCap * sizeof(uint2)));
 
 static int ensure_entries(ocr_batch* b) {
   const uint64_t need = (b->sum_pix * 10 + kChunk - 1) / kChunk;
@@ -154,7 +154,7 @@ static int ensure_entries(ocr_batch* b) {
   if (b->ent.ent) (void)hipFree(b->ent.ent);
   if (b->ent.cnt) (void)hipFree(b->ent.cnt);
   b->ent = CoverEntries{};
-  hipError_t e = hipMalloc((void**)&b->ent.ent, sizeof(uint4) * (size_t)kEntCap * want);
+  hipError_t e = hipMalloc((void**)&b->ent.ent, sizeof(uint2) * (size_t)kEntCap * want);
   if (e == hipSuccess) e = hipMalloc((void**)&b->ent.cnt, sizeof(uint32_t) * (size_t)want);
   if (e != hipSuccess) {
     (void)hipGetLastError();

@@ -157,16 +157,16 @@ struct TileBins {
 // tiles_x << 16 | walk_is_rect << 31.
 constexpr uint32_t kBoxRect = 1u << 31;
 
-// Covered-sample entries of a single view's render: the depth pass writes,
-// per chunk of kChunk raster samples, each covered sample that can still
-// fall in the id pass's window ({pixel, depth, record, 2*tri + piece + 1}),
-// up to kEntCap, and the chunk's count; the id pass reads the entries
-// instead of sweeping the chunk's samples again (chunks with more, or past
-// `chunks`, are swept).
+// Covered-sample entries of a render: the depth pass writes, per chunk of
+// kChunk raster samples, each covered sample that can still fall in the id
+// pass's window ({x | y << 16, record}), up to kEntCap, and the chunk's
+// count; the id pass reads the entries, forms each one's depth again from
+// its record's plane, instead of sweeping the chunk's samples again (chunks
+// with more, or past `chunks`, are swept).
 constexpr uint32_t kChunk = 64 * 16;   // raster samples per wavefront work item
 constexpr uint32_t kEntCap = 512;
 struct CoverEntries {
-  uint4* ent;         // chunks x kEntCap
+  uint2* ent;         // chunks x kEntCap
   uint32_t* cnt;      // per chunk
   uint32_t chunks;
 };

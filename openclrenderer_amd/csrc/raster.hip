@@ -498,7 +498,7 @@ template <RasterMode kMode, int kOp, bool kBatch>
 __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t view_width, uint32_t* view_depth,
                                               unsigned long long* view_ids, uint32_t shadow_dim, uint32_t my_begin,
                                               uint32_t s, uint32_t stage_end, uint32_t c0,
-                                              unsigned long long short_bits, uint4* ent, uint32_t& nent) {
+                                              unsigned long long short_bits, uint2* ent, uint32_t& nent) {
   const int lane = threadIdx.x & (kWave - 1);
   for (uint32_t base = s; base < stage_end; base += kWave) {
     // Lane p takes sample j = base + p.  Its staged triangle k is one less
@@ -576,7 +576,7 @@ __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t
         if (rec) {
           const uint32_t slot =
               nent + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-          if (slot < kEntCap) ent[slot] = uint4{pix32, d, c0 + (uint32_t)k, L.key[k] + 1u};
+          if (slot < kEntCap) ent[slot] = uint2{(uint32_t)(int)x | (uint32_t)(int)y << 16, c0 + (uint32_t)k};
         }
         nent += (uint32_t)__popcll(m);
       }
@@ -610,51 +610,61 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
     // covered-sample entries of the chunk: written by the depth pass of a
     // single view, read by its id pass instead of a second sweep
     const uint32_t chunk = s0 / kChunk;
-    uint4* ent = nullptr;
+    uint2* ent = nullptr;
     uint32_t nent = 0;
     if constexpr (kMode != RasterMode::Shadow) {
       if (Ls.ent.cnt && chunk < Ls.ent.chunks) {
         if constexpr (kMode == RasterMode::Id) {
           const uint32_t n = Ls.ent.cnt[chunk];
           if (n <= kEntCap) {
-            // four of the lane's entries at a time, then their depths, then
-            // the window tests: two memory latencies per 256 entries
+            // four of the lane's entries at a time, then their records' depth
+            // planes and keys, then the depths there, then the window tests;
+            // each entry's depth is formed again as the depth pass formed it
             constexpr int kPer = 4;
-            const uint4* E = Ls.ent.ent + (size_t)chunk * kEntCap;
+            const uint2* E = Ls.ent.ent + (size_t)chunk * kEntCap;
+            const uint4* R = reinterpret_cast<const uint4*>(Ls.rast);   // 4 x uint4 per record
             for (uint32_t e0 = 0; e0 < n; e0 += kPer * kWave) {
-              uint4 v[kPer];
-              uint32_t val[kPer];
+              uint2 v[kPer];
+              uint32_t d[kPer], key[kPer], pix[kPer], val[kPer];
+              uint32_t* dep[kPer];
+              unsigned long long* ids[kPer];
 #pragma unroll
               for (int i = 0; i < kPer; i++) {
                 const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
                 if (e < n) v[i] = E[e];
               }
-              // a batch's entries hold pixels of their record's view
-              uint32_t* dep[kPer];
-              unsigned long long* ids[kPer];
 #pragma unroll
               for (int i = 0; i < kPer; i++) {
+                const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
                 dep[i] = a.view0.depth;
                 ids[i] = a.view0.idkey;
-                if constexpr (kBatch) {
-                  const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
-                  if (e < n) {
-                    const DevView& V = a.views[Ls.cuts[v[i].z].view];
+                uint32_t pitch = a.view0.v.width;
+                if (e < n) {
+                  // RasterTri words 1 (key) and 5-7 (the 1/z plane A, B, C)
+                  const uint4 w0 = R[4 * (size_t)v[i].y], w1 = R[4 * (size_t)v[i].y + 1];
+                  if constexpr (kBatch) {
+                    // a batch's entries hold pixels of their record's view
+                    const DevView& V = a.views[Ls.cuts[v[i].y].view];
                     dep[i] = V.depth;
                     ids[i] = V.idkey;
+                    pitch = V.v.width;
                   }
+                  const float x = (float)(v[i].x & 0xFFFFu), y = (float)(v[i].x >> 16);
+                  d[i] = depth_quant(fmaf(__uint_as_float(w1.y), x, fmaf(__uint_as_float(w1.z), y, __uint_as_float(w1.w))));
+                  key[i] = w0.y;
+                  pix[i] = (v[i].x >> 16) * pitch + (v[i].x & 0xFFFFu);
                 }
               }
 #pragma unroll
               for (int i = 0; i < kPer; i++) {
                 const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
-                if (e < n) val[i] = dep[i][v[i].x];
+                if (e < n) val[i] = dep[i][pix[i]];
               }
 #pragma unroll
               for (int i = 0; i < kPer; i++) {
                 const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
-                if (e < n && v[i].y > val[i] - kBufError && v[i].y < val[i] + kBufError)
-                  atomicMax(&ids[i][v[i].x], ((unsigned long long)v[i].w << 32) | v[i].z);
+                if (e < n && d[i] > val[i] - kBufError && d[i] < val[i] + kBufError)
+                  atomicMax(&ids[i][pix[i]], ((unsigned long long)(key[i] + 1u) << 32) | v[i].y);
               }
             }
             continue;   // else: more covered samples than the entries hold, swept below

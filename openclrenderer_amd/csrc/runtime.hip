@@ -219,7 +219,7 @@ static int ensure_frame_bins(Frame* f, int b) {
 
 // Covered-sample entries of a frame's output set: room for the chunks of
 // 10 raster samples per pixel (the 1080p, 1 M-triangle frame has 8.2; a
-// render with more sweeps its later chunks in the id pass), 80 bytes per
+// render with more sweeps its later chunks in the id pass), 40 bytes per
 // pixel.
 constexpr uint32_t kEntSamplesPerPixel = 10;
 
