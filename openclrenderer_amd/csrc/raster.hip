@@ -535,7 +535,7 @@ __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t
                              L.ek[3][k], L.ek[4][k], L.ek[5][k], L.ek[6][k], x, y);
       }
     }
-    uint32_t d = 0, pix32 = 0;
+    uint32_t d = 0, pix32 = 0, xy = 0;
     if (hit) {
       d = depth_quant(fmaf(L.pa[k], x, fmaf(L.pb[k], y, L.pc[k])));
       uint32_t pitch;
@@ -551,7 +551,11 @@ __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t
         pitch = view_width;
         dst = view_depth;
       }
-      pix32 = (uint32_t)(int)y * pitch + (uint32_t)(int)x;
+      const uint32_t xi = (uint32_t)(int)x, yi = (uint32_t)(int)y;
+      xy = xi | yi << 16;
+      // y < 2^15 and the pitch < 2^16 (frames are at most 32768 wide and
+      // tall, shadow faces 4000): the 24-bit multiply is exact
+      pix32 = __umul24(yi, pitch) + xi;
       const size_t pix = (size_t)pix32;
       if constexpr (kMode == RasterMode::Id) {
         unsigned long long* ids = kBatch ? L.ids[kBatch ? k : 0] : view_ids;
@@ -562,8 +566,19 @@ __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t
         // The L2 performs atomics at ~26.6 G cache-line operations/s; a
         // plain load first drops the samples that cannot win (the stored
         // depth only decreases, so d >= any value a load returns loses).
-        const uint32_t val = dst[pix];
-        if (d < val) atomicMin(&dst[pix], d);
+        // A single view's buffer is addressed from its (uniform) base with
+        // a 32-bit offset.
+        uint32_t val;
+        if constexpr (kMode == RasterMode::Depth && !kBatch) {
+          typedef __attribute__((address_space(1))) char gchar;
+          typedef __attribute__((address_space(1))) uint32_t gu32;
+          gu32* g = (gu32*)((gchar*)(uintptr_t)dst + pix32 * 4u);   // 32-bit byte offset: frames < 2^30 px
+          val = *g;
+          if (d < val) __hip_atomic_fetch_min(g, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          val = dst[pix];
+          if (d < val) atomicMin(&dst[pix], d);
+        }
         // a sample at or beyond val + BUF_ERROR stays outside the id
         // pass's window of the final depth (<= val) unless val + BUF_ERROR
         // wraps
@@ -576,7 +591,7 @@ __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t
         if (rec) {
           const uint32_t slot =
               nent + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-          if (slot < kEntCap) ent[slot] = uint2{(uint32_t)(int)x | (uint32_t)(int)y << 16, c0 + (uint32_t)k};
+          if (slot < kEntCap) ent[slot] = uint2{xy, c0 + (uint32_t)k};
         }
         nent += (uint32_t)__popcll(m);
       }
