@@ -83,7 +83,7 @@ static_assert(sizeof(CutTri) == 64, "CutTri layout");
 // bits of (a + b*x + c*y) * sign).
 struct alignas(16) RasterTri {
   float min_x;
-  uint32_t key;
+  uint32_t key;     // | kKeyBoxForm when the samples are the box's pixels
   float min_y;
   float w;          // max_x - min_x
   float iw;         // 1 / w
@@ -92,6 +92,13 @@ struct alignas(16) RasterTri {
   uint32_t sample_begin;
 };
 static_assert(sizeof(RasterTri) == 64, "RasterTri layout");
+// Set in RasterTri::key (keys are below 2^31) when the record's samples are
+// its bounding box's pixels in row-major order, w * rows of them, instead of
+// the walk's: for triangles whose walk visits exactly that box
+// (walk_is_rect).  The depth and id passes keep a minimum and a maximum per
+// pixel, so only the set of pixels a triangle visits matters, not the order
+// or the repeats (the walk visits every fragment's first index twice).
+constexpr uint32_t kKeyBoxForm = 1u << 31;
 
 // Everything a kernel needs to know about one view.
 struct ViewParams {

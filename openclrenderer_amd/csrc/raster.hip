@@ -280,7 +280,7 @@ __device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds<kWaves
     const uint32_t key = key0 + key_step * (uint32_t)i;
     // shadow records are read only through their raster setup
     if constexpr (kCamera) store_cut<kCamera>(&L.cuts[c], tp[i], rc[i], key, oid, view);
-    rt[i].key = key;
+    rt[i].key = key | (rt[i].key & kKeyBoxForm);
     rt[i].sample_begin = smp;
     L.rast[c] = rt[i];
     if constexpr (kCamera) {
@@ -308,7 +308,10 @@ __device__ __forceinline__ void append_cuts(const Lists& L, BlockAllocLds<kWaves
 }
 
 // Cull, count and set up the (up to 2) clipped pieces of one triangle.
-// my[i]: piece i's max_y (its box's end row).
+// my[i]: piece i's max_y (its box's end row).  A piece whose walk visits
+// exactly its box gets the box form (rt.key = kKeyBoxForm; the caller adds
+// the key): one sample per box pixel, mapped without the walk's fragment
+// arithmetic.
 __device__ __forceinline__ void cut_pieces(const f3 tp[2][3], int num, float ew, float eh, bool two_sided, int op,
                                            float rc[2], RasterTri rt[2], uint32_t nf[2], uint32_t ns[2],
                                            float my[2]) {
@@ -318,7 +321,15 @@ __device__ __forceinline__ void cut_pieces(const f3 tp[2][3], int num, float ew,
     nf[i] = cull_count(tp[i], ew, eh, two_sided, op, rc[i]);
     if (!nf[i]) continue;
     raster_setup(tp[i], rc[i], ew, eh, rt[i], my[i]);
-    ns[i] = triangle_samples(rt[i], my[i], op, nf[i]);
+    // the box holds at most 16,000,000 pixels (frames and cube faces), an
+    // exact product, 0 when the walk has no samples (w or rows 0); the
+    // closed forms of walk_is_rect settle all but tall triangles near the
+    // top row, which keep the walk's samples
+    const float area = rt[i].w * (my[i] - rt[i].min_y);
+    const bool box = area >= 1.f && area <= (float)kMaxTriSamples &&
+                     walk_is_rect_closed(rt[i].min_y, my[i], rt[i].w, rt[i].iw);
+    rt[i].key = box ? kKeyBoxForm : 0u;
+    ns[i] = box ? (uint32_t)area : triangle_samples(rt[i], my[i], op, nf[i]);
   }
 }
 
@@ -490,8 +501,9 @@ __device__ __forceinline__ bool short_form_ok(uint32_t n, float min_y, float w, 
 // one sample each, in lane order, and L.start holds no value in
 // [s, stage_end) on entry.  A single view's target (width, depth, id keys)
 // comes in view_*; shadow maps come in view_depth.
-// Bit k of short_bits: staged triangle k maps by the short form
-// (short_form_ok); steps whose lanes all sample such triangles take it.
+// Bit k of short_bits: staged triangle k maps by the short form (box-form
+// records, and walk-form ones that short_form_ok certifies); lanes sampling
+// such triangles take it, the others the walk's general form.
 // ent (depth pass of a single view): the chunk's covered-sample entries,
 // nent of them so far (see CoverEntries).
 template <RasterMode kMode, int kOp, bool kBatch>
@@ -523,7 +535,9 @@ __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t
           (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
           (starts_here ? 1 : 0);
       const float slf = (float)(j - L.begin[k]);
-      if (__all((short_bits >> k) & 1ull)) {
+      // box-form triangles take the short form in every step (their sample
+      // indices are box pixels, not walk indices)
+      if ((short_bits >> k) & 1ull) {
         const float min_y = L.min_y[k], w = L.w[k];
         y = row_of(slf, L.iw[k], min_y);
         const float xo = slf - (y - min_y) * w;
@@ -666,7 +680,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
                   }
                   const float x = (float)(v[i].x & 0xFFFFu), y = (float)(v[i].x >> 16);
                   d[i] = depth_quant(fmaf(__uint_as_float(w1.y), x, fmaf(__uint_as_float(w1.z), y, __uint_as_float(w1.w))));
-                  key[i] = w0.y;
+                  key[i] = w0.y & ~kKeyBoxForm;
                   pix[i] = (v[i].x >> 16) * pitch + (v[i].x & 0xFFFFu);
                 }
               }
@@ -708,12 +722,14 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
       const uint32_t ci = c0 + lane;
       uint32_t my_begin = total;   // this lane's staged triangle's first sample
       float t_min_y = 0.f, t_w = 0.f, t_iw = 0.f;
+      bool t_box = false;
       if (ci < ncut) {
         const RasterTri t = Ls.rast[ci];
         my_begin = t.sample_begin;
         t_min_y = t.min_y;
         t_w = t.w;
         t_iw = t.iw;
+        t_box = (t.key & kKeyBoxForm) != 0u;
         L.min_x[lane] = t.min_x;
         L.min_y[lane] = t.min_y;
         L.w[lane] = t.w;
@@ -723,7 +739,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
         L.pc[lane] = t.C;
 #pragma unroll
         for (int e = 0; e < 7; e++) L.ek[e][lane] = t.ek[e];
-        L.key[lane] = t.key;
+        L.key[lane] = t.key & ~kKeyBoxForm;
         L.begin[lane] = t.sample_begin;
         if constexpr (kBatch) {
           const DevView& V = a.views[Ls.cuts[ci].view];
@@ -746,7 +762,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const uint32_t stage_end = min(s1, L.begin[kWave]);
       const unsigned long long short_bits =
-          __ballot(ci < ncut && short_form_ok<kOp>(L.begin[lane + 1] - my_begin, t_min_y, t_w, t_iw));
+          __ballot(ci < ncut && (t_box || short_form_ok<kOp>(L.begin[lane + 1] - my_begin, t_min_y, t_w, t_iw)));
       // ---- every sample of [s, stage_end) with all lanes
       sweep_samples<kMode, kOp, kBatch>(L, a.view0.v.width, kMode == RasterMode::Shadow ? sh.map : a.view0.depth,
                                         a.view0.idkey, sh.L, my_begin, s, stage_end, c0, short_bits, ent, nent);
@@ -870,9 +886,13 @@ __device__ __forceinline__ void local_sweep(ShadowStage& W, const RasterTri& rt,
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const uint32_t my_begin = (uint32_t)lane < nloc ? W.begin[lane] : total;
   const uint32_t my_end = (uint32_t)lane + 1 < nloc ? W.begin[lane + 1] : total;
+  // the box-form flag of staged triangle p to lane p (lanes without a
+  // piece push theirs to distinct lanes >= nloc)
+  const uint32_t to = n ? pos : nloc + ((uint32_t)lane - pos);
+  const int box = __builtin_amdgcn_ds_permute((int)(to * 4u), n > 0 && (rt.key & kKeyBoxForm) != 0u ? 1 : 0);
   const unsigned long long short_bits =
       __ballot((uint32_t)lane < nloc &&
-               short_form_ok<kOpSizeLight>(my_end - my_begin, W.min_y[lane], W.w[lane], W.iw[lane]));
+               (box != 0 || short_form_ok<kOpSizeLight>(my_end - my_begin, W.min_y[lane], W.w[lane], W.iw[lane])));
   uint32_t nent_unused = 0;
   sweep_samples<RasterMode::Shadow, kOpSizeLight, false>(W, 0u, sh.map, nullptr, sh.L, my_begin, 0u, total, 0u,
                                                          short_bits, nullptr, nent_unused);
@@ -987,7 +1007,7 @@ __global__ void __launch_bounds__(kShadowPreThreads) k_prearrange_shadow(const T
 #pragma unroll
             for (int i = 0; i < 2; i++) {
               if (!(i == 0 ? big0 : big1)) continue;
-              rt[i].key = key;
+              rt[i].key = key | (rt[i].key & kKeyBoxForm);
               rt[i].sample_begin = smp;
               lists.rast[c] = rt[i];
               smp += ns[i];

@@ -212,7 +212,7 @@ __device__ __forceinline__ uint32_t stage_chunk(TileStage& S, const Lists& Ls, u
       S.pc[lane] = t.C;
 #pragma unroll
       for (int k = 0; k < 7; k++) S.ek[k][lane] = t.ek[k];
-      S.key[lane] = t.key;
+      S.key[lane] = t.key & ~kKeyBoxForm;
       S.rec[lane] = c;
     }
     if (ns == 0) {

@@ -97,12 +97,19 @@ __device__ __forceinline__ bool sample_xy(float slf, float min_x, float min_y, f
 //   w (m + k) > 2^24 - 1, and m + k <= max_y + rows).
 // Others are checked boundary by boundary up to 64 rows; taller ones count
 // as not certified.  Checked against the reference walk on the CPU
-// (tests/tile_walk_check.c).
-__device__ __forceinline__ bool walk_is_rect(float min_y, float max_y, float w, float iw) {
+// (tests/tile_walk_check.c).  walk_is_rect_closed: the two bounds alone
+// (a sufficient condition, no loop), which the raster passes' box-form
+// records use.
+__device__ __forceinline__ bool walk_is_rect_closed(float min_y, float max_y, float w, float iw) {
   const float rows = max_y - min_y;
   const bool no_lag = fmaf(w, iw, -1.f) >= 0.f || rows < min_y;
   const bool no_lead = w * (max_y + rows) <= 16777215.f;
-  if (no_lag && no_lead) return true;
+  return no_lag && no_lead;
+}
+
+__device__ __forceinline__ bool walk_is_rect(float min_y, float max_y, float w, float iw) {
+  const float rows = max_y - min_y;
+  if (walk_is_rect_closed(min_y, max_y, w, iw)) return true;
   if (rows > 64.f) return false;
   bool ok = true;
 #pragma clang loop unroll(disable)
