@@ -262,14 +262,14 @@ int  ocr_frame_set_profiling(ocr_frame* f, int enable);
 
 /* engine::draw_bulk_objs_n -> render_tris (engine.cpp:1794-2161):
  * prearrange -> depth (kernel1) -> id (kernel2) -> shade (kernel3), with no
- * host round trip.  The view size must equal the frame's.  A frame holds two
- * output sets and each render writes the set the previous one did not, on a
- * stream of the frame's own, so render N+1 can start while render N shades;
- * `stream` waits for the render's end.  Render N+1 waits for the work on
- * `stream` enqueued before render N was issued: enqueue what reads a
- * render's outputs (ocr_frame_device_buffers changes with every render) on
- * `stream` before the next ocr_render on the frame, and use one stream for
- * the renders of one frame.  A frame serves either ocr_render or batches:
+ * host round trip.  The view size must equal the frame's.  A frame holds
+ * three output sets and each render writes the one written longest ago, on
+ * a stream of the frame's own, so render N+1 can start while renders N and
+ * N-1 are still running; `stream` waits for the render's end.  Render N+1
+ * waits for the work on `stream` enqueued before render N-1 was issued:
+ * enqueue what reads a render's outputs (ocr_frame_device_buffers changes
+ * with every render) on `stream` before the next ocr_render on the frame,
+ * and use one stream for the renders of one frame.  A frame serves either ocr_render or batches:
  * a frame already set on a batch is refused (OCR_ERR_INVALID). */
 int  ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream);
 /* No counterpart in the reference: ocr_render for callers that cannot

@@ -396,7 +396,8 @@ struct Frame {
     // covered-sample entries of the sample-space depth pass
     CoverEntries ent{};
   };
-  static constexpr int kSets = 2;   // three measured slower: renders compete for VALU
+  // three renders in flight: 1080p/1M frame 0.186 -> 0.183 ms (four: 0.197)
+  static constexpr int kSets = 3;
   OutBuf out[kSets];
   int cur = 0;
   uint32_t* depth = nullptr;

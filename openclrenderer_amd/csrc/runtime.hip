@@ -967,8 +967,8 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   if (!s->lights && s->nlights) return fail(OCR_ERR_INVALID, "ocr_render: lights missing");
   if (f->use == Frame::kBatch) return fail(OCR_ERR_INVALID, "ocr_render: the frame is rendered by batches");
   OCR_HIP_CHECK(hipSetDevice(s->ctx->device));
-  // this render's buffer set: the one the previous render did not use
-  const int prev = f->cur, b = (f->cur + 1) % Frame::kSets;
+  // this render's buffer set: the one the longest ago
+  const int b = (f->cur + 1) % Frame::kSets;
   const bool tiled = f->raster_mode == OCR_RASTER_TILES;
   int rc = frame_alloc_set(f, b);
   if (!rc) rc = ensure_frame_list(f, b, s->ntri);
@@ -988,11 +988,13 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
   hipStream_t sb = f->ss[b];
   // Render N runs on the stream of its set once render N-kSets (same
   // buffers, same stream) and everything enqueued on the caller's stream
-  // before render N-1 was issued have completed (the event recorded at that
-  // point), so it can overlap the renders before it; the caller's stream
-  // then waits for it.  The first render follows the caller's stream.
+  // before render N-kSets+1 was issued (the event recorded at that point,
+  // which follows render N-kSets) have completed, so it can overlap the
+  // kSets-1 renders before it; the caller's stream then waits for it.  The
+  // first renders follow the caller's stream.
+  const int oldest = (b + 1) % Frame::kSets;
   OCR_HIP_CHECK(hipEventRecord(f->entry[b], st));
-  OCR_HIP_CHECK(hipStreamWaitEvent(sb, f->entry_valid[prev] ? f->entry[prev] : f->entry[b], 0));
+  OCR_HIP_CHECK(hipStreamWaitEvent(sb, f->entry_valid[oldest] ? f->entry[oldest] : f->entry[b], 0));
   // and the scene's last stream-ordered descriptor update; the render reads
   // the current geometry version, registered as its reader below
   std::lock_guard<std::mutex> lk(s->ver_mu);
