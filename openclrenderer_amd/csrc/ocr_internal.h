@@ -227,6 +227,17 @@ enum Counter : int {
   kBinTotal = 48,          // tiled path: bin entries the render needed, byte 192
   kCounterWords = 64,
 };
+// Shadow-pass lists are split into kShadowParts partitions (one per lane of
+// the raster pass's wavefronts), each with its own record range, sample
+// space, packed counter and valid-prefix word on a 128-byte line of its own
+// after the render counters: prearrange block b allocates in partition
+// b mod kShadowParts.  (One counter for the whole pass serialised the
+// prearrange's wavefront allocations on one address.)
+constexpr uint32_t kShadowParts = 64;
+constexpr int kPartLineWords = 32;
+constexpr int kShadowCounterWords = kCounterWords + (int)kShadowParts * kPartLineWords;
+// partition p: packed counter (u64) at word kPartWord(p), valid prefix at +2
+__host__ __device__ constexpr int kPartWord(uint32_t p) { return kCounterWords + (int)p * kPartLineWords; }
 // Bits of the overflow word: the cut list is full / the sample space is full.
 constexpr uint32_t kOverflowCuts = 1u, kOverflowSamples = 2u, kOverflowBins = 4u;
 // Raster samples per render, and per cut triangle.  Frames and shadow faces

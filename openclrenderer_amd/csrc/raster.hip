@@ -613,91 +613,131 @@ __device__ __forceinline__ void sweep_samples(TriLds<kMode, kBatch>& L, uint32_t
   }
 }
 
+// The id pass over the covered-sample entries the depth pass kept: the
+// wavefront's chunks (wave0, wave0 + nwaves, ...) two at a time, their
+// entries as one list (at most kEntCap per chunk; a chunk with more is swept
+// by the caller), four of the lane's entries at a time: the entries, then
+// their records' depth planes and keys, then the depths there, then the
+// window tests.  Each entry's depth is formed again as the depth pass formed
+// it.  Two chunks per round halve the rounds of dependent loads a wavefront
+// waits through.
+template <bool kBatch>
+__device__ __forceinline__ void id_from_entries(const PassArgs& a, const Lists& Ls, uint32_t wave0, uint32_t nwaves,
+                                                uint32_t nchunks) {
+  const int lane = threadIdx.x & (kWave - 1);
+  constexpr int kPer = 4;
+  const uint4* R = reinterpret_cast<const uint4*>(Ls.rast);   // 4 x uint4 per record
+  for (uint32_t ca = wave0; ca < nchunks; ca += 2 * nwaves) {
+    const uint32_t cb = ca + nwaves;
+    const uint32_t na0 = Ls.ent.cnt[ca], nb0 = cb < nchunks ? Ls.ent.cnt[cb] : 0u;
+    const uint32_t na = na0 <= kEntCap ? na0 : 0u, nb = nb0 <= kEntCap ? nb0 : 0u;
+    const uint32_t n = na + nb;
+    const uint2* Ea = Ls.ent.ent + (size_t)ca * kEntCap;
+    // entry e >= na of the list is entry e - na of chunk cb
+    const uint2* Eb = Ls.ent.ent + (size_t)(cb < nchunks ? cb : ca) * kEntCap;
+    for (uint32_t e0 = 0; e0 < n; e0 += kPer * kWave) {
+      uint2 v[kPer];
+      uint32_t d[kPer], key[kPer], pix[kPer], val[kPer];
+      uint32_t* dep[kPer];
+      unsigned long long* ids[kPer];
+#pragma unroll
+      for (int i = 0; i < kPer; i++) {
+        const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
+        if (e < n) v[i] = e < na ? Ea[e] : Eb[e - na];
+      }
+#pragma unroll
+      for (int i = 0; i < kPer; i++) {
+        const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
+        dep[i] = a.view0.depth;
+        ids[i] = a.view0.idkey;
+        uint32_t pitch = a.view0.v.width;
+        if (e < n) {
+          // RasterTri words 1 (key) and 5-7 (the 1/z plane A, B, C)
+          const uint4 w0 = R[4 * (size_t)v[i].y], w1 = R[4 * (size_t)v[i].y + 1];
+          if constexpr (kBatch) {
+            // a batch's entries hold pixels of their record's view
+            const DevView& V = a.views[Ls.cuts[v[i].y].view];
+            dep[i] = V.depth;
+            ids[i] = V.idkey;
+            pitch = V.v.width;
+          }
+          const float x = (float)(v[i].x & 0xFFFFu), y = (float)(v[i].x >> 16);
+          d[i] = depth_quant(fmaf(__uint_as_float(w1.y), x, fmaf(__uint_as_float(w1.z), y, __uint_as_float(w1.w))));
+          key[i] = w0.y & ~kKeyBoxForm;
+          pix[i] = (v[i].x >> 16) * pitch + (v[i].x & 0xFFFFu);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kPer; i++) {
+        const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
+        if (e < n) val[i] = dep[i][pix[i]];
+      }
+#pragma unroll
+      for (int i = 0; i < kPer; i++) {
+        const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
+        if (e < n && d[i] > val[i] - kBufError && d[i] < val[i] + kBufError)
+          atomicMax(&ids[i][pix[i]], ((unsigned long long)(key[i] + 1u) << 32) | v[i].y);
+      }
+    }
+  }
+}
+
 template <RasterMode kMode, int kOp, bool kBatch>
 __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
   __shared__ TriLds<kMode, kBatch> lds_all[256 / kWave];
   const Lists& Ls = a.lists;
   if (Ls.counters[kOverflow]) return;
-  unsigned long long packed = *reinterpret_cast<const unsigned long long*>(Ls.counters + kPacked);
-  uint32_t ncut, total;
+  const int lane = threadIdx.x & (kWave - 1);
+  // the list's records and samples; shadow lists are kShadowParts partitions
+  // (lane p holds partition p's records, samples and inclusive chunk count)
+  uint32_t ncut = 0, total = 0, nchunks;
+  uint32_t part_ncut = 0, part_total = 0, part_chunks = 0;
   if constexpr (kMode == RasterMode::Shadow) {
     // (records << 40 | samples), up to the valid prefix
-    packed = min(packed, *reinterpret_cast<const unsigned long long*>(Ls.counters + kValidEnd));
-    ncut = (uint32_t)(packed >> kShadowSampleBits);
-    total = (uint32_t)(packed & kShadowSampleMask);
+    const unsigned long long* pc = reinterpret_cast<const unsigned long long*>(Ls.counters + kPartWord(lane));
+    const unsigned long long packed = min(pc[0], pc[1]);
+    part_ncut = (uint32_t)(packed >> kShadowSampleBits);
+    part_total = (uint32_t)(packed & kShadowSampleMask);   // at most 2^30
+    part_chunks = wave_incl_scan((part_total + kChunk - 1) / kChunk);
+    nchunks = __shfl(part_chunks, kWave - 1, kWave);
   } else {
+    const unsigned long long packed = *reinterpret_cast<const unsigned long long*>(Ls.counters + kPacked);
     ncut = (uint32_t)(packed >> 32);
     total = (uint32_t)packed;
+    nchunks = (total + kChunk - 1) / kChunk;
   }
-  const int lane = threadIdx.x & (kWave - 1);
   TriLds<kMode, kBatch>& L = lds_all[threadIdx.x / kWave];
   const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
   L.start[lane] = 0xFFFFFFFFu;   // never a sample index (total < 2^31)
-  for (uint32_t s0 = wave0 * kChunk; s0 < total; s0 += nwaves * kChunk) {
+  if constexpr (kMode == RasterMode::Id) {
+    if (Ls.ent.cnt) id_from_entries<kBatch>(a, Ls, wave0, nwaves, min(Ls.ent.chunks, nchunks));
+  }
+  for (uint32_t chunk = wave0; chunk < nchunks; chunk += nwaves) {
+    const RasterTri* rast = Ls.rast;
+    uint32_t s0 = chunk * kChunk;
+    if constexpr (kMode == RasterMode::Shadow) {
+      // the chunk's partition: the first whose inclusive chunk count passes it
+      const uint32_t part = (uint32_t)__popcll(__ballot(part_chunks <= chunk));
+      const uint32_t before = part ? (uint32_t)__shfl(part_chunks, (int)part - 1, kWave) : 0u;
+      ncut = __shfl(part_ncut, (int)part, kWave);
+      total = __shfl(part_total, (int)part, kWave);
+      rast = Ls.rast + (size_t)part * (Ls.cap_cut / kShadowParts);
+      s0 = (chunk - before) * kChunk;
+      // sample indices are the partition's own, so the start flags of an
+      // earlier chunk (another partition) may hold this chunk's indices
+      L.start[lane] = 0xFFFFFFFFu;
+    }
     const uint32_t s1 = min(s0 + kChunk, total);
     // covered-sample entries of the chunk: written by the depth pass of a
     // single view, read by its id pass instead of a second sweep
-    const uint32_t chunk = s0 / kChunk;
     uint2* ent = nullptr;
     uint32_t nent = 0;
     if constexpr (kMode != RasterMode::Shadow) {
       if (Ls.ent.cnt && chunk < Ls.ent.chunks) {
         if constexpr (kMode == RasterMode::Id) {
-          const uint32_t n = Ls.ent.cnt[chunk];
-          if (n <= kEntCap) {
-            // four of the lane's entries at a time, then their records' depth
-            // planes and keys, then the depths there, then the window tests;
-            // each entry's depth is formed again as the depth pass formed it
-            constexpr int kPer = 4;
-            const uint2* E = Ls.ent.ent + (size_t)chunk * kEntCap;
-            const uint4* R = reinterpret_cast<const uint4*>(Ls.rast);   // 4 x uint4 per record
-            for (uint32_t e0 = 0; e0 < n; e0 += kPer * kWave) {
-              uint2 v[kPer];
-              uint32_t d[kPer], key[kPer], pix[kPer], val[kPer];
-              uint32_t* dep[kPer];
-              unsigned long long* ids[kPer];
-#pragma unroll
-              for (int i = 0; i < kPer; i++) {
-                const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
-                if (e < n) v[i] = E[e];
-              }
-#pragma unroll
-              for (int i = 0; i < kPer; i++) {
-                const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
-                dep[i] = a.view0.depth;
-                ids[i] = a.view0.idkey;
-                uint32_t pitch = a.view0.v.width;
-                if (e < n) {
-                  // RasterTri words 1 (key) and 5-7 (the 1/z plane A, B, C)
-                  const uint4 w0 = R[4 * (size_t)v[i].y], w1 = R[4 * (size_t)v[i].y + 1];
-                  if constexpr (kBatch) {
-                    // a batch's entries hold pixels of their record's view
-                    const DevView& V = a.views[Ls.cuts[v[i].y].view];
-                    dep[i] = V.depth;
-                    ids[i] = V.idkey;
-                    pitch = V.v.width;
-                  }
-                  const float x = (float)(v[i].x & 0xFFFFu), y = (float)(v[i].x >> 16);
-                  d[i] = depth_quant(fmaf(__uint_as_float(w1.y), x, fmaf(__uint_as_float(w1.z), y, __uint_as_float(w1.w))));
-                  key[i] = w0.y & ~kKeyBoxForm;
-                  pix[i] = (v[i].x >> 16) * pitch + (v[i].x & 0xFFFFu);
-                }
-              }
-#pragma unroll
-              for (int i = 0; i < kPer; i++) {
-                const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
-                if (e < n) val[i] = dep[i][pix[i]];
-              }
-#pragma unroll
-              for (int i = 0; i < kPer; i++) {
-                const uint32_t e = e0 + (uint32_t)lane + (uint32_t)(i * kWave);
-                if (e < n && d[i] > val[i] - kBufError && d[i] < val[i] + kBufError)
-                  atomicMax(&ids[i][pix[i]], ((unsigned long long)(key[i] + 1u) << 32) | v[i].y);
-              }
-            }
-            continue;   // else: more covered samples than the entries hold, swept below
-          }
+          // chunks whose entries the depth pass kept were done above
+          if (Ls.ent.cnt[chunk] <= kEntCap) continue;
         } else {
           ent = Ls.ent.ent + (size_t)chunk * kEntCap;
         }
@@ -708,7 +748,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
     uint32_t lo = 0, hi = ncut;
     while (hi - lo > 1) {
       const uint32_t p = lo + (uint32_t)(((unsigned long long)(hi - lo) * (uint32_t)lane) / kWave);
-      const unsigned long long m = __ballot(Ls.rast[p].sample_begin <= s0);
+      const unsigned long long m = __ballot(rast[p].sample_begin <= s0);
       const int t = 63 - __clzll((long long)m);
       const uint32_t plo = __shfl(p, t, kWave);
       const uint32_t phi = __shfl(p, t < kWave - 1 ? t + 1 : t, kWave);
@@ -724,7 +764,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
       float t_min_y = 0.f, t_w = 0.f, t_iw = 0.f;
       bool t_box = false;
       if (ci < ncut) {
-        const RasterTri t = Ls.rast[ci];
+        const RasterTri t = rast[ci];
         my_begin = t.sample_begin;
         t_min_y = t.min_y;
         t_w = t.w;
@@ -747,7 +787,7 @@ __global__ void __launch_bounds__(256) k_raster(PassArgs a, ShadowTarget sh) {
           L.pitch[lane] = V.v.width;
           if constexpr (kMode == RasterMode::Id) L.ids[lane] = V.idkey;
         }
-        if (lane == kWave - 1) L.begin[kWave] = ci + 1 < ncut ? Ls.rast[ci + 1].sample_begin : total;
+        if (lane == kWave - 1) L.begin[kWave] = ci + 1 < ncut ? rast[ci + 1].sample_begin : total;
       } else {
         L.begin[lane] = total;
         if (lane == kWave - 1) L.begin[kWave] = total;
@@ -822,11 +862,14 @@ struct ShadowGroup {
 };
 
 // Shadow-pass allocation for the lanes with `take` set, per wavefront: one
-// atomicAdd on the packed counter.  An allocation that passes the list
-// capacity or kShadowMaxSamples is not written: the wavefront rasterises
-// those pieces itself and lowers the valid-prefix word to its start, and
-// every later allocation fails too (the counter only grows), so the raster
-// launch reads the list up to min(counter, valid prefix).
+// atomicAdd on the packed counter of the block's partition (records
+// [part * cap, part * cap + cap) of the list, cap = cap_cut / kShadowParts,
+// and a sample space of its own).  An allocation that passes the
+// partition's capacity or kShadowMaxSamples is not written: the wavefront
+// rasterises those pieces itself and lowers the partition's valid-prefix
+// word to its start, and every later allocation in the partition fails too
+// (the counter only grows), so the raster launch reads each partition up to
+// min(counter, valid prefix).
 __device__ __forceinline__ bool wave_alloc_fit(const Lists& L, uint32_t ncut, uint32_t nsmp, uint32_t& cut0,
                                                uint32_t& smp0) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -834,20 +877,21 @@ __device__ __forceinline__ bool wave_alloc_fit(const Lists& L, uint32_t ncut, ui
   const uint32_t tc = __shfl(ic, kWave - 1, kWave), ts = __shfl(is, kWave - 1, kWave);
   cut0 = smp0 = 0;
   if (tc == 0) return true;
+  const uint32_t part = blockIdx.x % kShadowParts, cap = L.cap_cut / kShadowParts;
   uint32_t base_lo = 0, base_hi = 0, ok = 0;
   if (lane == 0) {
-    unsigned long long* p = reinterpret_cast<unsigned long long*>(L.counters + kPacked);
+    unsigned long long* p = reinterpret_cast<unsigned long long*>(L.counters + kPartWord(part));
     const unsigned long long seen = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((seen >> kShadowSampleBits) < L.cap_cut && (seen & kShadowSampleMask) < kShadowMaxSamples) {
+    if ((seen >> kShadowSampleBits) < cap && (seen & kShadowSampleMask) < kShadowMaxSamples) {
       const unsigned long long old = atomicAdd(p, ((unsigned long long)tc << kShadowSampleBits) + ts);
-      ok = (old >> kShadowSampleBits) + tc <= L.cap_cut && (old & kShadowSampleMask) + ts <= kShadowMaxSamples;
-      if (!ok) atomicMin(reinterpret_cast<unsigned long long*>(L.counters + kValidEnd), old);
+      ok = (old >> kShadowSampleBits) + tc <= cap && (old & kShadowSampleMask) + ts <= kShadowMaxSamples;
+      if (!ok) atomicMin(p + 1, old);   // the valid-prefix word, same line
       base_lo = (uint32_t)(old & kShadowSampleMask);   // below 2^30 when ok
       base_hi = (uint32_t)(old >> kShadowSampleBits);
     }
   }
   ok = __shfl(ok, 0, kWave);
-  cut0 = __shfl(base_hi, 0, kWave) + ic - ncut;
+  cut0 = part * cap + __shfl(base_hi, 0, kWave) + ic - ncut;
   smp0 = __shfl(base_lo, 0, kWave) + is - nsmp;
   return ok != 0;
 }
@@ -1030,13 +1074,22 @@ __global__ void k_zero_counters(uint32_t* c) {
     c[threadIdx.x] = threadIdx.x == kValidEnd || threadIdx.x == kValidEnd + 1 ? 0xFFFFFFFFu : 0u;
 }
 
+// A shadow pass's counters: the render words and every partition's packed
+// counter (0) and valid prefix (all ones).
+__global__ void __launch_bounds__(256) k_zero_shadow_counters(uint32_t* c) {
+  for (uint32_t i = threadIdx.x; i < (uint32_t)kShadowCounterWords; i += blockDim.x) {
+    const uint32_t w = i < (uint32_t)kCounterWords ? 0u : (i - kCounterWords) % kPartLineWords;
+    c[i] = i >= (uint32_t)kCounterWords && (w == 2u || w == 3u) ? 0xFFFFFFFFu : 0u;
+  }
+}
+
 hipError_t launch_zero_counters(uint32_t* counters, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, counters);
   return hipGetLastError();
 }
 
 // Tuning of the shadow launches, read at every pass: pieces of at most
-// OCR_SHADOW_LOCAL_SAMPLES samples (default 32) are rasterised by the
+// OCR_SHADOW_LOCAL_SAMPLES samples (default 16) are rasterised by the
 // prearrange launch itself, larger ones go to the list for the chip-wide
 // raster launch; OCR_SHADOW_LIST_RECORDS lowers the list's capacity (the
 // pieces that do not fit are rasterised by the prearrange launch too).
@@ -1048,7 +1101,7 @@ static uint32_t env_u32(const char* name, uint32_t dflt) {
   const unsigned long v = strtoul(e, &end, 10);
   return end && *end == 0 && v <= 0xFFFFFFFFul ? (uint32_t)v : dflt;
 }
-constexpr uint32_t kShadowLocalSamples = 32;
+constexpr uint32_t kShadowLocalSamples = 16;
 
 hipError_t launch_shadow_group(Scene* sc, const float (*lpos)[3], const uint32_t* slots, uint32_t n,
                                int only_static, uint32_t* maps, hipStream_t s) {
@@ -1066,7 +1119,7 @@ hipError_t launch_shadow_group(Scene* sc, const float (*lpos)[3], const uint32_t
   Lists lists{nullptr, sc->sh_rast, sc->sh_counters,
               std::min(sc->sh_cap_cut, env_u32("OCR_SHADOW_LIST_RECORDS", sc->sh_cap_cut)), 0u};
   const ShadowTarget target{maps, sc->lbuf_dim};
-  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, sc->sh_counters);
+  hipLaunchKernelGGL(k_zero_shadow_counters, dim3(1), dim3(256), 0, s, sc->sh_counters);
   if (sc->ntri)
     hipLaunchKernelGGL(k_prearrange_shadow, dim3((sc->ntri + kShadowPreThreads - 1) / kShadowPreThreads),
                        dim3(kShadowPreThreads), 0, s, sc->world, sc->ntri, sc->objs, sc->nobj, g, target,

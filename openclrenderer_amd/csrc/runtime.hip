@@ -362,7 +362,7 @@ int ocr_scene_create(ocr_ctx* ctx, const ocr_triangle* tris, uint32_t ntri, cons
   if ((rc = dev_alloc(&s->tri_pos, ntri, "triangles")) || (rc = dev_alloc(&s->world, ntri, "world-space triangles")) ||
       (rc = dev_alloc(&s->tri_attr, ntri, "attributes")) ||
       (rc = dev_alloc(&s->objs, nobj, "descriptors")) || (rc = dev_alloc(&s->motion, (size_t)nobj * 16, "motion poses")) ||
-      (rc = dev_alloc(&s->sh_counters, kCounterWords, "counters"))) {
+      (rc = dev_alloc(&s->sh_counters, kShadowCounterWords, "counters"))) {
     ocr_scene_destroy(s);
     return rc;
   }

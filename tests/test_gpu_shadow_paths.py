@@ -6,9 +6,13 @@ cl2.cl:4419-4636 and 5129-5246), bit for bit:
   the rest rasterised by the prearrange launch's wavefronts;
 * everything listed (threshold 0) or everything local (threshold above any
   piece);
-* a list too small for the listed pieces: allocations past its capacity fail,
-  their wavefronts rasterise those pieces themselves and the raster launch
-  reads only the valid prefix (capacities 0, 1, 16 and 1000 records);
+* a list too small for the listed pieces: allocations past a partition's
+  capacity fail, their wavefronts rasterise those pieces themselves and the
+  raster launch reads only each partition's valid prefix (the list is split
+  into 64 partitions; capacities 0, 1, 16, 1000 and 6400 records);
+* the raster launch over 64 partitions whose chunks a wavefront takes in
+  turn (sample indices restart in every partition), on the config-3
+  generator at several thresholds;
 * more shadow-casting lights than one launch takes (18 lights: two groups),
   with a static light among them.
 """
@@ -35,8 +39,8 @@ def near_maps(near_scene, oracle):
     return oracle.shadow_maps(near_scene[0], 16)
 
 
-@pytest.mark.parametrize("local,cap", [("32", None), ("0", None), ("4000000000", None), ("0", "0"), ("0", "1"),
-                                       ("0", "16"), ("8", "1000")])
+@pytest.mark.parametrize("local,cap", [("16", None), ("32", None), ("0", None), ("4000000000", None), ("0", "0"),
+                                       ("0", "1"), ("0", "16"), ("8", "1000"), ("4", "6400")])
 def test_shadow_maps_every_path(gpu_ctx, near_scene, near_maps, monkeypatch, local, cap):
     sc, _ = near_scene
     monkeypatch.setenv("OCR_SHADOW_LOCAL_SAMPLES", local)
@@ -52,6 +56,23 @@ def test_shadow_maps_every_path(gpu_ctx, near_scene, near_maps, monkeypatch, loc
     odyn, _ = near_maps
     assert np.array_equal(dyn, odyn), f"{int((dyn != odyn).sum())} texels differ"
     assert (odyn != UINT_MAX).sum() > 100000
+
+
+@pytest.fixture(scope="module")
+def cfg3_small(oracle):
+    sc, view = scenes.config3(n_inst=100, target_tris=200_000, width=960, height=540)
+    return sc, oracle.shadow_maps(sc, 16)[0]
+
+
+@pytest.mark.parametrize("local", ["8", "32", "64"])
+def test_config3_partitions(gpu_ctx, cfg3_small, monkeypatch, local):
+    sc, odyn = cfg3_small
+    monkeypatch.setenv("OCR_SHADOW_LOCAL_SAMPLES", local)
+    print(f"config-3 generator, 200k triangles, local {local}", flush=True)
+    g = runtime.GpuScene(gpu_ctx, sc)
+    g.shadow_pass()
+    dyn, _ = g.read_shadow_maps()
+    assert np.array_equal(dyn, odyn), f"{int((dyn != odyn).sum())} texels differ"
 
 
 def test_eighteen_lights_two_groups(gpu_ctx, oracle):
