@@ -257,7 +257,9 @@ int  ocr_frame_get_raster(ocr_frame* f, ocr_raster_config* cfg);
 /* Record hipEvent timings on every render of this frame (up to 1024
  * renders are averaged; enabling again restarts the average): enable 1
  * times every pass, 2 only the shading pass (two events per render, for
- * timing a stream of renders with as little added as possible), 0 stops. */
+ * timing a stream of renders with as little added as possible), 0 stops.
+ * The shading pass is timed from the end of its wait for the scene's
+ * shadow maps. */
 int  ocr_frame_set_profiling(ocr_frame* f, int enable);
 
 /* engine::draw_bulk_objs_n -> render_tris (engine.cpp:1794-2161):

@@ -426,7 +426,7 @@ struct Frame {
   uint32_t raster_mode = OCR_RASTER_SAMPLES, tw_log2 = 4, th_log2 = 4, want_bin = 0;
   uint32_t* post_in = nullptr;       // colour snapshot of the screen-space post passes
   int profiling = 0;   // 0 off, 1 every pass, 2 the shading pass only
-  static constexpr int kEvents = 6;       // prearrange, clear, depth, id, shade boundaries
+  static constexpr int kEvents = 7;       // pass boundaries, and the shading pass's start
   static constexpr int kRing = 1024;      // renders averaged per profiling window
   std::vector<hipEvent_t> ev;             // kRing x kEvents, created on first use
   int n_profiled = 0;

@@ -1018,7 +1018,7 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
     if (prof) OCR_HIP_CHECK(hipEventRecord(ev[2], sb));
     OCR_HIP_CHECK(launch_tile_raster(pa, sb));
     if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], sb));
-    if (prof_shade) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
   } else {
     OCR_HIP_CHECK(launch_zero_counters(o.counters, sb));
     if (prof) OCR_HIP_CHECK(hipEventRecord(ev[0], sb));
@@ -1029,11 +1029,13 @@ int ocr_render(ocr_scene* s, const ocr_view* view, ocr_frame* f, void* stream) {
     OCR_HIP_CHECK(launch_depth(pa, kRasterBlocksPerCu, sb));
     if (prof) OCR_HIP_CHECK(hipEventRecord(ev[3], sb));
     OCR_HIP_CHECK(launch_id(pa, kRasterBlocksPerCu, sb));
-    if (prof_shade) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
+    if (prof) OCR_HIP_CHECK(hipEventRecord(ev[4], sb));
   }
   // Shading reads the shadow maps: it waits for the scene's last shadow
   // pass, wherever that was enqueued (the raster passes do not read them).
+  // The shading pass is timed from after that wait.
   OCR_HIP_CHECK(s->wait_shadow_maps(sb));
+  if (prof_shade) OCR_HIP_CHECK(hipEventRecord(ev[6], sb));
   OCR_HIP_CHECK(launch_shade(pa, s->ctx->face_sc, sb));
   OCR_HIP_CHECK(s->note_read(sb, true));
   if (prof_shade) OCR_HIP_CHECK(hipEventRecord(ev[5], sb));
@@ -1112,9 +1114,10 @@ int ocr_frame_get_stats(ocr_frame* f, void* stream, ocr_frame_stats* out) {
   out->overflow = c[kOverflow] ? 1u : 0u;
   out->n_profiled = (uint32_t)f->n_profiled;
   if (f->n_profiled > 0) {
-    // clear, prearrange, depth, id, shade, and the render's whole span
+    // clear, prearrange, depth, id, shade (from the end of its wait for the
+    // shadow maps), and the render's whole span
     double sum[6] = {0, 0, 0, 0, 0, 0};
-    const int from[6] = {1, 0, 2, 3, 4, 0}, to[6] = {2, 1, 3, 4, 5, 5};
+    const int from[6] = {1, 0, 2, 3, 6, 0}, to[6] = {2, 1, 3, 4, 5, 5};
     for (int r = 0; r < f->n_profiled; r++) {
       hipEvent_t* ev = &f->ev[(size_t)r * Frame::kEvents];
       float ms = 0.f;
