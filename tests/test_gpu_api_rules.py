@@ -135,3 +135,31 @@ def test_server_frame_pool_within_budget(gpu_ctx):
         srv.release(rid)
     assert srv.info()["n_evicted"] > 0
     srv.close()
+
+
+def test_profiling_modes(gpu_ctx):
+    """ocr_frame_set_profiling: 1 times every pass of every render, 2 the
+    shading pass of every render; other values are refused."""
+    sc, view = scenes.random_scene(77, n_tris=200)
+    g = runtime.GpuScene(gpu_ctx, sc)
+    fr = runtime.Frame(gpu_ctx, int(view["width"][0]), int(view["height"][0]))
+    ref = fr.render_read(g, view)
+    for mode, n_renders in ((1, 5), (2, 7)):
+        fr.set_profiling(True, shade_only=mode == 2)
+        for _ in range(n_renders):
+            fr.render(g, view)
+        st = fr.stats()
+        print(f"profiling mode {mode}: {st['n_profiled']} renders timed", flush=True)
+        assert st["n_profiled"] == n_renders
+        assert st["ms_shade"] > 0.0
+        if mode == 1:
+            assert st["ms_depth"] > 0.0 and st["ms_prearrange"] > 0.0 and st["ms_total"] >= st["ms_shade"]
+        else:
+            assert st["ms_depth"] == 0.0 and st["ms_prearrange"] == 0.0
+    fr.set_profiling(False)
+    out = fr.read()
+    assert np.array_equal(out["depth"], ref["depth"]) and np.array_equal(out["rgba"], ref["rgba"])
+    for bad in (3, -1):
+        with pytest.raises(runtime.OcrError) as e:
+            runtime.check(fr.lib.ocr_frame_set_profiling(fr.h, bad))
+        assert e.value.code == abi.OCR_ERR_INVALID
