@@ -248,11 +248,11 @@ constexpr uint32_t kMaxSamples = 0x7FFFFFFFu;
 constexpr uint32_t kMaxTriSamples = (1u << 24) - 1;
 constexpr uint64_t kMaxTargetPixels = 16000000ull;
 
-// Shadow-pass lists pack their counter as (records << 40 | samples): a
-// wavefront allocates at most 2^25 samples at once (a lane's two pieces
-// hold fewer than 2^25) and only if the counter it loads first is below
-// the limits, and at most 8192 wavefronts are resident, so the sample part
-// stays below 2^30 + 2^13 * 2^25 < 2^40 and the record part below
+// Shadow-pass lists pack each partition's counter as (records << 40 |
+// samples): a wavefront allocates at most 2^25 samples at once (a lane's two
+// pieces hold fewer than 2^25) and only if the counter it loads first is
+// below the limits, and at most 8192 wavefronts are resident, so the sample
+// part stays below 2^30 + 2^13 * 2^25 < 2^40 and the record part below
 // kShadowMaxRecords + 2^13 * 128 < 2^24.
 constexpr int kShadowSampleBits = 40;
 constexpr unsigned long long kShadowSampleMask = (1ull << kShadowSampleBits) - 1;
