@@ -316,11 +316,16 @@ def test_barycentric_closed_forms(oracle):
 
 
 def test_encode_normal_round_trips_through_decode(oracle):
-    """decode_normal (cl2.cl:5632-5646) inverts encode_normal."""
+    """decode_normal (cl2.cl:5632-5646) inverts encode_normal.  Near the
+    poles the 16-bit code of z = (vx^2 + vy^2) * 2 - 1 loses x and y (their
+    error grows like 1/hypot(x, y)), so normals within 0.1 of the z axis
+    are left out; the draw has its own seed, so the cases do not depend on
+    which tests ran before in the process."""
+    rng = np.random.default_rng(5632)
     for _ in range(500):
-        n = RNG.normal(size=3)
+        n = rng.normal(size=3)
         n /= np.linalg.norm(n)
-        if abs(n[0]) + abs(n[1]) < 0.05:
+        if math.hypot(n[0], n[1]) < 0.1:
             continue
         na, npp = fp(n)
         e = oracle.lib.oracle_encode_normal(npp)
